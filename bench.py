@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X space-time multigrid slab solve.
+
+Metric: space-time DoF/s per slab solve to 1e-8 relative residual with the
+FP32 multigrid preconditioner inside FP64 FGMRES (value), next to the plain
+FP64 solve, plus the FP64 space-time vmult in HBM GB/s.
+
+Workload (default): 3D heat on the unit cube, 32^3 cells with vertices
+perturbed by up to 0.2h (seeded), Q3 space, dG(2) time (3 time dofs per
+slab, 2.74M space-time dofs per slab), T = 1 in 8 slabs, log-uniform
+cell coefficient in [1e-2, 1e2] (seeded), seeded Gaussian-bump source.
+A step is one slab solve (rhs assembly, extrapolation, FGMRES + GMG).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dealii-stfem_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def box_vertices(dim, nc, perturb, seed):
+    import numpy as np
+    n = [c + 1 for c in nc]
+    axes = [np.linspace(0.0, 1.0, n[d]) for d in range(dim)]
+    grids = np.meshgrid(*axes[::-1], indexing="ij")[::-1]
+    X = np.stack([g.reshape(-1) for g in grids], axis=1)
+    rng = np.random.default_rng(seed)
+    idx = np.indices(n[::-1]).reshape(dim, -1)[::-1]
+    interior = np.ones(X.shape[0], dtype=bool)
+    for d in range(dim):
+        interior &= (idx[d] > 0) & (idx[d] < n[d] - 1)
+    for d in range(dim):
+        X[interior, d] += perturb * (1.0 / nc[d]) * rng.uniform(-1, 1, interior.sum())
+    return np.ascontiguousarray(X)
+
+
+def workload(refine):
+    """Spec, vertices and coefficient of the 3D heat workload on 2^refine cells per side."""
+    import numpy as np
+    dim = 3
+    nc = [1 << refine] * dim
+    X = box_vertices(dim, nc, 0.2, seed=2024)
+    rng = np.random.default_rng(2025)
+    kappa = np.exp(rng.uniform(np.log(1e-2), np.log(1e2), int(np.prod(nc))))
+    spec = dict(dim=3, refine=refine, p=3, type=2, r=2, nts=1, r_min=0, tau=0.125,
+                end_time=1.0, source=3, n_bumps=4, seed=7, reltol=1e-8, abstol=1e-300,
+                maxiter=200, basis=30, vanka_factored=1)
+    return spec, X, kappa
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_gpu(spec, X, kappa, steps, warmup):
+    import stfem_amd as st
+    t0 = time.perf_counter()
+    g = st.Problem(spec, vertices=X, kappa=kappa)
+    setup_s = time.perf_counter() - t0
+    log(f"setup mg_float={spec.get('mg_float')} {setup_s:.1f} s, {g.n_levels} levels, "
+        f"{g.n_nodes * g.nt} space-time dofs")
+    if warmup > 0:
+        g.time_slabs(warmup)
+    ms, its = g.time_slabs(steps)
+    log(f"timed {steps} slabs: {ms:.1f} ms, {its} FGMRES iterations")
+    return g, setup_s, ms, its
+
+
+def cpu_baseline(budget_s=25.0):
+    """Oracle (FP64 CPU restatement, single thread) on a bounded sample: the
+    same discretisation and solver on 8^3 cells, slab solves until the
+    budget is used."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as orc
+    spec, X, kappa = workload(3)
+    ospec = dict(dim=3, refine=3, p=3, type=2, r=2, nts=1, r_min=0, tau=0.125, reltol=1e-8,
+                 abstol=1e-300, rhs=1)
+    o = orc.Problem(ospec, coords=X, kappa=kappa)
+    info = o.level_info(o.n_levels - 1)
+    n, nt = info["n_nodes"], info["nt"]
+    # start from a nonzero state so every slab does the full solve
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(n * nt)
+    t0 = time.perf_counter()
+    slabs, its = 0, 0
+    tm = 0.0
+    while time.perf_counter() - t0 < budget_s and slabs < 8:
+        prev = x[(nt - 1) * n:].copy()
+        rc, x, k = orc.solve_slab(o, x, prev, tm)
+        its += k
+        slabs += 1
+        tm += 0.125
+    dt = time.perf_counter() - t0
+    return dict(value=slabs * n * nt / dt, unit="space-time DoF/s", cores=1, kind="port",
+                sample=f"oracle FP64 slab solves of the same Q3/dG(2) perturbed-mesh heat problem "
+                       f"on 8^3 cells ({n * nt} space-time dofs/slab), {slabs} slabs, {its} "
+                       f"FGMRES its, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--refine", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(the multi-GPU decomposition is a design only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    torch.cuda.synchronize()
+    spec, X, kappa = workload(a.refine)
+
+    # FP32 multigrid inside FP64 FGMRES (headline)
+    s32 = dict(spec, mg_float=1)
+    g32, setup32, ms32, its32 = run_gpu(s32, X, kappa, a.steps, a.warmup)
+    nst = g32.n_nodes * g32.nt
+    dofs32 = a.steps * nst / (ms32 * 1e-3)
+
+    # kernels: FP64 fine vmult, FP32 finest-level vmult and Vanka
+    L = g32.n_levels - 1
+    vm64_ms = g32.time_kernel(0, warmup=3, reps=20)
+    vm64_bytes = g32.bytes_model(0)
+    vm32_ms = g32.time_kernel(1, L, warmup=3, reps=20)
+    nv = (g32.level_info(L)["nc0"] + 1) ** 3
+    vm32_bytes = 4.0 * 2 * nst + 4.0 * g32.n_cells + 8.0 * 3 * nv
+    vk_ms = g32.time_kernel(2, L, warmup=2, reps=10)
+    vk_bytes = g32.bytes_model(1, L)
+    g32.close()
+
+    dofs64, its64, ms64, setup64 = None, None, None, None
+    if not a.no_fp64:
+        s64 = dict(spec, mg_float=0)
+        g64, setup64, ms64, its64 = run_gpu(s64, X, kappa, a.steps, a.warmup)
+        dofs64 = a.steps * nst / (ms64 * 1e-3)
+        g64.close()
+
+    # dominant kernel of an FGMRES iteration on the finest level: two Vanka
+    # sweeps (pre/post), three level vmults and one FP64 vmult
+    if 2 * vk_ms >= 3 * vm32_ms + vm64_ms:
+        dom = dict(kernel="vanka_factored_fp32", ms=vk_ms, bytes=vk_bytes)
+    else:
+        dom = dict(kernel="st_vmult_fp64", ms=vm64_ms, bytes=vm64_bytes)
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+
+    cpu = None if a.no_cpu else cpu_baseline()
+    out = {
+        "metric": "space-time DoF/s per solve to 1e-8 (FP32-MG vs FP64) + vmult HBM GB/s",
+        "value": dofs32,
+        "unit": "space-time DoF/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms32 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (FGMRES) + f32 (multigrid)",
+        "data": "synthetic",
+        "config": {"workload": f"3D heat, {1 << a.refine}^3 cells perturbed 0.2h, Q3 space, "
+                               f"dG(2) time, log-uniform kappa [1e-2,1e2], Gaussian bumps, "
+                               f"tol 1e-8",
+                   "space_time_dofs_per_slab": nst, "n_levels": g32.n_levels,
+                   "vanka": "factored"},
+        "fp64_dofs_per_s": dofs64,
+        "speedup_fp32mg_vs_fp64": (dofs32 / dofs64) if dofs64 else None,
+        "fgmres_its_fp32mg": its32,
+        "fgmres_its_fp64": its64,
+        "setup_s": {"fp32mg": setup32, "fp64": setup64},
+        "vmult_fp64": {"ms": vm64_ms, "GBps": vm64_bytes / (vm64_ms * 1e-3) / 1e9,
+                       "hbm_frac": vm64_bytes / (vm64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "dofs_per_s": nst / (vm64_ms * 1e-3)},
+        "vmult_fp32": {"ms": vm32_ms, "GBps": vm32_bytes / (vm32_ms * 1e-3) / 1e9},
+        "vanka_fp32": {"ms": vk_ms, "GBps": vk_bytes / (vk_ms * 1e-3) / 1e9},
+        "roofline": {"kernel": dom["kernel"], "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,247 @@
+// Vector kernels of the Krylov and multigrid drivers (gfx950).
+//
+// Reductions are two-stage with a fixed grid so results are bitwise
+// reproducible run to run: stage 1 writes one partial sum per workgroup,
+// stage 2 (one workgroup) sums the partials in a fixed order and may post-
+// process (sqrt) and store to a device scalar, e.g. a Hessenberg entry, so
+// the Gram-Schmidt chain needs no host round trip.
+#include "kernels.h"
+
+#include <cmath>
+
+namespace stfem
+{
+  namespace
+  {
+    constexpr int RB = 1024; // stage-1 workgroups
+    constexpr int RT = 256;  // threads per workgroup
+
+    __device__ __forceinline__ double block_sum(double v)
+    {
+      __shared__ double red[RT / 64];
+      for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_down(v, off, 64);
+      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      if (lane == 0)
+        red[w] = v;
+      __syncthreads();
+      double s = 0.0;
+      if (threadIdx.x == 0)
+        for (int i = 0; i < RT / 64; ++i)
+          s += red[i];
+      return s;
+    }
+
+    template <typename T>
+    __global__ void __launch_bounds__(RT)
+      k_dot_partial(long n, const T *__restrict__ x, const T *__restrict__ y, double *partial)
+    {
+      double s = 0.0;
+      for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)RB * RT)
+        s += double(x[i]) * double(y[i]);
+      s = block_sum(s);
+      if (threadIdx.x == 0)
+        partial[blockIdx.x] = s;
+    }
+
+    __global__ void __launch_bounds__(RT) k_finish(const double *partial, double *out, int op)
+    {
+      double s = 0.0;
+      for (int i = threadIdx.x; i < RB; i += RT)
+        s += partial[i];
+      s = block_sum(s);
+      if (threadIdx.x == 0)
+        out[0] = (op == 1) ? sqrt(s) : s;
+    }
+
+    __global__ void __launch_bounds__(RT)
+      k_axpy_dot_partial(long n, double *__restrict__ x, const double *coef, double factor,
+                         const double *__restrict__ y, const double *__restrict__ z,
+                         double *partial)
+    {
+      const double c = (coef ? coef[0] : 1.0) * factor;
+      double       s = 0.0;
+      for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)RB * RT)
+        {
+          const double v = x[i] + c * y[i];
+          x[i]           = v;
+          s += v * (z ? z[i] : v);
+        }
+      s = block_sum(s);
+      if (threadIdx.x == 0)
+        partial[blockIdx.x] = s;
+    }
+
+    template <typename T>
+    __global__ void k_axpby(long n, T a, const T *__restrict__ x, T b, T *__restrict__ y)
+    {
+      const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (i < n)
+        y[i] = a * x[i] + (b == T(0) ? T(0) : b * y[i]);
+    }
+
+    struct MultiAxpy
+    {
+      const double *z[32];
+      double        c[32];
+      int           m;
+    };
+    __global__ void k_multi_axpy(long n, MultiAxpy A, double *__restrict__ x)
+    {
+      const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (i >= n)
+        return;
+      double s = x[i];
+      for (int j = 0; j < A.m; ++j)
+        s += A.c[j] * A.z[j][i];
+      x[i] = s;
+    }
+
+    template <typename TD, typename TS>
+    __global__ void k_convert(long n, const TS *__restrict__ s, TD *__restrict__ d)
+    {
+      const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (i < n)
+        d[i] = TD(s[i]);
+    }
+
+    template <typename T>
+    __global__ void k_b2i(long nn, int nt, const T *__restrict__ b, T *__restrict__ it)
+    {
+      const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (i >= nn * nt)
+        return;
+      const long node = i / nt;
+      const int  t    = int(i - node * nt);
+      it[i]           = b[(long)t * nn + node];
+    }
+    template <typename T>
+    __global__ void k_i2b(long nn, int nt, const T *__restrict__ it, T *__restrict__ b)
+    {
+      const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (i >= nn * nt)
+        return;
+      const long node = i / nt;
+      const int  t    = int(i - node * nt);
+      b[(long)t * nn + node] = it[i];
+    }
+
+    template <typename T>
+    __global__ void k_zero_dirichlet(GridDesc g, int nt, T *x)
+    {
+      const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (i >= g.n_nodes)
+        return;
+      const long gx = i % g.nn[0], r = i / g.nn[0];
+      const long gy = r % g.nn[1], gz = r / g.nn[1];
+      const int  d  = g.dirichlet;
+      bool       c  = (gx == 0 && (d & 1)) || (gx == g.nn[0] - 1 && (d & 2)) ||
+               (gy == 0 && (d & 4)) || (gy == g.nn[1] - 1 && (d & 8));
+      if (g.dim == 3)
+        c = c || (gz == 0 && (d & 16)) || (gz == g.nn[2] - 1 && (d & 32));
+      if (c)
+        for (int t = 0; t < nt; ++t)
+          x[i * nt + t] = T(0);
+    }
+
+    inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+  } // namespace
+
+  int reduce_slots() { return RB; }
+
+  template <typename T>
+  void launch_dot(long n, const T *x, const T *y, double *partial, double *out, int op,
+                  hipStream_t s)
+  {
+    k_dot_partial<T><<<RB, RT, 0, s>>>(n, x, y, partial);
+    k_finish<<<1, RT, 0, s>>>(partial, out, op);
+    STFEM_HIP(hipGetLastError());
+  }
+
+  void launch_axpy_dot(long n, double *x, const double *coef, double factor, const double *y,
+                       const double *z, double *partial, double *out, int op, hipStream_t s)
+  {
+    k_axpy_dot_partial<<<RB, RT, 0, s>>>(n, x, coef, factor, y, z, partial);
+    k_finish<<<1, RT, 0, s>>>(partial, out, op);
+    STFEM_HIP(hipGetLastError());
+  }
+
+  template <typename T>
+  void launch_axpby(long n, T a, const T *x, T b, T *y, hipStream_t s)
+  {
+    if (n == 0)
+      return;
+    k_axpby<T><<<nblk(n), 256, 0, s>>>(n, a, x, b, y);
+    STFEM_HIP(hipGetLastError());
+  }
+
+  void launch_multi_axpy(long n, int m, const double *const *z, const double *coef, double *x,
+                         hipStream_t s)
+  {
+    for (int j0 = 0; j0 < m; j0 += 32)
+      {
+        MultiAxpy A;
+        A.m = (m - j0 < 32) ? m - j0 : 32;
+        for (int j = 0; j < A.m; ++j)
+          {
+            A.z[j] = z[j0 + j];
+            A.c[j] = coef[j0 + j];
+          }
+        k_multi_axpy<<<nblk(n), 256, 0, s>>>(n, A, x);
+      }
+    STFEM_HIP(hipGetLastError());
+  }
+
+  template <typename TD, typename TS>
+  void launch_convert(long n, const TS *src, TD *dst, hipStream_t s)
+  {
+    if (n == 0)
+      return;
+    k_convert<TD, TS><<<nblk(n), 256, 0, s>>>(n, src, dst);
+    STFEM_HIP(hipGetLastError());
+  }
+
+  template <typename T>
+  void launch_blocks_to_interleaved(long nn, int nt, const T *b, T *it, hipStream_t s)
+  {
+    if (nn * nt == 0)
+      return;
+    k_b2i<T><<<nblk(nn * nt), 256, 0, s>>>(nn, nt, b, it);
+    STFEM_HIP(hipGetLastError());
+  }
+  template <typename T>
+  void launch_interleaved_to_blocks(long nn, int nt, const T *it, T *b, hipStream_t s)
+  {
+    if (nn * nt == 0)
+      return;
+    k_i2b<T><<<nblk(nn * nt), 256, 0, s>>>(nn, nt, it, b);
+    STFEM_HIP(hipGetLastError());
+  }
+  template <typename T>
+  void launch_zero_dirichlet(const GridDesc &g, int nt, T *x, hipStream_t s)
+  {
+    if (g.n_nodes == 0)
+      return;
+    k_zero_dirichlet<T><<<nblk(g.n_nodes), 256, 0, s>>>(g, nt, x);
+    STFEM_HIP(hipGetLastError());
+  }
+
+  template void launch_dot<double>(long, const double *, const double *, double *, double *, int,
+                                   hipStream_t);
+  template void launch_dot<float>(long, const float *, const float *, double *, double *, int,
+                                  hipStream_t);
+  template void launch_axpby<double>(long, double, const double *, double, double *, hipStream_t);
+  template void launch_axpby<float>(long, float, const float *, float, float *, hipStream_t);
+  template void launch_convert<float, double>(long, const double *, float *, hipStream_t);
+  template void launch_convert<double, float>(long, const float *, double *, hipStream_t);
+  template void launch_blocks_to_interleaved<double>(long, int, const double *, double *,
+                                                     hipStream_t);
+  template void launch_blocks_to_interleaved<float>(long, int, const float *, float *,
+                                                    hipStream_t);
+  template void launch_interleaved_to_blocks<double>(long, int, const double *, double *,
+                                                     hipStream_t);
+  template void launch_interleaved_to_blocks<float>(long, int, const float *, float *,
+                                                    hipStream_t);
+  template void launch_zero_dirichlet<double>(const GridDesc &, int, double *, hipStream_t);
+  template void launch_zero_dirichlet<float>(const GridDesc &, int, float *, hipStream_t);
+} // namespace stfem

@@ -1,0 +1,89 @@
+// Host-side launchers of the gfx950 kernels.  All pointers are device
+// pointers; all launches are asynchronous on the given stream.
+#pragma once
+
+#include "common.h"
+
+namespace stfem
+{
+  // ------------------------------------------------------------ shape data
+  // 1D tables of the FE_Q(p) element on Gauss-Lobatto nodes at the
+  // Gauss(p+1) points: S[q][a] = phi_a(x_q), D[q][q'] = derivative of the
+  // Lagrange polynomial on the Gauss points (collocation), W[q] = weight.
+  // Uploaded once per device into constant memory for p = 1..MAX_N-1.
+  void upload_shape_tables(const double *S, const double *D, const double *W, const double *X,
+                           int N);
+
+  // ------------------------------------------------------- space-time op
+  // dst[node][j] (+)= sum_i Alpha(j,i) (K src_i)[node] + Beta(j,i) (M src_i)[node]
+  // K: stiffness int kappa grad phi . grad phi (Q1 mapping), M: mass.
+  // Dirichlet nodes are read as zero and never written.  dst must be
+  // zeroed by the caller (atomic accumulation).
+  template <typename T>
+  struct StOpArgs
+  {
+    GridDesc      g;
+    const double *vertices;  // (nc+1)^dim vertex coordinates, dim per vertex
+    int           coef_mode; // 0: constant coef_value, 1: per cell, 2: per (cell, q)
+    T             coef_value;
+    const T      *coef;
+    const T      *alpha;     // nt_out x nt_in (stiffness weights), device
+    const T      *beta;      // nt_out x nt_in (mass weights), device
+    int           nt_in, nt_out;
+  };
+  template <typename T>
+  void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, hipStream_t s);
+
+  // ------------------------------------------------------------ Vanka
+  // dst = sum_cells (1/valence) * scatter( Binv_c * gather(src) ) with
+  // explicit per-cell inverses stored column-major, nb = nt * (p+1)^dim.
+  template <typename T>
+  void launch_vanka_explicit(const GridDesc &g, int nt, const T *binv, const T *src, T *dst,
+                             hipStream_t s);
+  // Factored form: Binv_c = (I x S_c) blockdiag_k (W_{c,k}) (I x S_c^T),
+  // S_c (n_c x n_c, row-major), W_{c,k} (nt x nt row-major) per mode k.
+  template <typename T>
+  void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *W, const T *src,
+                             T *dst, hipStream_t s);
+
+  // ------------------------------------------------------------ transfers
+  // Per node: dst[node][a] (= or +=) sum_b M(a,b) src[node][b].
+  template <typename T>
+  void launch_time_transfer(long n_nodes, const T *M, int nt_dst, int nt_src, const T *src,
+                            T *dst, bool add, hipStream_t s);
+  // 1D banded operator applied along direction d of a node array with
+  // dims n[0..2] and nt interleaved components.  Row o of the operator has
+  // cnt[o] entries (col, w) stored at o*K.
+  template <typename T>
+  void launch_sweep(const T *in, T *out, int nt, int d, const int n_in[3], int m_out,
+                    const int *cnt, const int *col, const T *w, int K, bool add, hipStream_t s);
+
+  // ------------------------------------------------------------ BLAS-1
+  // Deterministic two-stage reductions.  `partial` needs reduce_slots()
+  // doubles.
+  int  reduce_slots();
+  // out[0] (op)= dot(x, y)  (op: 0 assign, 1 assign sqrt)
+  template <typename T>
+  void launch_dot(long n, const T *x, const T *y, double *partial, double *out, int op,
+                  hipStream_t s);
+  // x += coef[0] * factor * y ; then out = dot(x, z) (z may equal x; then
+  // out = sqrt(dot) when op == 1)
+  void launch_axpy_dot(long n, double *x, const double *coef, double factor, const double *y,
+                       const double *z, double *partial, double *out, int op, hipStream_t s);
+  template <typename T>
+  void launch_axpby(long n, T a, const T *x, T b, T *y, hipStream_t s); // y = a x + b y
+  void launch_multi_axpy(long n, int m, const double *const *z, const double *coef,
+                         double *x, hipStream_t s);                      // x += sum coef_j z_j
+  template <typename TD, typename TS>
+  void launch_convert(long n, const TS *src, TD *dst, hipStream_t s);
+  // [t][node] <-> [node][t]
+  template <typename T>
+  void launch_blocks_to_interleaved(long n_nodes, int nt, const T *blocks, T *inter,
+                                    hipStream_t s);
+  template <typename T>
+  void launch_interleaved_to_blocks(long n_nodes, int nt, const T *inter, T *blocks,
+                                    hipStream_t s);
+  // zero the Dirichlet nodes of an interleaved vector
+  template <typename T>
+  void launch_zero_dirichlet(const GridDesc &g, int nt, T *x, hipStream_t s);
+} // namespace stfem

@@ -1,0 +1,1701 @@
+// Host side of the MI355X space-time multigrid slab solver: setup (mesh
+// hierarchy, element and patch matrices, Vanka inverses, transfer
+// operators, relaxation estimates) and the drivers (V-cycle, FGMRES, heat
+// time integrator) that launch the gfx950 kernels.
+#include "stfem_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <set>
+#include <array>
+#include <random>
+#include <sstream>
+
+#include <omp.h>
+
+namespace stfem
+{
+  // =============================================================== Context
+  Context::Context(int dev)
+    : device(dev)
+  {
+    STFEM_HIP(hipSetDevice(dev));
+    STFEM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    partial.alloc(reduce_slots());
+    scalars.alloc(64);
+    // shape tables for every supported degree, once per device
+    static std::mutex           mtx;
+    static std::set<int>        done;
+    std::lock_guard<std::mutex> lk(mtx);
+    if (!done.count(dev))
+      {
+        for (int p = 1; p < MAX_N; ++p)
+          {
+            Shape1D sh(p);
+            upload_shape_tables(sh.S.data(), sh.D.data(), sh.qw.data(), sh.qx.data(), p + 1);
+          }
+        done.insert(dev);
+      }
+  }
+  Context::~Context()
+  {
+    if (stream)
+      (void)hipStreamDestroy(stream);
+  }
+  void Context::sync() { STFEM_HIP(hipStreamSynchronize(stream)); }
+
+  // =============================================================== Shape1D
+  Shape1D::Shape1D(int p_)
+    : p(p_)
+    , n(p_ + 1)
+  {
+    gll           = quad_gauss_lobatto(n).x;
+    const auto gq = quad_gauss(n);
+    qx            = gq.x;
+    qw            = gq.w;
+    L.nodes       = gll;
+    LagrangeBasis Lq{qx};
+    S.resize(size_t(n) * n);
+    D.resize(size_t(n) * n);
+    Sd.resize(size_t(n) * n);
+    for (int q = 0; q < n; ++q)
+      for (int a = 0; a < n; ++a)
+        {
+          S[q * n + a]  = L.value(a, qx[q]);
+          Sd[q * n + a] = L.deriv(a, qx[q]);
+          D[q * n + a]  = Lq.deriv(a, qx[q]);
+        }
+  }
+
+  // ============================================================ mesh levels
+  std::vector<MeshLevel> build_mesh_levels(int dim, const int subdiv[3], int refine, int p,
+                                           const double *fine_vertices, const double *lo,
+                                           const double *hi, const double *fine_kappa,
+                                           int dirichlet_mask)
+  {
+    STFEM_REQUIRE(dim == 2 || dim == 3, "mesh: dim must be 2 or 3");
+    STFEM_REQUIRE(p >= 1 && p < MAX_N, "mesh: space degree must be in [1, 4]");
+    STFEM_REQUIRE(refine >= 0 && refine < 12, "mesh: refinement out of range");
+    int fnc[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d)
+      {
+        STFEM_REQUIRE(subdiv[d] >= 1, "mesh: subdivisions must be positive");
+        fnc[d] = subdiv[d] << refine;
+      }
+    const long fnv0 = fnc[0] + 1, fnv1 = fnc[1] + 1, fnv2 = (dim == 3) ? fnc[2] + 1 : 1;
+    std::vector<double> FX(size_t(fnv0 * fnv1 * fnv2) * dim);
+    for (long k = 0; k < fnv2; ++k)
+      for (long j = 0; j < fnv1; ++j)
+        for (long i = 0; i < fnv0; ++i)
+          {
+            const long id = i + fnv0 * (j + fnv1 * k);
+            const long ii[3] = {i, j, k};
+            for (int d = 0; d < dim; ++d)
+              FX[id * dim + d] = fine_vertices ?
+                                   fine_vertices[id * dim + d] :
+                                   lo[d] + (hi[d] - lo[d]) * double(ii[d]) / fnc[d];
+          }
+    const Shape1D          sh(p);
+    const int              n = p + 1, nq = n;
+    std::vector<MeshLevel> out(refine + 1);
+    for (int s = 0; s <= refine; ++s)
+      {
+        MeshLevel &m   = out[s];
+        const int  fac = 1 << (refine - s);
+        m.g.dim        = dim;
+        m.g.p          = p;
+        m.g.n_cells    = 1;
+        m.g.n_nodes    = 1;
+        for (int d = 0; d < 3; ++d)
+          {
+            m.g.nc[d] = (d < dim) ? fnc[d] / fac : 1;
+            m.g.nn[d] = (d < dim) ? m.g.nc[d] * p + 1 : 1;
+            m.g.n_cells *= m.g.nc[d];
+            m.g.n_nodes *= m.g.nn[d];
+          }
+        m.g.dirichlet = dirichlet_mask;
+        const long nv0 = m.g.nc[0] + 1, nv1 = m.g.nc[1] + 1, nv2 = (dim == 3) ? m.g.nc[2] + 1 : 1;
+        m.X.resize(size_t(nv0 * nv1 * nv2) * dim);
+        for (long k = 0; k < nv2; ++k)
+          for (long j = 0; j < nv1; ++j)
+            for (long i = 0; i < nv0; ++i)
+              {
+                const long fid = (i * fac) + fnv0 * ((j * fac) + fnv1 * (k * fac));
+                const long id  = i + nv0 * (j + nv1 * k);
+                for (int d = 0; d < dim; ++d)
+                  m.X[id * dim + d] = FX[fid * dim + d];
+              }
+        if (!fine_kappa)
+          {
+            m.coef_mode  = 0;
+            m.coef_value = 1.0;
+          }
+        else if (s == refine)
+          {
+            m.coef_mode = 1;
+            m.coef.assign(fine_kappa, fine_kappa + m.g.n_cells);
+          }
+        else
+          {
+            // value of the finest cell containing each quadrature point
+            m.coef_mode     = 2;
+            const long nqc  = (dim == 3) ? nq * nq * nq : nq * nq;
+            m.coef.resize(size_t(m.g.n_cells * nqc));
+            for (long c = 0; c < m.g.n_cells; ++c)
+              {
+                const int cx = int(c % m.g.nc[0]);
+                const int cy = int((c / m.g.nc[0]) % m.g.nc[1]);
+                const int cz = (dim == 3) ? int(c / ((long)m.g.nc[0] * m.g.nc[1])) : 0;
+                for (long q = 0; q < nqc; ++q)
+                  {
+                    const int qi[3] = {int(q % nq), int((q / nq) % nq), int(q / (nq * nq))};
+                    long      f[3] = {cx, cy, cz};
+                    for (int d = 0; d < dim; ++d)
+                      {
+                        int sub = int(std::floor(sh.qx[qi[d]] * fac));
+                        sub     = std::min(std::max(sub, 0), fac - 1);
+                        f[d]    = f[d] * fac + sub;
+                      }
+                    const long fid = f[0] + (long)fnc[0] * (f[1] + (long)fnc[1] * f[2]);
+                    m.coef[c * nqc + q] = fine_kappa[fid];
+                  }
+              }
+          }
+      }
+    return out;
+  }
+
+  namespace
+  {
+    inline void cell_xyz(const GridDesc &g, long c, int &cx, int &cy, int &cz)
+    {
+      cx = int(c % g.nc[0]);
+      cy = int((c / g.nc[0]) % g.nc[1]);
+      cz = (g.dim == 3) ? int(c / ((long)g.nc[0] * g.nc[1])) : 0;
+    }
+    inline bool node_constrained(const GridDesc &g, long gx, long gy, long gz)
+    {
+      const int d = g.dirichlet;
+      bool      c = (gx == 0 && (d & 1)) || (gx == g.nn[0] - 1 && (d & 2)) ||
+                   (gy == 0 && (d & 4)) || (gy == g.nn[1] - 1 && (d & 8));
+      if (g.dim == 3)
+        c = c || (gz == 0 && (d & 16)) || (gz == g.nn[2] - 1 && (d & 32));
+      return c;
+    }
+
+    // element mass and stiffness matrices of one cell
+    void element_matrices(const MeshLevel &m, const Shape1D &sh, long c, double *Me, double *Ke)
+    {
+      const int dim = m.g.dim, n = sh.n, nq = n;
+      const int nl  = (dim == 3) ? n * n * n : n * n;
+      int       cx, cy, cz;
+      cell_xyz(m.g, c, cx, cy, cz);
+      double    V[8][3] = {};
+      const long nv0 = m.g.nc[0] + 1, nv1 = m.g.nc[1] + 1;
+      for (int v = 0; v < (1 << dim); ++v)
+        {
+          const long id = (cx + (v & 1)) + nv0 * ((cy + ((v >> 1) & 1)) + nv1 * (cz + ((v >> 2) & 1)));
+          for (int d = 0; d < dim; ++d)
+            V[v][d] = m.X[id * dim + d];
+        }
+      std::fill(Me, Me + size_t(nl) * nl, 0.0);
+      std::fill(Ke, Ke + size_t(nl) * nl, 0.0);
+      std::vector<double> phi(nl), gr(size_t(nl) * 3);
+      const int           nqz = (dim == 3) ? nq : 1;
+      for (int qz = 0; qz < nqz; ++qz)
+        for (int qy = 0; qy < nq; ++qy)
+          for (int qx = 0; qx < nq; ++qx)
+            {
+              const double xi[3] = {sh.qx[qx], sh.qx[qy], dim == 3 ? sh.qx[qz] : 0.0};
+              double       J[3][3] = {};
+              for (int v = 0; v < (1 << dim); ++v)
+                for (int e = 0; e < dim; ++e)
+                  {
+                    double g = 1;
+                    for (int f = 0; f < dim; ++f)
+                      {
+                        const int bit = (v >> f) & 1;
+                        g *= (f == e) ? (bit ? 1.0 : -1.0) : (bit ? xi[f] : 1 - xi[f]);
+                      }
+                    for (int d = 0; d < dim; ++d)
+                      J[d][e] += g * V[v][d];
+                  }
+              double Ji[3][3], det;
+              if (dim == 2)
+                {
+                  det      = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+                  Ji[0][0] = J[1][1] / det;
+                  Ji[0][1] = -J[0][1] / det;
+                  Ji[1][0] = -J[1][0] / det;
+                  Ji[1][1] = J[0][0] / det;
+                }
+              else
+                {
+                  double a[3][3];
+                  a[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+                  a[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+                  a[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+                  a[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+                  a[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+                  a[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+                  a[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+                  a[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+                  a[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+                  det     = J[0][0] * a[0][0] + J[0][1] * a[1][0] + J[0][2] * a[2][0];
+                  for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                      Ji[i][j] = a[i][j] / det;
+                }
+              const double w   = sh.qw[qx] * sh.qw[qy] * (dim == 3 ? sh.qw[qz] : 1.0);
+              const double JxW = std::abs(det) * w;
+              double       kap = m.coef_value;
+              if (m.coef_mode == 1)
+                kap = m.coef[c];
+              else if (m.coef_mode == 2)
+                {
+                  const long nqc = (dim == 3) ? nq * nq * nq : nq * nq;
+                  kap            = m.coef[c * nqc + qx + nq * (qy + nq * qz)];
+                }
+              int l = 0;
+              for (int c3 = 0; c3 < ((dim == 3) ? n : 1); ++c3)
+                for (int b = 0; b < n; ++b)
+                  for (int a = 0; a < n; ++a, ++l)
+                    {
+                      const double sx = sh.S[qx * n + a], sy = sh.S[qy * n + b];
+                      const double sz = (dim == 3) ? sh.S[qz * n + c3] : 1.0;
+                      const double dx = sh.Sd[qx * n + a], dy = sh.Sd[qy * n + b];
+                      const double dz = (dim == 3) ? sh.Sd[qz * n + c3] : 0.0;
+                      phi[l]          = sx * sy * sz;
+                      const double g[3] = {dx * sy * sz, sx * dy * sz, sx * sy * dz};
+                      for (int d = 0; d < dim; ++d)
+                        {
+                          double s = 0;
+                          for (int e = 0; e < dim; ++e)
+                            s += Ji[e][d] * g[e];
+                          gr[l * 3 + d] = s;
+                        }
+                    }
+              for (int i = 0; i < nl; ++i)
+                {
+                  const double pi = phi[i] * JxW;
+                  for (int j = 0; j < nl; ++j)
+                    {
+                      Me[size_t(i) * nl + j] += pi * phi[j];
+                      double s = 0;
+                      for (int d = 0; d < dim; ++d)
+                        s += gr[i * 3 + d] * gr[j * 3 + d];
+                      Ke[size_t(i) * nl + j] += kap * JxW * s;
+                    }
+                }
+            }
+    }
+  } // namespace
+
+  void cell_patch_matrices(const MeshLevel &m, const Shape1D &sh, std::vector<double> &Kp,
+                           std::vector<double> &Mp)
+  {
+    const GridDesc &g   = m.g;
+    const int       dim = g.dim, n = sh.n, p = sh.p;
+    const int       nl  = (dim == 3) ? n * n * n : n * n;
+    const long      nc  = g.n_cells;
+    const size_t    bs  = size_t(nl) * nl;
+    std::vector<double> Ke(size_t(nc) * bs), Me(size_t(nc) * bs);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long c = 0; c < nc; ++c)
+      element_matrices(m, sh, c, Me.data() + c * bs, Ke.data() + c * bs);
+    Kp.assign(size_t(nc) * bs, 0.0);
+    Mp.assign(size_t(nc) * bs, 0.0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long c = 0; c < nc; ++c)
+      {
+        int cx, cy, cz;
+        cell_xyz(g, c, cx, cy, cz);
+        double *K = Kp.data() + c * bs, *M = Mp.data() + c * bs;
+        std::vector<int> li, lj;
+        for (int dz = (dim == 3 ? -1 : 0); dz <= (dim == 3 ? 1 : 0); ++dz)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx)
+              {
+                const int ox = cx + dx, oy = cy + dy, oz = cz + dz;
+                if (ox < 0 || oy < 0 || oz < 0 || ox >= g.nc[0] || oy >= g.nc[1] ||
+                    (dim == 3 && oz >= g.nc[2]))
+                  continue;
+                const long o = ox + (long)g.nc[0] * (oy + (long)g.nc[1] * oz);
+                // local nodes of c shared with o, and their index in o
+                li.clear();
+                lj.clear();
+                const int dd[3] = {dx, dy, dz};
+                for (int c3 = 0; c3 < ((dim == 3) ? n : 1); ++c3)
+                  for (int b = 0; b < n; ++b)
+                    for (int a = 0; a < n; ++a)
+                      {
+                        const int loc[3] = {a, b, c3};
+                        int       oth[3];
+                        bool      ok = true;
+                        for (int d = 0; d < dim; ++d)
+                          {
+                            if (dd[d] == 0)
+                              oth[d] = loc[d];
+                            else if (dd[d] == -1)
+                              {
+                                ok     = ok && loc[d] == 0;
+                                oth[d] = p;
+                              }
+                            else
+                              {
+                                ok     = ok && loc[d] == p;
+                                oth[d] = 0;
+                              }
+                          }
+                        if (!ok)
+                          continue;
+                        li.push_back(a + n * (b + n * c3));
+                        lj.push_back(oth[0] + n * (oth[1] + n * (dim == 3 ? oth[2] : 0)));
+                      }
+                const double *Ko = Ke.data() + o * bs, *Mo = Me.data() + o * bs;
+                for (size_t i = 0; i < li.size(); ++i)
+                  for (size_t j = 0; j < li.size(); ++j)
+                    {
+                      K[size_t(li[i]) * nl + li[j]] += Ko[size_t(lj[i]) * nl + lj[j]];
+                      M[size_t(li[i]) * nl + li[j]] += Mo[size_t(lj[i]) * nl + lj[j]];
+                    }
+              }
+        // constrained rows/columns keep only their diagonal
+        for (int l = 0; l < nl; ++l)
+          {
+            const int  a = l % n, b = (l / n) % n, c3 = (dim == 3) ? l / (n * n) : 0;
+            const long gx = (long)cx * p + a, gy = (long)cy * p + b, gz = (long)cz * p + c3;
+            if (!node_constrained(g, gx, gy, gz))
+              continue;
+            for (int j = 0; j < nl; ++j)
+              if (j != l)
+                {
+                  K[size_t(l) * nl + j] = K[size_t(j) * nl + l] = 0.0;
+                  M[size_t(l) * nl + j] = M[size_t(j) * nl + l] = 0.0;
+                }
+          }
+      }
+  }
+
+  // ========================================================== SystemMatrix
+  template <typename T>
+  SystemMatrix<T>::SystemMatrix(Context &ctx, const MeshLevel &m, const DMat &Alpha,
+                                const DMat &Beta)
+    : ctx_(ctx)
+    , grid_(m.g)
+    , coef_mode_(m.coef_mode)
+    , coef_value_(T(m.coef_value))
+    , A_(Alpha)
+    , B_(Beta)
+    , nt_in_(Alpha.n)
+    , nt_out_(Alpha.m)
+  {
+    STFEM_REQUIRE(Alpha.m == Beta.m && Alpha.n == Beta.n, "SystemMatrix: Alpha/Beta shapes differ");
+    verts_.upload(m.X.data(), m.X.size(), ctx.stream);
+    if (coef_mode_ != 0)
+      {
+        std::vector<T> c(m.coef.begin(), m.coef.end());
+        coef_.upload(c.data(), c.size(), ctx.stream);
+      }
+    std::vector<T> a(A_.a.begin(), A_.a.end()), b(B_.a.begin(), B_.a.end());
+    dA_.upload(a.data(), a.size(), ctx.stream);
+    dB_.upload(b.data(), b.size(), ctx.stream);
+    if (nt_in_ == nt_out_)
+      {
+        const DMat     At = A_.transposed(), Bt = B_.transposed();
+        std::vector<T> at(At.a.begin(), At.a.end()), bt(Bt.a.begin(), Bt.a.end());
+        dAt_.upload(at.data(), at.size(), ctx.stream);
+        dBt_.upload(bt.data(), bt.size(), ctx.stream);
+      }
+    ctx.sync();
+  }
+
+  template <typename T>
+  void SystemMatrix<T>::apply(T *dst, const T *src, const DBuf<T> &a, const DBuf<T> &b,
+                              bool accumulate) const
+  {
+    if (!accumulate)
+      STFEM_HIP(hipMemsetAsync(dst, 0, size_t(grid_.n_nodes) * nt_out_ * sizeof(T), ctx_.stream));
+    StOpArgs<T> args;
+    args.g          = grid_;
+    args.vertices   = verts_.get();
+    args.coef_mode  = coef_mode_;
+    args.coef_value = coef_value_;
+    args.coef       = coef_.get();
+    args.alpha      = a.get();
+    args.beta       = b.get();
+    args.nt_in      = nt_in_;
+    args.nt_out     = nt_out_;
+    launch_st_apply<T>(args, src, dst, ctx_.stream);
+  }
+  template <typename T>
+  void SystemMatrix<T>::vmult(T *dst, const T *src) const
+  {
+    apply(dst, src, dA_, dB_, false);
+  }
+  template <typename T>
+  void SystemMatrix<T>::Tvmult(T *dst, const T *src) const
+  {
+    STFEM_REQUIRE(nt_in_ == nt_out_, "Tvmult needs square time matrices");
+    apply(dst, src, dAt_, dBt_, false);
+  }
+  template <typename T>
+  void SystemMatrix<T>::vmult_add_nx1(T *dst, const T *src) const
+  {
+    STFEM_REQUIRE(nt_in_ == 1, "vmult_add_nx1 needs nt x 1 time matrices");
+    apply(dst, src, dA_, dB_, true);
+  }
+  template <typename T>
+  double SystemMatrix<T>::vmult_bytes() const
+  {
+    const double s  = sizeof(T);
+    const double nv = double(grid_.nc[0] + 1) * (grid_.nc[1] + 1) *
+                      (grid_.dim == 3 ? grid_.nc[2] + 1 : 1);
+    double       kb = 0;
+    if (coef_mode_ == 1)
+      kb = s * grid_.n_cells;
+    else if (coef_mode_ == 2)
+      kb = s * grid_.n_cells * std::pow(grid_.p + 1, grid_.dim);
+    return s * grid_.n_nodes * (nt_in_ + nt_out_) + kb + 8.0 * grid_.dim * nv;
+  }
+
+  // ===================================================== PreconditionVanka
+  namespace
+  {
+    // in-place Gauss-Jordan inverse with partial pivoting (row-major)
+    bool gj_invert(double *A, int n, std::vector<double> &work)
+    {
+      work.assign(size_t(n) * n, 0.0);
+      for (int i = 0; i < n; ++i)
+        work[size_t(i) * n + i] = 1.0;
+      for (int c = 0; c < n; ++c)
+        {
+          int    r = c;
+          double best = std::abs(A[size_t(c) * n + c]);
+          for (int k = c + 1; k < n; ++k)
+            if (std::abs(A[size_t(k) * n + c]) > best)
+              best = std::abs(A[size_t(k) * n + c]), r = k;
+          if (best == 0.0)
+            return false;
+          if (r != c)
+            for (int j = 0; j < n; ++j)
+              {
+                std::swap(A[size_t(r) * n + j], A[size_t(c) * n + j]);
+                std::swap(work[size_t(r) * n + j], work[size_t(c) * n + j]);
+              }
+          const double d = 1.0 / A[size_t(c) * n + c];
+          for (int j = 0; j < n; ++j)
+            {
+              A[size_t(c) * n + j] *= d;
+              work[size_t(c) * n + j] *= d;
+            }
+          for (int k = 0; k < n; ++k)
+            if (k != c)
+              {
+                const double f = A[size_t(k) * n + c];
+                if (f == 0.0)
+                  continue;
+                double *Ak = A + size_t(k) * n, *Wk = work.data() + size_t(k) * n;
+                const double *Ac = A + size_t(c) * n, *Wc = work.data() + size_t(c) * n;
+                for (int j = 0; j < n; ++j)
+                  {
+                    Ak[j] -= f * Ac[j];
+                    Wk[j] -= f * Wc[j];
+                  }
+              }
+        }
+      std::memcpy(A, work.data(), sizeof(double) * size_t(n) * n);
+      return true;
+    }
+
+    // symmetric tridiagonal QL with implicit shifts (eigenvalues d, vectors z)
+    // after Householder reduction; dense symmetric input a (row-major n x n)
+    void sym_eigen(std::vector<double> &a, int n, std::vector<double> &d)
+    {
+      // Householder tridiagonalization (a becomes the orthogonal matrix Q)
+      std::vector<double> e(n, 0.0);
+      d.assign(n, 0.0);
+      for (int i = n - 1; i > 0; --i)
+        {
+          const int l = i - 1;
+          double    h = 0.0, scale = 0.0;
+          if (l > 0)
+            {
+              for (int k = 0; k <= l; ++k)
+                scale += std::abs(a[size_t(i) * n + k]);
+              if (scale == 0.0)
+                e[i] = a[size_t(i) * n + l];
+              else
+                {
+                  for (int k = 0; k <= l; ++k)
+                    {
+                      a[size_t(i) * n + k] /= scale;
+                      h += a[size_t(i) * n + k] * a[size_t(i) * n + k];
+                    }
+                  double f = a[size_t(i) * n + l];
+                  double g = (f >= 0.0 ? -std::sqrt(h) : std::sqrt(h));
+                  e[i]     = scale * g;
+                  h -= f * g;
+                  a[size_t(i) * n + l] = f - g;
+                  f                    = 0.0;
+                  for (int j = 0; j <= l; ++j)
+                    {
+                      a[size_t(j) * n + i] = a[size_t(i) * n + j] / h;
+                      g                    = 0.0;
+                      for (int k = 0; k <= j; ++k)
+                        g += a[size_t(j) * n + k] * a[size_t(i) * n + k];
+                      for (int k = j + 1; k <= l; ++k)
+                        g += a[size_t(k) * n + j] * a[size_t(i) * n + k];
+                      e[j] = g / h;
+                      f += e[j] * a[size_t(i) * n + j];
+                    }
+                  const double hh = f / (h + h);
+                  for (int j = 0; j <= l; ++j)
+                    {
+                      f    = a[size_t(i) * n + j];
+                      e[j] = g = e[j] - hh * f;
+                      for (int k = 0; k <= j; ++k)
+                        a[size_t(j) * n + k] -= (f * e[k] + g * a[size_t(i) * n + k]);
+                    }
+                }
+            }
+          else
+            e[i] = a[size_t(i) * n + l];
+          d[i] = h;
+        }
+      d[0] = 0.0;
+      e[0] = 0.0;
+      for (int i = 0; i < n; ++i)
+        {
+          const int l = i - 1;
+          if (d[i] != 0.0)
+            for (int j = 0; j <= l; ++j)
+              {
+                double g = 0.0;
+                for (int k = 0; k <= l; ++k)
+                  g += a[size_t(i) * n + k] * a[size_t(k) * n + j];
+                for (int k = 0; k <= l; ++k)
+                  a[size_t(k) * n + j] -= g * a[size_t(k) * n + i];
+              }
+          d[i]                 = a[size_t(i) * n + i];
+          a[size_t(i) * n + i] = 1.0;
+          for (int j = 0; j <= l; ++j)
+            a[size_t(j) * n + i] = a[size_t(i) * n + j] = 0.0;
+        }
+      // implicit QL on the tridiagonal (d, e), accumulating into a
+      for (int i = 1; i < n; ++i)
+        e[i - 1] = e[i];
+      e[n - 1] = 0.0;
+      for (int l = 0; l < n; ++l)
+        {
+          int iter = 0, m;
+          do
+            {
+              for (m = l; m < n - 1; ++m)
+                {
+                  const double dd = std::abs(d[m]) + std::abs(d[m + 1]);
+                  if (std::abs(e[m]) <= 1e-15 * dd)
+                    break;
+                }
+              if (m != l)
+                {
+                  if (++iter > 60)
+                    throw Error(STATUS_ERR_INTERNAL, "eigen: QL did not converge");
+                  double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                  double r = std::hypot(g, 1.0);
+                  g        = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::abs(r) : -std::abs(r)));
+                  double s = 1.0, c = 1.0, pp = 0.0;
+                  int    i;
+                  for (i = m - 1; i >= l; --i)
+                    {
+                      double f = s * e[i], b = c * e[i];
+                      e[i + 1] = (r = std::hypot(f, g));
+                      if (r == 0.0)
+                        {
+                          d[i + 1] -= pp;
+                          e[m] = 0.0;
+                          break;
+                        }
+                      s        = f / r;
+                      c        = g / r;
+                      g        = d[i + 1] - pp;
+                      r        = (d[i] - g) * s + 2.0 * c * b;
+                      d[i + 1] = g + (pp = s * r);
+                      g        = c * r - b;
+                      for (int k = 0; k < n; ++k)
+                        {
+                          f                        = a[size_t(k) * n + i + 1];
+                          a[size_t(k) * n + i + 1] = s * a[size_t(k) * n + i] + c * f;
+                          a[size_t(k) * n + i]     = c * a[size_t(k) * n + i] - s * f;
+                        }
+                    }
+                  if (r == 0.0 && i >= l)
+                    continue;
+                  d[l] -= pp;
+                  e[l] = g;
+                  e[m] = 0.0;
+                }
+            }
+          while (m != l);
+        }
+    }
+  } // namespace
+
+  template <typename T>
+  PreconditionVanka<T>::PreconditionVanka(Context &ctx, const MeshLevel &m, const Shape1D &sh,
+                                          const DMat &Alpha, const DMat &Beta, Form form)
+    : ctx_(ctx)
+    , grid_(m.g)
+    , nt_(Alpha.m)
+    , form_(form)
+  {
+    std::vector<double> Kp, Mp;
+    cell_patch_matrices(m, sh, Kp, Mp);
+    const int    nl = (m.g.dim == 3) ? sh.n * sh.n * sh.n : sh.n * sh.n;
+    const int    nt = nt_, nb = nt * nl;
+    const long   nc = m.g.n_cells;
+    const size_t bs = size_t(nl) * nl;
+    const long   chunk = std::max<long>(1, std::min<long>(nc, (256L << 20) / (long(nb) * nb * 8)));
+    if (form_ == Form::explicit_inverse)
+      {
+        binv_.alloc(size_t(nc) * nb * nb);
+        std::vector<T> stage;
+        for (long c0 = 0; c0 < nc; c0 += chunk)
+          {
+            const long c1 = std::min(nc, c0 + chunk);
+            stage.resize(size_t(c1 - c0) * nb * nb);
+            bool singular = false;
+#pragma omp parallel
+            {
+              std::vector<double> B(size_t(nb) * nb), work;
+#pragma omp for schedule(dynamic, 4)
+              for (long c = c0; c < c1; ++c)
+                {
+                  const double *K = Kp.data() + c * bs, *M = Mp.data() + c * bs;
+                  std::fill(B.begin(), B.end(), 0.0);
+                  for (int i = 0; i < nt; ++i)
+                    for (int j = 0; j < nt; ++j)
+                      {
+                        const double a = Alpha(i, j), b = Beta(i, j);
+                        if (a == 0.0 && b == 0.0)
+                          continue;
+                        for (int k = 0; k < nl; ++k)
+                          for (int l = 0; l < nl; ++l)
+                            B[size_t(k + i * nl) * nb + l + j * nl] =
+                              b * M[size_t(k) * nl + l] + a * K[size_t(k) * nl + l];
+                      }
+                  if (!gj_invert(B.data(), nb, work))
+                    singular = true;
+                  // column-major store: element (row, col) at col * nb + row
+                  T *dst = stage.data() + size_t(c - c0) * nb * nb;
+                  for (int row = 0; row < nb; ++row)
+                    for (int col = 0; col < nb; ++col)
+                      dst[size_t(col) * nb + row] = T(B[size_t(row) * nb + col]);
+                }
+            }
+            if (singular)
+              throw Error(STATUS_ERR_INTERNAL, "Vanka: singular patch matrix");
+            STFEM_HIP(hipMemcpy(binv_.get() + size_t(c0) * nb * nb, stage.data(),
+                                stage.size() * sizeof(T), hipMemcpyHostToDevice));
+          }
+      }
+    else
+      {
+        STFEM_REQUIRE(nt <= 16, "Vanka factored form supports at most 16 time dofs");
+        S_.alloc(size_t(nc) * bs);
+        W_.alloc(size_t(nc) * nl * nt * nt);
+        const long     ch = std::max<long>(1, std::min<long>(nc, 4096));
+        std::vector<T> sS, sW;
+        for (long c0 = 0; c0 < nc; c0 += ch)
+          {
+            const long c1 = std::min(nc, c0 + ch);
+            sS.resize(size_t(c1 - c0) * bs);
+            sW.resize(size_t(c1 - c0) * nl * nt * nt);
+            bool bad = false;
+#pragma omp parallel
+            {
+              std::vector<double> Lc(bs), C(bs), lam;
+#pragma omp for schedule(dynamic, 4)
+              for (long c = c0; c < c1; ++c)
+                {
+                  const double *K = Kp.data() + c * bs, *M = Mp.data() + c * bs;
+                  // Cholesky M = L L^T
+                  std::fill(Lc.begin(), Lc.end(), 0.0);
+                  bool ok = true;
+                  for (int j = 0; j < nl && ok; ++j)
+                    {
+                      double s = M[size_t(j) * nl + j];
+                      for (int k = 0; k < j; ++k)
+                        s -= Lc[size_t(j) * nl + k] * Lc[size_t(j) * nl + k];
+                      if (s <= 0)
+                        {
+                          ok = false;
+                          break;
+                        }
+                      const double ljj        = std::sqrt(s);
+                      Lc[size_t(j) * nl + j] = ljj;
+                      for (int i = j + 1; i < nl; ++i)
+                        {
+                          double t = M[size_t(i) * nl + j];
+                          for (int k = 0; k < j; ++k)
+                            t -= Lc[size_t(i) * nl + k] * Lc[size_t(j) * nl + k];
+                          Lc[size_t(i) * nl + j] = t / ljj;
+                        }
+                    }
+                  if (!ok)
+                    {
+                      bad = true;
+                      continue;
+                    }
+                  // C = L^{-1} K L^{-T}: solve L Y = K, then L C^T = Y^T
+                  std::vector<double> Y(K, K + bs);
+                  for (int col = 0; col < nl; ++col)
+                    for (int i = 0; i < nl; ++i)
+                      {
+                        double s = Y[size_t(i) * nl + col];
+                        for (int k = 0; k < i; ++k)
+                          s -= Lc[size_t(i) * nl + k] * Y[size_t(k) * nl + col];
+                        Y[size_t(i) * nl + col] = s / Lc[size_t(i) * nl + i];
+                      }
+                  for (int row = 0; row < nl; ++row)
+                    for (int i = 0; i < nl; ++i)
+                      {
+                        double s = Y[size_t(row) * nl + i];
+                        for (int k = 0; k < i; ++k)
+                          s -= Lc[size_t(i) * nl + k] * C[size_t(row) * nl + k];
+                        C[size_t(row) * nl + i] = s / Lc[size_t(i) * nl + i];
+                      }
+                  for (int i = 0; i < nl; ++i)
+                    for (int j = i + 1; j < nl; ++j)
+                      {
+                        const double s = 0.5 * (C[size_t(i) * nl + j] + C[size_t(j) * nl + i]);
+                        C[size_t(i) * nl + j] = C[size_t(j) * nl + i] = s;
+                      }
+                  sym_eigen(C, nl, lam); // C now holds the eigenvectors Q (columns)
+                  // S = L^{-T} Q
+                  T *Sd = sS.data() + size_t(c - c0) * bs;
+                  for (int col = 0; col < nl; ++col)
+                    for (int i = nl - 1; i >= 0; --i)
+                      {
+                        double s = C[size_t(i) * nl + col];
+                        for (int k = i + 1; k < nl; ++k)
+                          s -= Lc[size_t(k) * nl + i] * double(Y[size_t(k) * nl + col]);
+                        s /= Lc[size_t(i) * nl + i];
+                        Y[size_t(i) * nl + col] = s;
+                      }
+                  for (int i = 0; i < nl; ++i)
+                    for (int col = 0; col < nl; ++col)
+                      Sd[size_t(i) * nl + col] = T(Y[size_t(i) * nl + col]);
+                  // W_k = (lambda_k Alpha + Beta)^{-1}
+                  T *Wd = sW.data() + size_t(c - c0) * nl * nt * nt;
+                  std::vector<double> Bk(size_t(nt) * nt), wk;
+                  for (int k = 0; k < nl; ++k)
+                    {
+                      for (int i = 0; i < nt; ++i)
+                        for (int j = 0; j < nt; ++j)
+                          Bk[size_t(i) * nt + j] = lam[k] * Alpha(i, j) + Beta(i, j);
+                      if (!gj_invert(Bk.data(), nt, wk))
+                        bad = true;
+                      for (int i = 0; i < nt * nt; ++i)
+                        Wd[size_t(k) * nt * nt + i] = T(Bk[i]);
+                    }
+                }
+            }
+            if (bad)
+              throw Error(STATUS_ERR_INTERNAL, "Vanka factored: patch not positive definite");
+            STFEM_HIP(hipMemcpy(S_.get() + size_t(c0) * bs, sS.data(), sS.size() * sizeof(T),
+                                hipMemcpyHostToDevice));
+            STFEM_HIP(hipMemcpy(W_.get() + size_t(c0) * nl * nt * nt, sW.data(),
+                                sW.size() * sizeof(T), hipMemcpyHostToDevice));
+          }
+      }
+  }
+
+  template <typename T>
+  void PreconditionVanka<T>::vmult(T *dst, const T *src) const
+  {
+    STFEM_HIP(hipMemsetAsync(dst, 0, size_t(grid_.n_nodes) * nt_ * sizeof(T), ctx_.stream));
+    if (form_ == Form::explicit_inverse)
+      launch_vanka_explicit<T>(grid_, nt_, binv_.get(), src, dst, ctx_.stream);
+    else
+      launch_vanka_factored<T>(grid_, nt_, S_.get(), W_.get(), src, dst, ctx_.stream);
+  }
+  template <typename T>
+  double PreconditionVanka<T>::sweep_bytes() const
+  {
+    const double s  = sizeof(T);
+    const double nl = std::pow(grid_.p + 1, grid_.dim);
+    const double nst = double(grid_.n_nodes) * nt_;
+    if (form_ == Form::explicit_inverse)
+      return s * grid_.n_cells * (nt_ * nl) * (nt_ * nl) + 3 * s * nst;
+    return s * grid_.n_cells * (nl * nl + nl * nt_ * nt_) + 3 * s * nst;
+  }
+
+  // ===================================================== time transfer
+  template <typename T>
+  MGTwoLevelTransferST<T>::MGTwoLevelTransferST(Context &ctx, TimeStepType type, int r, int nts,
+                                                TimeMGType kind, long n_nodes, bool overwrite)
+    : ctx_(ctx)
+    , n_nodes_(n_nodes)
+    , overwrite_(overwrite)
+  {
+    P_ = (kind == TimeMGType::k) ? get_time_projection_matrix(type, r - 1, r, nts) :
+                                   get_time_prolongation_matrix(type, r, nts);
+    const DMat     R = P_.transposed();
+    std::vector<T> p(P_.a.begin(), P_.a.end()), rr(R.a.begin(), R.a.end());
+    dP_.upload(p.data(), p.size(), ctx.stream);
+    dR_.upload(rr.data(), rr.size(), ctx.stream);
+    ctx.sync();
+  }
+  template <typename T>
+  void MGTwoLevelTransferST<T>::prolongate_and_add(T *dst, const T *src) const
+  {
+    launch_time_transfer<T>(n_nodes_, dP_.get(), P_.m, P_.n, src, dst, !overwrite_, ctx_.stream);
+  }
+  template <typename T>
+  void MGTwoLevelTransferST<T>::restrict_and_add(T *dst, const T *src) const
+  {
+    launch_time_transfer<T>(n_nodes_, dR_.get(), P_.n, P_.m, src, dst, !overwrite_, ctx_.stream);
+  }
+
+  // ===================================================== space transfer
+  template <typename T>
+  MGTwoLevelBlockTransfer<T>::MGTwoLevelBlockTransfer(Context &ctx, const GridDesc &fine,
+                                                      const GridDesc &coarse, const Shape1D &sh,
+                                                      int nt)
+    : ctx_(ctx)
+    , fine_(fine)
+    , coarse_(coarse)
+    , nt_(nt)
+  {
+    const int p = sh.p;
+    for (int d = 0; d < fine.dim; ++d)
+      {
+        STFEM_REQUIRE(fine.nc[d] == 2 * coarse.nc[d], "space transfer: levels not nested");
+        const int nf = fine.nn[d], ncn = coarse.nn[d];
+        const bool lo_d = (fine.dirichlet >> (2 * d)) & 1, hi_d = (fine.dirichlet >> (2 * d + 1)) & 1;
+        // 1D prolongation rows
+        std::vector<std::vector<std::pair<int, double>>> rows(nf);
+        for (int f = 0; f < nf; ++f)
+          {
+            if ((f == 0 && lo_d) || (f == nf - 1 && hi_d))
+              continue;
+            int fc = f / p;
+            int a  = f - fc * p;
+            if (fc >= fine.nc[d])
+              fc = fine.nc[d] - 1, a = p;
+            const int    cc = fc / 2, child = fc % 2;
+            const double xi = 0.5 * (sh.gll[a] + child);
+            for (int b = 0; b <= p; ++b)
+              {
+                const double w   = sh.L.value(b, xi);
+                const int    col = cc * p + b;
+                if (std::abs(w) < 1e-14)
+                  continue;
+                if ((col == 0 && lo_d) || (col == ncn - 1 && hi_d))
+                  continue;
+                rows[f].push_back({col, w});
+              }
+          }
+        std::vector<std::vector<std::pair<int, double>>> cols(ncn);
+        for (int f = 0; f < nf; ++f)
+          for (auto &e : rows[f])
+            cols[e.first].push_back({f, e.second});
+        auto pack = [&](Ell &E, const std::vector<std::vector<std::pair<int, double>>> &R) {
+          E.rows = int(R.size());
+          E.K    = 1;
+          for (auto &r : R)
+            E.K = std::max<int>(E.K, int(r.size()));
+          std::vector<int> cnt(E.rows), col(size_t(E.rows) * E.K, 0);
+          std::vector<T>   w(size_t(E.rows) * E.K, T(0));
+          for (int i = 0; i < E.rows; ++i)
+            {
+              cnt[i] = int(R[i].size());
+              for (size_t k = 0; k < R[i].size(); ++k)
+                {
+                  col[size_t(i) * E.K + k] = R[i][k].first;
+                  w[size_t(i) * E.K + k]   = T(R[i][k].second);
+                }
+            }
+          E.cnt.upload(cnt.data(), cnt.size(), ctx.stream);
+          E.col.upload(col.data(), col.size(), ctx.stream);
+          E.w.upload(w.data(), w.size(), ctx.stream);
+        };
+        pack(P_[d], rows);
+        pack(R_[d], cols);
+      }
+    // temporaries: largest intermediate of the sweeps
+    const long f0 = fine.nn[0], f1 = fine.nn[1], f2 = fine.nn[2];
+    const long c0 = coarse.nn[0], c1 = coarse.nn[1], c2 = coarse.nn[2];
+    const long big = std::max(std::max(f0 * c1 * c2, f0 * f1 * c2), std::max(c0 * f1 * f2, c0 * c1 * f2));
+    tmp1_.alloc(size_t(big) * nt);
+    tmp2_.alloc(size_t(big) * nt);
+    ctx.sync();
+  }
+
+  template <typename T>
+  void MGTwoLevelBlockTransfer<T>::prolongate_and_add(T *dst, const T *src) const
+  {
+    int        n[3] = {coarse_.nn[0], coarse_.nn[1], coarse_.nn[2]};
+    const int  D    = fine_.dim;
+    const T   *in   = src;
+    T         *bufs[2] = {const_cast<T *>(tmp1_.get()), const_cast<T *>(tmp2_.get())};
+    for (int d = 0; d < D; ++d)
+      {
+        const bool last = d == D - 1;
+        T         *out  = last ? dst : bufs[d % 2];
+        launch_sweep<T>(in, out, nt_, d, n, fine_.nn[d], P_[d].cnt.get(), P_[d].col.get(),
+                        P_[d].w.get(), P_[d].K, last, ctx_.stream);
+        n[d] = fine_.nn[d];
+        in   = out;
+      }
+  }
+  template <typename T>
+  void MGTwoLevelBlockTransfer<T>::restrict_and_add(T *dst, const T *src) const
+  {
+    int        n[3] = {fine_.nn[0], fine_.nn[1], fine_.nn[2]};
+    const int  D    = fine_.dim;
+    const T   *in   = src;
+    T         *bufs[2] = {const_cast<T *>(tmp1_.get()), const_cast<T *>(tmp2_.get())};
+    for (int d = 0; d < D; ++d)
+      {
+        const bool last = d == D - 1;
+        T         *out  = last ? dst : bufs[d % 2];
+        launch_sweep<T>(in, out, nt_, d, n, coarse_.nn[d], R_[d].cnt.get(), R_[d].col.get(),
+                        R_[d].w.get(), R_[d].K, last, ctx_.stream);
+        n[d] = coarse_.nn[d];
+        in   = out;
+      }
+  }
+
+  // =================================================================== GMG
+  template <typename T>
+  GMG<T>::GMG(Context &ctx, std::vector<GMGLevel<T>> &&levels,
+              std::vector<TwoLevelTransfer<T>> &&transfers, const GMGOptions &opt)
+    : ctx_(ctx)
+    , lv_(std::move(levels))
+    , tr_(std::move(transfers))
+    , opt_(opt)
+  {
+    const int nl = int(lv_.size());
+    sol_.resize(nl);
+    def_.resize(nl);
+    t1_.resize(nl);
+    t2_.resize(nl);
+    for (int l = 0; l < nl; ++l)
+      {
+        const size_t sz = size_t(lv_[l].A->m());
+        sol_[l].alloc(sz);
+        def_[l].alloc(sz);
+        t1_[l].alloc(sz);
+        t2_[l].alloc(sz);
+      }
+    for (int l = 0; l < nl; ++l)
+      lv_[l].omega = opt_.estimate_relaxation ? estimate_omega(l) : 1.0;
+  }
+
+  template <typename T>
+  int GMG<T>::steps(int l) const
+  {
+    const int L = int(lv_.size()) - 1;
+    return opt_.variable ? (1 << (L - l)) : 1;
+  }
+
+  // PreconditionChebyshev::estimate_eigenvalues (power iteration on
+  // Vanka^{-1} A, constant start vector), omega = 2 / (alpha + lambda_max),
+  // alpha = 0.9 lambda_max (gmg.h:792-844).
+  template <typename T>
+  double GMG<T>::estimate_omega(int l) const
+  {
+    if (l == 0)
+      return 1.0;
+    const auto  &L  = lv_[l];
+    const long   sz = L.A->m();
+    T           *ev = sol_[l].get(), *v1 = t1_[l].get(), *v2 = t2_[l].get();
+    std::vector<T> ones(sz, T(1));
+    STFEM_HIP(hipMemcpyAsync(ev, ones.data(), sz * sizeof(T), hipMemcpyHostToDevice, ctx_.stream));
+    launch_zero_dirichlet<T>(L.A->grid(), L.A->n_blocks(), ev, ctx_.stream);
+    double *dsc = ctx_.scalars.get();
+    double  h[2];
+    launch_dot<T>(sz, ev, ev, ctx_.partial.get(), dsc, 1, ctx_.stream);
+    STFEM_HIP(hipMemcpyAsync(h, dsc, sizeof(double), hipMemcpyDeviceToHost, ctx_.stream));
+    ctx_.sync();
+    launch_axpby<T>(sz, T(1.0 / h[0]), ev, T(0), ev, ctx_.stream);
+    double est = 0;
+    for (int it = 0; it < opt_.eig_iterations; ++it)
+      {
+        L.A->vmult(v2, ev);
+        L.V->vmult(v1, v2);
+        launch_dot<T>(sz, ev, v1, ctx_.partial.get(), dsc, 0, ctx_.stream);
+        launch_dot<T>(sz, v1, v1, ctx_.partial.get(), dsc + 1, 1, ctx_.stream);
+        STFEM_HIP(hipMemcpyAsync(h, dsc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx_.stream));
+        ctx_.sync();
+        est = h[0];
+        launch_axpby<T>(sz, T(1.0 / h[1]), v1, T(0), ev, ctx_.stream);
+      }
+    const double lmax = std::abs(est) * opt_.eig_safety;
+    return 2.0 / (0.9 * lmax + lmax);
+  }
+
+  template <typename T>
+  void GMG<T>::relax(int l, T *u, const T *rhs, bool first) const
+  {
+    const auto &L  = lv_[l];
+    const long  sz = L.A->m();
+    const T     om = T(L.omega);
+    if (first)
+      {
+        L.V->vmult(u, rhs);
+        if (L.omega != 1.0)
+          launch_axpby<T>(sz, om, u, T(0), u, ctx_.stream);
+        return;
+      }
+    T *t1 = t1_[l].get(), *t2 = t2_[l].get();
+    L.A->vmult(t1, u);
+    launch_axpby<T>(sz, T(1), rhs, T(-1), t1, ctx_.stream);
+    L.V->vmult(t2, t1);
+    launch_axpby<T>(sz, om, t2, T(1), u, ctx_.stream);
+  }
+  template <typename T>
+  void GMG<T>::smoother_apply(int l, T *u, const T *rhs) const
+  {
+    const int s2 = steps(l);
+    for (int i = 0; i < opt_.smoothing_steps; ++i)
+      relax(l, u, rhs, i == 0);
+    for (int k = 1; k < s2; ++k)
+      for (int i = 1; i <= opt_.smoothing_steps; ++i)
+        relax(l, u, rhs, false);
+  }
+  template <typename T>
+  void GMG<T>::smoother_smooth(int l, T *u, const T *rhs) const
+  {
+    const int s2 = steps(l);
+    for (int k = 0; k < s2; ++k)
+      for (int i = 1; i <= opt_.smoothing_steps; ++i)
+        relax(l, u, rhs, false);
+  }
+  template <typename T>
+  void GMG<T>::level_v_step(int l) const
+  {
+    if (l == 0)
+      {
+        smoother_apply(0, sol_[0].get(), def_[0].get());
+        return;
+      }
+    smoother_apply(l, sol_[l].get(), def_[l].get());
+    const long sz = lv_[l].A->m();
+    T         *t  = t1_[l].get();
+    lv_[l].A->vmult(t, sol_[l].get());
+    launch_axpby<T>(sz, T(1), def_[l].get(), T(-1), t, ctx_.stream);
+    def_[l - 1].zero(ctx_.stream);
+    tr_[l].restrict_and_add(def_[l - 1].get(), t);
+    sol_[l - 1].zero(ctx_.stream);
+    level_v_step(l - 1);
+    tr_[l].prolongate_and_add(sol_[l].get(), sol_[l - 1].get());
+    smoother_smooth(l, sol_[l].get(), def_[l].get());
+  }
+  template <typename T>
+  void GMG<T>::vcycle() const
+  {
+    level_v_step(int(lv_.size()) - 1);
+  }
+  template <typename T>
+  void GMG<T>::vmult(double *dst, const double *src) const
+  {
+    const int  L  = int(lv_.size()) - 1;
+    const long sz = lv_[L].A->m();
+    if constexpr (std::is_same_v<T, double>)
+      STFEM_HIP(hipMemcpyAsync(def_[L].get(), src, sz * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx_.stream));
+    else
+      launch_convert<T, double>(sz, src, def_[L].get(), ctx_.stream);
+    vcycle();
+    if constexpr (std::is_same_v<T, double>)
+      STFEM_HIP(hipMemcpyAsync(dst, sol_[L].get(), sz * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx_.stream));
+    else
+      launch_convert<double, T>(sz, sol_[L].get(), dst, ctx_.stream);
+  }
+
+  // ================================================================ FGMRES
+  SolverFGMRES::SolverFGMRES(Context &ctx, long n, const SolverControlData &c)
+    : ctx_(ctx)
+    , n_(n)
+    , c_(c)
+  {
+    STFEM_REQUIRE(c.basis >= 2 && c.basis <= 60, "FGMRES: basis size must be in [2, 60]");
+    aux_.alloc(n);
+  }
+
+  template <typename Op, typename Prec>
+  int SolverFGMRES::solve(const Op &A, const Prec &P, double *x, const double *b)
+  {
+    const int m = c_.basis;
+    while ((int)v_.size() < m + 1)
+      {
+        v_.emplace_back(size_t(n_));
+        z_.emplace_back(size_t(n_));
+      }
+    hist_.clear();
+    double       *aux = aux_.get();
+    double       *dH  = ctx_.scalars.get();
+    double       *prt = ctx_.partial.get();
+    int           acc = 0;
+    double        reduced_tol = 0.0;
+    int           state = 0;
+    auto          check = [&](int step, double val) {
+      hist_.push_back(val);
+      if (step == 0)
+        reduced_tol = val * c_.reltol;
+      last_step_ = step;
+      if (!std::isfinite(val))
+        return -2;
+      if (val <= reduced_tol || val <= c_.abstol)
+        return 1;
+      if (step >= c_.maxiter)
+        return -1;
+      return 0;
+    };
+    std::vector<double> hcol(m + 1);
+    do
+      {
+        A.vmult(aux, x);
+        launch_axpby<double>(n_, 1.0, b, -1.0, aux, ctx_.stream);
+        launch_dot<double>(n_, aux, aux, prt, dH, 1, ctx_.stream);
+        double beta;
+        STFEM_HIP(hipMemcpyAsync(&beta, dH, sizeof(double), hipMemcpyDeviceToHost, ctx_.stream));
+        ctx_.sync();
+        state = check(acc, beta);
+        if (state != 0)
+          break;
+        std::vector<std::vector<double>> H(m + 1, std::vector<double>(m, 0.0));
+        std::vector<double>              y;
+        double                           a = beta;
+        for (int j = 0; j < m; ++j)
+          {
+            if (a != 0.0)
+              launch_axpby<double>(n_, 1.0 / a, aux, 0.0, v_[j].get(), ctx_.stream);
+            else
+              v_[j].zero(ctx_.stream);
+            P.vmult(z_[j].get(), v_[j].get());
+            A.vmult(aux, z_[j].get());
+            // modified Gram-Schmidt with fused update + dot; coefficients
+            // stay on the device
+            launch_dot<double>(n_, aux, v_[0].get(), prt, dH, 0, ctx_.stream);
+            for (int i = 1; i <= j; ++i)
+              launch_axpy_dot(n_, aux, dH + i - 1, -1.0, v_[i - 1].get(), v_[i].get(), prt,
+                              dH + i, 0, ctx_.stream);
+            launch_axpy_dot(n_, aux, dH + j, -1.0, v_[j].get(), nullptr, prt, dH + j + 1, 1,
+                            ctx_.stream);
+            STFEM_HIP(hipMemcpyAsync(hcol.data(), dH, (j + 2) * sizeof(double),
+                                     hipMemcpyDeviceToHost, ctx_.stream));
+            ctx_.sync();
+            for (int i = 0; i <= j + 1; ++i)
+              H[i][j] = hcol[i];
+            a = hcol[j + 1];
+            const int ncol = j; // projected solve once a second vector exists
+            if (ncol > 0)
+              {
+                std::vector<std::vector<double>> R(ncol + 1, std::vector<double>(ncol));
+                for (int r0 = 0; r0 <= ncol; ++r0)
+                  for (int c0 = 0; c0 < ncol; ++c0)
+                    R[r0][c0] = H[r0][c0];
+                std::vector<double> g(ncol + 1, 0.0);
+                g[0] = beta;
+                for (int c0 = 0; c0 < ncol; ++c0)
+                  {
+                    const double h1 = R[c0][c0], h2 = R[c0 + 1][c0];
+                    const double rr = std::hypot(h1, h2);
+                    const double cs = rr == 0 ? 1 : h1 / rr, sn = rr == 0 ? 0 : h2 / rr;
+                    for (int c1 = c0; c1 < ncol; ++c1)
+                      {
+                        const double t1 = R[c0][c1], t2 = R[c0 + 1][c1];
+                        R[c0][c1]     = cs * t1 + sn * t2;
+                        R[c0 + 1][c1] = -sn * t1 + cs * t2;
+                      }
+                    const double g1 = g[c0], g2 = g[c0 + 1];
+                    g[c0]     = cs * g1 + sn * g2;
+                    g[c0 + 1] = -sn * g1 + cs * g2;
+                  }
+                y.assign(ncol, 0.0);
+                for (int c0 = ncol - 1; c0 >= 0; --c0)
+                  {
+                    double s = g[c0];
+                    for (int c1 = c0 + 1; c1 < ncol; ++c1)
+                      s -= R[c0][c1] * y[c1];
+                    y[c0] = s / R[c0][c0];
+                  }
+                state = check(++acc, std::abs(g[ncol]));
+                if (state != 0)
+                  break;
+              }
+          }
+        if (!y.empty())
+          {
+            std::vector<const double *> zp(y.size());
+            for (size_t j = 0; j < y.size(); ++j)
+              zp[j] = z_[j].get();
+            launch_multi_axpy(n_, int(y.size()), zp.data(), y.data(), x, ctx_.stream);
+          }
+      }
+    while (state == 0);
+    ctx_.sync();
+    if (state == -2)
+      return STATUS_ERR_NOT_FINITE;
+    return state == 1 ? STATUS_OK : STATUS_ERR_NO_CONVERGE;
+  }
+
+  // ============================================================ ProblemSpec
+  ProblemSpec ProblemSpec::parse(const std::string &s)
+  {
+    ProblemSpec                        ps;
+    std::map<std::string, std::string> kv;
+    std::stringstream                  ss(s);
+    std::string                        tok;
+    while (std::getline(ss, tok, ';'))
+      {
+        const auto e = tok.find('=');
+        if (e == std::string::npos)
+          continue;
+        kv[tok.substr(0, e)] = tok.substr(e + 1);
+      }
+    auto geti = [&](const char *k, int &v) {
+      if (kv.count(k))
+        v = std::stoi(kv[k]);
+    };
+    auto getd = [&](const char *k, double &v) {
+      if (kv.count(k))
+        v = std::stod(kv[k]);
+    };
+    auto getb = [&](const char *k, bool &v) {
+      if (kv.count(k))
+        v = std::stoi(kv[k]) != 0;
+    };
+    geti("dim", ps.dim);
+    if (kv.count("subdiv"))
+      {
+        std::stringstream s2(kv["subdiv"]);
+        for (int d = 0; d < 3 && std::getline(s2, tok, ','); ++d)
+          ps.subdiv[d] = std::stoi(tok);
+      }
+    if (kv.count("lo"))
+      {
+        std::stringstream s2(kv["lo"]);
+        for (int d = 0; d < 3 && std::getline(s2, tok, ','); ++d)
+          ps.lo[d] = std::stod(tok);
+      }
+    if (kv.count("hi"))
+      {
+        std::stringstream s2(kv["hi"]);
+        for (int d = 0; d < 3 && std::getline(s2, tok, ','); ++d)
+          ps.hi[d] = std::stod(tok);
+      }
+    geti("refine", ps.refine);
+    geti("p", ps.p);
+    if (kv.count("type"))
+      ps.type = (kv["type"] == "CGP" || kv["type"] == "1") ? TimeStepType::CGP : TimeStepType::DG;
+    geti("r", ps.r);
+    geti("nts", ps.nts);
+    getd("tau", ps.tau);
+    geti("space_time_mg", ps.space_time_mg);
+    geti("nts_min", ps.nts_min);
+    geti("r_min", ps.r_min);
+    geti("time_before_space", ps.time_before_space);
+    geti("mg_space", ps.mg_space);
+    geti("mg_float", ps.mg_float);
+    geti("dirichlet", ps.dirichlet);
+    geti("smoothing_steps", ps.mg.smoothing_steps);
+    getb("variable", ps.mg.variable);
+    getb("estimate_relaxation", ps.mg.estimate_relaxation);
+    geti("eig_iters", ps.mg.eig_iterations);
+    getd("eig_safety", ps.mg.eig_safety);
+    getb("overwrite_transfer", ps.mg.overwrite_time_transfer);
+    getb("vanka_factored", ps.mg.vanka_factored);
+    geti("maxiter", ps.solver.maxiter);
+    getd("abstol", ps.solver.abstol);
+    getd("reltol", ps.solver.reltol);
+    geti("basis", ps.solver.basis);
+    geti("extrapolate", ps.extrapolate);
+    getd("end_time", ps.end_time);
+    getd("frequency", ps.frequency);
+    geti("source", ps.source);
+    geti("n_bumps", ps.n_bumps);
+    if (kv.count("seed"))
+      ps.seed = std::stol(kv["seed"]);
+    // Parameters::parse derivations (gmg.h:637-648)
+    if (ps.nts_min == -1)
+      ps.nts_min = ps.nts / 2;
+    ps.nts_min = std::clamp(ps.nts_min, 1, std::max(1, ps.nts));
+    const int lowest = ps.type == TimeStepType::DG ? 0 : 1;
+    if (ps.r_min == -1)
+      ps.r_min = ps.r - 1;
+    ps.r_min = std::clamp(ps.r_min, lowest, std::max(lowest, ps.r));
+    if (ps.p < 0)
+      ps.p = ps.r + 1;
+    STFEM_REQUIRE(ps.dim == 2 || ps.dim == 3, "spec: dim must be 2 or 3");
+    STFEM_REQUIRE(ps.p >= 1 && ps.p < MAX_N, "spec: space degree must be in [1, 4]");
+    STFEM_REQUIRE(ps.r >= (ps.type == TimeStepType::DG ? 0 : 1), "spec: bad time degree");
+    STFEM_REQUIRE(ps.nts >= 1 && (ps.nts & (ps.nts - 1)) == 0,
+                  "spec: n_timesteps_at_once must be a power of two");
+    return ps;
+  }
+
+  // ================================================================ Problem
+  namespace
+  {
+    template <typename T>
+    class Problem final : public ProblemBase
+    {
+    public:
+      Problem(const ProblemSpec &s, const double *vertices, const double *kappa, int device)
+        : ctx_(device)
+        , sh_(s.p)
+      {
+        spec = s;
+        mesh_ = build_mesh_levels(spec.dim, spec.subdiv, spec.refine, spec.p, vertices, spec.lo,
+                                  spec.hi, kappa, spec.dirichlet);
+        // time step (tp_01.cc:87, 124-127) unless given
+        if (spec.tau > 0)
+          tau = spec.tau;
+        else
+          {
+            double diag = 0;
+            for (int d = 0; d < spec.dim; ++d)
+              {
+                const double h = (spec.hi[d] - spec.lo[d]) / spec.subdiv[d];
+                diag += h * h;
+              }
+            const double   spc    = std::sqrt(diag) / std::sqrt(double(spec.dim));
+            const unsigned nsteps = unsigned(spec.end_time / spc);
+            STFEM_REQUIRE(nsteps >= 1, "spec: end time shorter than the coarse step");
+            tau = spec.end_time * std::pow(2.0, -(spec.refine + 1)) / nsteps;
+          }
+        // level sequence (tp_01.cc:205-219)
+        const int n_sp = spec.mg_space ? spec.refine + 1 : 1;
+        const int rmin = spec.space_time_mg ? spec.r_min : spec.r;
+        const int tmin = spec.space_time_mg ? std::max(spec.nts_min, 1) : spec.nts;
+        const auto seq = get_time_mg_sequence(n_sp, spec.r, rmin, spec.nts, tmin,
+                                              spec.time_before_space != 0);
+        const int nlev = int(seq.size()) + 1;
+        std::vector<int>    sref(nlev), rl(nlev), nl(nlev);
+        std::vector<double> tl(nlev);
+        sref[nlev - 1] = spec.refine;
+        rl[nlev - 1]   = spec.r;
+        nl[nlev - 1]   = spec.nts;
+        tl[nlev - 1]   = tau;
+        for (int l = nlev - 2; l >= 0; --l)
+          {
+            const TimeMGType t = seq[l];
+            sref[l] = sref[l + 1] - ((t == TimeMGType::none && spec.mg_space) ? 1 : 0);
+            rl[l]   = rl[l + 1] - (t == TimeMGType::k ? 1 : 0);
+            nl[l]   = nl[l + 1] / (t == TimeMGType::tau ? 2 : 1);
+            tl[l]   = tl[l + 1] * (t == TimeMGType::tau ? 2.0 : 1.0);
+          }
+        std::vector<GMGLevel<T>> levels(nlev);
+        const auto form = spec.mg.vanka_factored ? PreconditionVanka<T>::Form::factored :
+                                                   PreconditionVanka<T>::Form::explicit_inverse;
+        for (int l = 0; l < nlev; ++l)
+          {
+            const TimeWeights w = get_fe_time_weights(spec.type, rl[l], tl[l], nl[l]);
+            const MeshLevel  &m = mesh_[sref[l]];
+            levels[l].A  = std::make_unique<SystemMatrix<T>>(ctx_, m, w.Alpha, w.Beta);
+            levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form);
+            levels[l].r  = rl[l];
+            levels[l].nts = nl[l];
+            levels[l].type_to_here = l > 0 ? seq[l - 1] : TimeMGType::none;
+          }
+        std::vector<TwoLevelTransfer<T>> tr(nlev);
+        for (int l = 1; l < nlev; ++l)
+          {
+            const TimeMGType t = seq[l - 1];
+            if (t == TimeMGType::none)
+              tr[l].space = std::make_unique<MGTwoLevelBlockTransfer<T>>(
+                ctx_, mesh_[sref[l]].g, mesh_[sref[l - 1]].g, sh_, levels[l].A->n_blocks());
+            else
+              tr[l].time = std::make_unique<MGTwoLevelTransferST<T>>(
+                ctx_, spec.type, rl[l], nl[l], t, mesh_[sref[l]].g.n_nodes,
+                spec.mg.overwrite_time_transfer);
+          }
+        gmg_ = std::make_unique<GMG<T>>(ctx_, std::move(levels), std::move(tr), spec.mg);
+        // FP64 fine system and rhs matrices (tp_01.cc:152-198, heat)
+        const MeshLevel  &F = mesh_.back();
+        const TimeWeights w = get_fe_time_weights(spec.type, spec.r, tau, spec.nts);
+        nt                  = w.Alpha.m;
+        fine_ = std::make_unique<SystemMatrix<double>>(ctx_, F, w.Alpha, w.Beta);
+        DMat zero(w.Gamma.m, 1);
+        const DMat rK = (spec.type == TimeStepType::CGP) ? w.Gamma : zero;
+        const DMat rM = (spec.type == TimeStepType::CGP) ? w.Zeta : w.Gamma;
+        rhs_ = std::make_unique<SystemMatrix<double>>(ctx_, F, rK, rM);
+        const TimeWeights w1 = get_fe_time_weights(spec.type, spec.r, tau, 1);
+        alpha1_ = w1.Alpha;
+        gamma1_ = w1.Gamma;
+        solver_ = std::make_unique<SolverFGMRES>(ctx_, fine_->m(), spec.solver);
+        rhs_vec_.alloc(size_t(fine_->m()));
+        coefs_.alloc(64);
+        ones_.alloc(64);
+        {
+          std::vector<double> o(64, 1.0);
+          ones_.upload(o.data(), 64, ctx_.stream);
+        }
+        if (spec.source == 0)
+          build_load([this](const double *x) {
+            const double PI = 3.14159265358979323846, f = spec.frequency;
+            double       hv = 1.0;
+            for (int d = 0; d < spec.dim; ++d)
+              hv *= std::sin(2 * PI * f * x[d]);
+            return hv;
+          });
+        else if (spec.source == 3)
+          {
+            // seeded Gaussian bumps: centres uniform in [0.2, 0.8]^dim,
+            // amplitudes uniform in [0.5, 1.5], width 0.1
+            std::mt19937_64                  rng(uint64_t(spec.seed));
+            std::uniform_real_distribution<> Uc(0.2, 0.8), Ua(0.5, 1.5);
+            std::vector<std::array<double, 4>> bumps(std::max(1, spec.n_bumps));
+            for (auto &b : bumps)
+              {
+                for (int d = 0; d < 3; ++d)
+                  b[d] = Uc(rng);
+                b[3] = Ua(rng);
+              }
+            const double s2 = 2.0 * 0.1 * 0.1;
+            build_load([&](const double *x) {
+              double v = 0.0;
+              for (const auto &b : bumps)
+                {
+                  double r2 = 0.0;
+                  for (int d = 0; d < spec.dim; ++d)
+                    r2 += (x[d] - b[d]) * (x[d] - b[d]);
+                  v += b[3] * std::exp(-r2 / s2);
+                }
+              return v;
+            });
+          }
+        ctx_.sync();
+      }
+
+      int n_levels() const override { return gmg_->n_levels(); }
+      void level_info(int l, long *info) const override
+      {
+        const auto &L = gmg_->level(l);
+        const auto &g = L.A->grid();
+        info[0] = g.n_nodes;
+        info[1] = L.A->n_blocks();
+        info[2] = g.n_cells;
+        info[3] = L.r;
+        info[4] = L.nts;
+        info[5] = l > 0 ? int(L.type_to_here) : 0;
+        info[6] = g.nc[0];
+        info[7] = g.nc[1];
+        info[8] = g.nc[2];
+        info[9] = sizeof(T);
+      }
+      double level_omega(int l) const override { return gmg_->level(l).omega; }
+      void level_vmult(int l, void *dst, const void *src, bool transpose) override
+      {
+        const auto &L = gmg_->level(l);
+        if (transpose)
+          L.A->Tvmult((T *)dst, (const T *)src);
+        else
+          L.A->vmult((T *)dst, (const T *)src);
+      }
+      void level_vanka(int l, void *dst, const void *src) override
+      {
+        gmg_->level(l).V->vmult((T *)dst, (const T *)src);
+      }
+      void level_prolongate(int l, void *dst, const void *src) override
+      {
+        STFEM_REQUIRE(l >= 1 && l < n_levels(), "prolongate: bad level");
+        gmg_->transfer(l).prolongate_and_add((T *)dst, (const T *)src);
+      }
+      void level_restrict(int l, void *dst, const void *src) override
+      {
+        STFEM_REQUIRE(l >= 1 && l < n_levels(), "restrict: bad level");
+        gmg_->transfer(l).restrict_and_add((T *)dst, (const T *)src);
+      }
+      void fine_vmult(double *dst, const double *src, bool transpose) override
+      {
+        transpose ? fine_->Tvmult(dst, src) : fine_->vmult(dst, src);
+      }
+      void gmg_vmult(double *dst, const double *src) override { gmg_->vmult(dst, src); }
+      int  fgmres(double *x, const double *b, int *its, std::vector<double> *hist) override
+      {
+        const int rc = solver_->solve(*fine_, *gmg_, x, b);
+        if (its)
+          *its = solver_->last_step();
+        if (hist)
+          *hist = solver_->history();
+        return rc;
+      }
+      // TimeIntegratorHeat::solve (time_integrators.h:157-180)
+      int solve_slab(double *x, const double *prev, double time, int *its) override
+      {
+        const long N  = fine_->n_nodes();
+        double    *rh = rhs_vec_.get();
+        STFEM_HIP(hipMemsetAsync(rh, 0, size_t(N) * nt * sizeof(double), ctx_.stream));
+        rhs_->vmult_add_nx1(rh, prev);
+        if (spec.source != 1 && load_.size())
+          {
+            // assemble_force (time_integrators.h:60-87): source = g(t) * h(x)
+            const LagrangeBasis tb = get_time_basis(spec.type, spec.r);
+            const int           ni = alpha1_.m;
+            std::vector<double> c(nt, 0.0);
+            for (int it = 0; it < spec.nts; ++it)
+              for (int j = 0; j < tb.size(); ++j)
+                {
+                  const double t = time + tau * it + tau * tb.nodes[j];
+                  const double g = source_time(t);
+                  if (spec.type == TimeStepType::DG)
+                    c[j + it * ni] += alpha1_(j, j) * g;
+                  else if (j == 0)
+                    for (int i = 0; i < gamma1_.m; ++i)
+                      c[i + it * ni] += -gamma1_(i, 0) * g;
+                  else
+                    c[j - 1 + it * ni] += alpha1_(j - 1, j - 1) * g;
+                }
+            coefs_.upload(c.data(), c.size(), ctx_.stream);
+            launch_time_transfer<double>(N, coefs_.get(), nt, 1, load_.get(), rh, true,
+                                         ctx_.stream);
+          }
+        // extrapolate: every block starts from prev
+        launch_time_transfer<double>(N, ones_.get(), nt, 1, prev, x, false, ctx_.stream);
+        const int rc = solver_->solve(*fine_, *gmg_, x, rh);
+        launch_zero_dirichlet<double>(fine_->grid(), nt, x, ctx_.stream);
+        if (its)
+          *its = solver_->last_step();
+        return rc;
+      }
+      void set_load_vector(const double *h) override
+      {
+        load_.upload(h, size_t(fine_->n_nodes()), ctx_.stream);
+        ctx_.sync();
+      }
+      long     n_nodes() const override { return fine_->n_nodes(); }
+      Context &context() override { return ctx_; }
+      double   fine_vmult_bytes() const override { return fine_->vmult_bytes(); }
+      double   vanka_sweep_bytes(int l) const override
+      {
+        return gmg_->level(l).V->sweep_bytes();
+      }
+      const std::vector<double> &history() const { return solver_->history(); }
+
+    private:
+      // time profile g(t) of the source f(t,x) = g(t) h(x): the manufactured
+      // heat source of exact_solution.h:53-75 splits this way; the other
+      // sources are constant in time
+      double source_time(double t) const
+      {
+        if (spec.source != 0)
+          return 1.0;
+        const double PI = 3.14159265358979323846, f = spec.frequency;
+        return spec.dim * 4 * PI * PI * f * f * std::sin(2 * PI * f * t) +
+               2 * PI * f * std::cos(2 * PI * f * t);
+      }
+      // load vector h_i = int h(x) phi_i with Gauss(p+1) quadrature on the
+      // mapped cells (VectorTools::create_right_hand_side), Dirichlet rows 0
+      template <typename F>
+      void build_load(F hfun)
+      {
+        const MeshLevel &m   = mesh_.back();
+        const GridDesc  &g   = m.g;
+        const int        dim = g.dim, n = sh_.n, nq = n;
+        std::vector<double> h(g.n_nodes, 0.0);
+        const long          nv0 = g.nc[0] + 1, nv1 = g.nc[1] + 1;
+        for (long c = 0; c < g.n_cells; ++c)
+          {
+            int cx, cy, cz;
+            cell_xyz(g, c, cx, cy, cz);
+            double V[8][3] = {};
+            for (int v = 0; v < (1 << dim); ++v)
+              {
+                const long id =
+                  (cx + (v & 1)) + nv0 * ((cy + ((v >> 1) & 1)) + nv1 * (cz + ((v >> 2) & 1)));
+                for (int d = 0; d < dim; ++d)
+                  V[v][d] = m.X[id * dim + d];
+              }
+            for (int qz = 0; qz < ((dim == 3) ? nq : 1); ++qz)
+              for (int qy = 0; qy < nq; ++qy)
+                for (int qx = 0; qx < nq; ++qx)
+                  {
+                    const double xi[3] = {sh_.qx[qx], sh_.qx[qy], dim == 3 ? sh_.qx[qz] : 0.0};
+                    double       x[3] = {0, 0, 0}, J[3][3] = {};
+                    for (int v = 0; v < (1 << dim); ++v)
+                      {
+                        double N0 = 1;
+                        for (int e = 0; e < dim; ++e)
+                          N0 *= ((v >> e) & 1) ? xi[e] : 1 - xi[e];
+                        for (int e = 0; e < dim; ++e)
+                          {
+                            double gg = 1;
+                            for (int ff = 0; ff < dim; ++ff)
+                              {
+                                const int bit = (v >> ff) & 1;
+                                gg *= (ff == e) ? (bit ? 1.0 : -1.0) : (bit ? xi[ff] : 1 - xi[ff]);
+                              }
+                            for (int d = 0; d < dim; ++d)
+                              J[d][e] += gg * V[v][d];
+                          }
+                        for (int d = 0; d < dim; ++d)
+                          x[d] += N0 * V[v][d];
+                      }
+                    double det;
+                    if (dim == 2)
+                      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+                    else
+                      det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                    const double w =
+                      sh_.qw[qx] * sh_.qw[qy] * (dim == 3 ? sh_.qw[qz] : 1.0) * std::abs(det);
+                    const double hv = hfun(x);
+                    for (int c3 = 0; c3 < ((dim == 3) ? n : 1); ++c3)
+                      for (int b = 0; b < n; ++b)
+                        for (int a = 0; a < n; ++a)
+                          {
+                            const long gx = (long)cx * sh_.p + a, gy = (long)cy * sh_.p + b,
+                                       gz = (long)cz * sh_.p + c3;
+                            if (node_constrained(g, gx, gy, gz))
+                              continue;
+                            const double phi = sh_.S[qx * n + a] * sh_.S[qy * n + b] *
+                                               (dim == 3 ? sh_.S[qz * n + c3] : 1.0);
+                            h[gx + g.nn[0] * (gy + g.nn[1] * gz)] += hv * phi * w;
+                          }
+                  }
+          }
+        load_.upload(h.data(), h.size(), ctx_.stream);
+      }
+
+      Context                               ctx_;
+      Shape1D                               sh_;
+      std::vector<MeshLevel>                mesh_;
+      std::unique_ptr<GMG<T>>               gmg_;
+      std::unique_ptr<SystemMatrix<double>> fine_, rhs_;
+      std::unique_ptr<SolverFGMRES>         solver_;
+      DMat                                  alpha1_, gamma1_;
+      DBuf<double>                          rhs_vec_, load_, coefs_, ones_;
+    };
+  } // namespace
+
+  std::unique_ptr<ProblemBase> make_problem(const std::string &spec, const double *vertices,
+                                            const double *kappa, int device)
+  {
+    const ProblemSpec ps = ProblemSpec::parse(spec);
+    if (ps.mg_float)
+      return std::make_unique<Problem<float>>(ps, vertices, kappa, device);
+    return std::make_unique<Problem<double>>(ps, vertices, kappa, device);
+  }
+
+  template class SystemMatrix<double>;
+  template class SystemMatrix<float>;
+  template class PreconditionVanka<double>;
+  template class PreconditionVanka<float>;
+  template class MGTwoLevelTransferST<double>;
+  template class MGTwoLevelTransferST<float>;
+  template class MGTwoLevelBlockTransfer<double>;
+  template class MGTwoLevelBlockTransfer<float>;
+  template class GMG<double>;
+  template class GMG<float>;
+} // namespace stfem

@@ -1,0 +1,413 @@
+// Host C++ surface of the MI355X space-time multigrid slab solver.
+//
+// Class names and call semantics follow nlsmrg/dealii-stfem:
+//   SystemMatrix ............ tests/include/operators.h:71-256
+//   PreconditionVanka ....... tests/include/gmg.h:402-540
+//   MGTwoLevelTransferST .... tests/include/gmg.h:77-136
+//   MGTwoLevelBlockTransfer . tests/include/gmg.h:35-75 (space h-transfer)
+//   GMG ..................... tests/include/gmg.h:652-906
+//   SolverFGMRES + ReductionControl (deal.II) as used in
+//                             tests/include/time_integrators.h:21-120
+//   TimeIntegratorHeat ...... tests/include/time_integrators.h:123-181
+// Vectors live in device memory in node-major layout: the nt time values of
+// a node are contiguous (x[node * nt + t]).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "time_fe.h"
+
+namespace stfem
+{
+  // ------------------------------------------------------------- device mem
+  template <typename T>
+  class DBuf
+  {
+  public:
+    DBuf() = default;
+    explicit DBuf(size_t n) { alloc(n); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept
+      : p_(o.p_)
+      , n_(o.n_)
+    {
+      o.p_ = nullptr;
+      o.n_ = 0;
+    }
+    DBuf &operator=(DBuf &&o) noexcept
+    {
+      if (this != &o)
+        {
+          release();
+          p_   = o.p_;
+          n_   = o.n_;
+          o.p_ = nullptr;
+          o.n_ = 0;
+        }
+      return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t n)
+    {
+      release();
+      n_ = n;
+      if (n_)
+        STFEM_HIP(hipMalloc(&p_, n_ * sizeof(T)));
+    }
+    void release()
+    {
+      if (p_)
+        (void)hipFree(p_);
+      p_ = nullptr;
+      n_ = 0;
+    }
+    void upload(const T *h, size_t n, hipStream_t s)
+    {
+      if (n_ < n)
+        alloc(n);
+      if (n)
+        STFEM_HIP(hipMemcpyAsync(p_, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void zero(hipStream_t s)
+    {
+      if (n_)
+        STFEM_HIP(hipMemsetAsync(p_, 0, n_ * sizeof(T), s));
+    }
+    T       *get() { return p_; }
+    const T *get() const { return p_; }
+    size_t   size() const { return n_; }
+
+  private:
+    T     *p_ = nullptr;
+    size_t n_ = 0;
+  };
+
+  // --------------------------------------------------------------- context
+  struct Context
+  {
+    int              device = 0;
+    hipStream_t      stream = nullptr;
+    DBuf<double>     partial;  // reduction scratch
+    DBuf<double>     scalars;  // device scalars (Hessenberg column, norms)
+    explicit Context(int dev);
+    ~Context();
+    void sync();
+  };
+
+  // ------------------------------------------------------------ geometry
+  // FE_Q(p) on Gauss-Lobatto nodes with Gauss(p+1) quadrature.
+  struct Shape1D
+  {
+    int                 p = 1, n = 2;
+    std::vector<double> gll;            // nodes
+    std::vector<double> qx, qw;         // Gauss points/weights
+    std::vector<double> S, D, Sd;       // S[q][a], D[q][q'] (collocation), Sd[q][a] = phi_a'(x_q)
+    LagrangeBasis       L;
+    explicit Shape1D(int p_);
+  };
+
+  // One structured mesh level with its geometry and coefficient.
+  struct MeshLevel
+  {
+    GridDesc            g{};
+    std::vector<double> X;      // vertex coordinates
+    int                 coef_mode = 0;
+    double              coef_value = 1.0;
+    std::vector<double> coef;   // per cell (mode 1) or per (cell, q) (mode 2)
+  };
+
+  // Builds the level hierarchy of a subdivided box refined `refine` times:
+  // level s has subdiv * 2^s cells per direction and takes the finest
+  // level's vertex positions (coarse vertices are a subset).  A per-cell
+  // coefficient of the finest level is evaluated on coarser levels at the
+  // quadrature points (value of the finest cell containing the point).
+  std::vector<MeshLevel> build_mesh_levels(int dim, const int subdiv[3], int refine, int p,
+                                           const double *fine_vertices, const double *lo,
+                                           const double *hi, const double *fine_kappa,
+                                           int dirichlet_mask);
+
+  // Element mass / stiffness matrices of all cells (lexicographic local
+  // numbering, n_c x n_c each), and their assembly into the principal
+  // submatrices K[I_c, I_c], M[I_c, I_c] of the globally assembled matrices
+  // with constrained rows/columns reduced to the diagonal.
+  void cell_patch_matrices(const MeshLevel &m, const Shape1D &sh, std::vector<double> &Kp,
+                           std::vector<double> &Mp);
+
+  // ------------------------------------------------------ SystemMatrix
+  template <typename T>
+  class SystemMatrix
+  {
+  public:
+    SystemMatrix(Context &ctx, const MeshLevel &m, const DMat &Alpha, const DMat &Beta);
+    // dst = (Alpha x K + Beta x M) src   (operators.h:94-122)
+    void vmult(T *dst, const T *src) const;
+    // dst = (Alpha^T x K + Beta^T x M) src   (operators.h:124-152)
+    void Tvmult(T *dst, const T *src) const;
+    // dst(all blocks) += (Alpha(:,0) x K + Beta(:,0) x M) src(one block)
+    // for an nt x 1 weight pair (operators.h:154-187)
+    void vmult_add_nx1(T *dst, const T *src) const;
+    long        n_nodes() const { return grid_.n_nodes; }
+    int         n_blocks() const { return nt_out_; }
+    long        m() const { return grid_.n_nodes * nt_out_; }
+    const GridDesc &grid() const { return grid_; }
+    const DMat &alpha() const { return A_; }
+    const DMat &beta() const { return B_; }
+    // algorithmic bytes of one vmult (vectors, coefficient, vertices)
+    double      vmult_bytes() const;
+
+  private:
+    void apply(T *dst, const T *src, const DBuf<T> &a, const DBuf<T> &b, bool accumulate) const;
+    Context     &ctx_;
+    GridDesc     grid_;
+    DBuf<double> verts_;
+    int          coef_mode_;
+    T            coef_value_;
+    DBuf<T>      coef_;
+    DMat         A_, B_;
+    int          nt_in_, nt_out_;
+    DBuf<T>      dA_, dB_, dAt_, dBt_;
+  };
+
+  // ------------------------------------------------------ PreconditionVanka
+  template <typename T>
+  class PreconditionVanka
+  {
+  public:
+    enum class Form
+    {
+      explicit_inverse,
+      factored
+    };
+    PreconditionVanka(Context &ctx, const MeshLevel &m, const Shape1D &sh, const DMat &Alpha,
+                      const DMat &Beta, Form form);
+    // dst = sum_c (1/valence) B_c^{-1} src|_c   (gmg.h:476-514)
+    void   vmult(T *dst, const T *src) const;
+    Form   form() const { return form_; }
+    double sweep_bytes() const;
+    size_t storage_bytes() const { return (binv_.size() + S_.size() + W_.size()) * sizeof(T); }
+
+  private:
+    Context &ctx_;
+    GridDesc grid_;
+    int      nt_;
+    Form     form_;
+    DBuf<T>  binv_, S_, W_;
+  };
+
+  // ------------------------------------------------------ transfers
+  template <typename T>
+  class MGTwoLevelTransferST
+  {
+  public:
+    // gmg.h:86-115 with restrict_is_transpose_prolongate; `overwrite`
+    // reproduces the assigning transfer of tests/transfer_01.cc:42-83.
+    MGTwoLevelTransferST(Context &ctx, TimeStepType type, int r, int nts, TimeMGType kind,
+                         long n_nodes, bool overwrite = false);
+    void prolongate_and_add(T *dst, const T *src) const;
+    void restrict_and_add(T *dst, const T *src) const;
+    int  nt_fine() const { return P_.m; }
+    int  nt_coarse() const { return P_.n; }
+    const DMat &prolongation() const { return P_; }
+
+  private:
+    Context &ctx_;
+    DMat     P_;
+    long     n_nodes_;
+    bool     overwrite_;
+    DBuf<T>  dP_, dR_;
+  };
+
+  template <typename T>
+  class MGTwoLevelBlockTransfer
+  {
+  public:
+    MGTwoLevelBlockTransfer(Context &ctx, const GridDesc &fine, const GridDesc &coarse,
+                            const Shape1D &sh, int nt);
+    void prolongate_and_add(T *dst, const T *src) const;
+    void restrict_and_add(T *dst, const T *src) const;
+
+  private:
+    struct Ell
+    {
+      int          rows = 0, K = 0;
+      DBuf<int>    cnt, col;
+      DBuf<T>      w;
+    };
+    Context &ctx_;
+    GridDesc fine_, coarse_;
+    int      nt_;
+    Ell      P_[3], R_[3];
+    DBuf<T>  tmp1_, tmp2_;
+  };
+
+  // One two-level transfer: time (tau/k) or space (h), as the std::variant
+  // of TwoLevelTransferOperator (gmg.h:138-177).
+  template <typename T>
+  struct TwoLevelTransfer
+  {
+    std::unique_ptr<MGTwoLevelTransferST<T>>    time;
+    std::unique_ptr<MGTwoLevelBlockTransfer<T>> space;
+    void prolongate_and_add(T *dst, const T *src) const
+    {
+      time ? time->prolongate_and_add(dst, src) : space->prolongate_and_add(dst, src);
+    }
+    void restrict_and_add(T *dst, const T *src) const
+    {
+      time ? time->restrict_and_add(dst, src) : space->restrict_and_add(dst, src);
+    }
+  };
+
+  // ------------------------------------------------------------ GMG
+  struct GMGOptions
+  {
+    int    smoothing_steps         = 1;    // PreconditionRelaxation n_iterations
+    bool   variable                = true; // MGSmootherPrecondition steps 2^(L-l)
+    bool   estimate_relaxation     = true;
+    int    eig_iterations          = 20;
+    double eig_safety              = 1.15;
+    bool   overwrite_time_transfer = false;
+    bool   vanka_factored          = false;
+  };
+
+  // Level data of the hierarchy (precision T).
+  template <typename T>
+  struct GMGLevel
+  {
+    std::unique_ptr<SystemMatrix<T>>      A;
+    std::unique_ptr<PreconditionVanka<T>> V;
+    double                                omega = 1.0;
+    int                                   r = 0, nts = 1;
+    TimeMGType                            type_to_here = TimeMGType::none;
+  };
+
+  // Geometric space-time multigrid V-cycle used as FGMRES preconditioner
+  // (gmg.h:652-906): Vanka relaxation smoother with variable step counts,
+  // coarse level solved by the smoother, FP64 <-> T conversion at the
+  // boundary (gmg.h:846-859).
+  template <typename T>
+  class GMG
+  {
+  public:
+    GMG(Context &ctx, std::vector<GMGLevel<T>> &&levels,
+        std::vector<TwoLevelTransfer<T>> &&transfers, const GMGOptions &opt);
+    void   vmult(double *dst, const double *src) const;
+    // one V-cycle on level vectors (def[L] given, sol[L] returned)
+    void   vcycle() const;
+    double estimate_omega(int level) const;
+    int    n_levels() const { return int(lv_.size()); }
+    const GMGLevel<T> &level(int l) const { return lv_[l]; }
+    TwoLevelTransfer<T> &transfer(int l) { return tr_[l]; }
+    T *level_defect(int l) const { return def_[l].get(); }
+    T *level_solution(int l) const { return sol_[l].get(); }
+
+  private:
+    void relax(int l, T *u, const T *rhs, bool first) const;
+    void smoother_apply(int l, T *u, const T *rhs) const;
+    void smoother_smooth(int l, T *u, const T *rhs) const;
+    void level_v_step(int l) const;
+    int  steps(int l) const;
+
+    Context                         &ctx_;
+    std::vector<GMGLevel<T>>         lv_;
+    std::vector<TwoLevelTransfer<T>> tr_;
+    GMGOptions                       opt_;
+    mutable std::vector<DBuf<T>>     sol_, def_, t1_, t2_;
+  };
+
+  // ------------------------------------------------------------ FGMRES
+  struct SolverControlData
+  {
+    int    maxiter = 200;
+    double abstol  = 1e-12;
+    double reltol  = 1e-12;
+    int    basis   = 30;
+  };
+
+  // Right-preconditioned flexible GMRES (deal.II SolverFGMRES semantics:
+  // modified Gram-Schmidt, projected least-squares solve once two basis
+  // vectors exist, ReductionControl on the projected residual).
+  class SolverFGMRES
+  {
+  public:
+    SolverFGMRES(Context &ctx, long n, const SolverControlData &c);
+    template <typename Op, typename Prec>
+    int solve(const Op &A, const Prec &P, double *x, const double *b);
+    int                        last_step() const { return last_step_; }
+    const std::vector<double> &history() const { return hist_; }
+
+  private:
+    Context                  &ctx_;
+    long                      n_;
+    SolverControlData         c_;
+    std::vector<DBuf<double>> v_, z_;
+    DBuf<double>              aux_;
+    int                       last_step_ = 0;
+    std::vector<double>       hist_;
+  };
+
+  // ------------------------------------------------------------ problem
+  // Everything tp_01.cc:56-359 sets up for one (refinement, degree) pair:
+  // mesh hierarchy, FP64 system and rhs matrices, MG levels, transfers, GMG,
+  // FGMRES and the heat time integrator.
+  struct ProblemSpec
+  {
+    int          dim = 2;
+    int          subdiv[3] = {1, 1, 1};
+    double       lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    int          refine = 2;
+    int          p = -1;
+    TimeStepType type = TimeStepType::DG;
+    int          r = 1;
+    int          nts = 1;
+    double       tau = -1;
+    int          space_time_mg = 1;
+    int          nts_min = -1, r_min = -1;
+    int          time_before_space = 0;
+    int          mg_space = 1;
+    int          mg_float = 0;
+    int          dirichlet = 63;
+    GMGOptions   mg;
+    SolverControlData solver;
+    int          extrapolate = 1;
+    double       end_time = 1.0;
+    double       frequency = 1.0;
+    int          source = 0; // 0 manufactured heat source, 1 none, 2 load vector given,
+                             // 3 seeded Gaussian bumps
+    int          n_bumps = 4;
+    long         seed    = 1;
+    static ProblemSpec parse(const std::string &s);
+  };
+
+  class ProblemBase
+  {
+  public:
+    virtual ~ProblemBase() = default;
+    ProblemSpec spec;
+    double      tau = 0;
+    int         nt  = 1;
+    virtual int  n_levels() const = 0;
+    virtual void level_info(int l, long *info) const = 0;
+    virtual double level_omega(int l) const = 0;
+    virtual void level_vmult(int l, void *dst, const void *src, bool transpose) = 0;
+    virtual void level_vanka(int l, void *dst, const void *src) = 0;
+    virtual void level_prolongate(int l, void *dst, const void *src) = 0;
+    virtual void level_restrict(int l, void *dst, const void *src) = 0;
+    virtual void fine_vmult(double *dst, const double *src, bool transpose) = 0;
+    virtual void gmg_vmult(double *dst, const double *src) = 0;
+    virtual int  fgmres(double *x, const double *b, int *its, std::vector<double> *hist) = 0;
+    virtual int  solve_slab(double *x, const double *prev, double time, int *its) = 0;
+    virtual void set_load_vector(const double *h_host) = 0;
+    virtual long n_nodes() const = 0;
+    virtual Context &context() = 0;
+    virtual double fine_vmult_bytes() const = 0;
+    virtual double vanka_sweep_bytes(int l) const = 0;
+  };
+
+  std::unique_ptr<ProblemBase> make_problem(const std::string &spec, const double *vertices,
+                                            const double *kappa, int device);
+} // namespace stfem

@@ -1,0 +1,272 @@
+"""ctypes binding of libstfem_amd.so (C ABI declared in include/stfem.h).
+
+Python mirror of the reference's host interface for the space-time slab
+solve (nlsmrg/dealii-stfem tests/include/{fe_time,operators,gmg,
+time_integrators}.h).  Device vectors are torch tensors on the problem's
+device; the library does the arithmetic in hand-written HIP kernels.  There
+is no CPU fallback: every call fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libstfem_amd.so")
+_LIB = None
+
+DG = 2
+CGP = 1
+FINE = -1  # level index of the FP64 fine system
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+_vp = ctypes.c_void_p
+
+# exported entry points and their argument types
+_SIGS = {
+    "stfem_version": ([], ctypes.c_int),
+    "stfem_last_error": ([], ctypes.c_char_p),
+    "stfem_fe_time_weights": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                               _dp, _dp, _dp, _dp], ctypes.c_int),
+    "stfem_dg_weights": ([ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
+    "stfem_cg_weights": ([ctypes.c_int, _dp, _dp], ctypes.c_int),
+    "stfem_fe_time_weights_wave": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                    _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "stfem_time_prolongation": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
+    "stfem_time_restriction": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
+    "stfem_time_projection": ([ctypes.c_int] * 4 + [_dp, _ip, _ip], ctypes.c_int),
+    "stfem_quadrature": ([ctypes.c_int, ctypes.c_int, _dp, _dp], ctypes.c_int),
+    "stfem_time_mg_sequence": ([ctypes.c_int] * 6 + [_ip, _ip], ctypes.c_int),
+    "stfem_problem_create": ([ctypes.c_char_p, _dp, _dp, ctypes.c_int,
+                              ctypes.POINTER(_vp)], ctypes.c_int),
+    "stfem_problem_destroy": ([_vp], ctypes.c_int),
+    "stfem_problem_info": ([_vp, _lp], ctypes.c_int),
+    "stfem_problem_tau": ([_vp, _dp], ctypes.c_int),
+    "stfem_level_info": ([_vp, ctypes.c_int, _lp], ctypes.c_int),
+    "stfem_level_omega": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "stfem_st_vmult": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "stfem_st_tvmult": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "stfem_vanka_vmult": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "stfem_prolongate_and_add": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "stfem_restrict_and_add": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "stfem_gmg_vmult": ([_vp, _vp, _vp], ctypes.c_int),
+    "stfem_fgmres_solve": ([_vp, _vp, _vp, _ip, _dp, ctypes.c_int], ctypes.c_int),
+    "stfem_solve_slab": ([_vp, _vp, _vp, ctypes.c_double, _ip], ctypes.c_int),
+    "stfem_set_load_vector": ([_vp, _dp], ctypes.c_int),
+    "stfem_bytes_model": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
+    "stfem_blocks_to_nodes": ([ctypes.c_long, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "stfem_nodes_to_blocks": ([ctypes.c_long, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "stfem_time_kernel": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
+                          ctypes.c_int),
+    "stfem_time_slabs": ([_vp, ctypes.c_int, _dp, _ip], ctypes.c_int),
+}
+
+
+class StfemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"stfem error {code}: {msg}")
+        self.code = code
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run python dealii-stfem_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise StfemError(rc, load().stfem_last_error().decode())
+    return rc
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (contiguous)."""
+    assert t.is_contiguous()
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def nt_intvl(type_, r):
+    return r if type_ == CGP else r + 1
+
+
+# ------------------------------------------------------------------ time FE
+def fe_time_weights(type_, r, tau, nts=1):
+    n = nt_intvl(type_, r) * nts
+    A, B, G, Z = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, 1)), np.zeros((n, 1))
+    _check(load().stfem_fe_time_weights(type_, r, tau, nts, _p(A), _p(B), _p(G), _p(Z)))
+    return A, B, G, Z
+
+
+def dg_weights(r):
+    m, d, j = np.zeros((r + 1, r + 1)), np.zeros((r + 1, r + 1)), np.zeros((r + 1, 1))
+    _check(load().stfem_dg_weights(r, _p(m), _p(d), _p(j)))
+    return m, d, j
+
+
+def cg_weights(r):
+    m, d = np.zeros((r, r + 1)), np.zeros((r, r + 1))
+    _check(load().stfem_cg_weights(r, _p(m), _p(d)))
+    return m, d
+
+
+def fe_time_weights_wave(type_, r, tau, nts=1):
+    n = nt_intvl(type_, r) * nts
+    out = [np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, 1)), np.zeros((n, 1)),
+           np.zeros((n, 1))]
+    _check(load().stfem_fe_time_weights_wave(type_, r, tau, nts, *[_p(o) for o in out]))
+    return out
+
+
+def time_prolongation(type_, r, nts=2):
+    ni = nt_intvl(type_, r)
+    P = np.zeros((ni * nts, ni * nts // 2))
+    _check(load().stfem_time_prolongation(type_, r, nts, _p(P)))
+    return P
+
+
+def time_restriction(type_, r, nts=2):
+    ni = nt_intvl(type_, r)
+    R = np.zeros((ni * nts // 2, ni * nts))
+    _check(load().stfem_time_restriction(type_, r, nts, _p(R)))
+    return R
+
+
+def time_projection(type_, r_src, r_dst, nts=1):
+    rows, cols = ctypes.c_int(), ctypes.c_int()
+    _check(load().stfem_time_projection(type_, r_src, r_dst, nts, None, ctypes.byref(rows),
+                                        ctypes.byref(cols)))
+    P = np.zeros((rows.value, cols.value))
+    _check(load().stfem_time_projection(type_, r_src, r_dst, nts, _p(P), ctypes.byref(rows),
+                                        ctypes.byref(cols)))
+    return P
+
+
+def quadrature(kind, n):
+    x, w = np.zeros(n), np.zeros(n)
+    _check(load().stfem_quadrature(kind, n, _p(x), _p(w)))
+    return x, w
+
+
+def time_mg_sequence(n_sp, kmax, kmin, nts, nts_min, time_before_space=False):
+    n = ctypes.c_int()
+    _check(load().stfem_time_mg_sequence(n_sp, kmax, kmin, nts, nts_min, int(time_before_space),
+                                         None, ctypes.byref(n)))
+    out = (ctypes.c_int * max(n.value, 1))()
+    _check(load().stfem_time_mg_sequence(n_sp, kmax, kmin, nts, nts_min, int(time_before_space),
+                                         out, ctypes.byref(n)))
+    return [out[i] for i in range(n.value)]
+
+
+# ------------------------------------------------------------------ problem
+def spec_string(spec):
+    def fmt(v):
+        if isinstance(v, (list, tuple)):
+            return ",".join(str(x) for x in v)
+        return str(v)
+    return ";".join(f"{k}={fmt(v)}" for k, v in spec.items())
+
+
+class Problem:
+    """Hierarchy + solver built from a spec dict (keys listed in include/stfem.h)."""
+
+    def __init__(self, spec, vertices=None, kappa=None, device=0):
+        self._v = None if vertices is None else np.ascontiguousarray(vertices, np.float64)
+        self._k = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+        h = ctypes.c_void_p()
+        _check(load().stfem_problem_create(spec_string(spec).encode(), _p(self._v), _p(self._k),
+                                           device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+        info = np.zeros(6, np.int64)
+        _check(load().stfem_problem_info(self.h, info.ctypes.data_as(_lp)))
+        self.n_levels, self.nt, self.n_nodes, self.n_cells, self.level_size, self.nts = \
+            [int(x) for x in info]
+        tau = ctypes.c_double()
+        _check(load().stfem_problem_tau(self.h, ctypes.byref(tau)))
+        self.tau = tau.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().stfem_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def level_info(self, l):
+        info = np.zeros(10, np.int64)
+        _check(load().stfem_level_info(self.h, l, info.ctypes.data_as(_lp)))
+        keys = ["n_nodes", "nt", "n_cells", "r", "nts", "mg_type", "nc0", "nc1", "nc2",
+                "number_size"]
+        return dict(zip(keys, [int(x) for x in info]))
+
+    def omega(self, l):
+        w = ctypes.c_double()
+        _check(load().stfem_level_omega(self.h, l, ctypes.byref(w)))
+        return w.value
+
+    def vmult(self, level, dst, src):
+        _check(load().stfem_st_vmult(self.h, level, _ptr(dst), _ptr(src)))
+
+    def Tvmult(self, level, dst, src):
+        _check(load().stfem_st_tvmult(self.h, level, _ptr(dst), _ptr(src)))
+
+    def vanka(self, level, dst, src):
+        _check(load().stfem_vanka_vmult(self.h, level, _ptr(dst), _ptr(src)))
+
+    def prolongate_and_add(self, level, dst, src):
+        _check(load().stfem_prolongate_and_add(self.h, level, _ptr(dst), _ptr(src)))
+
+    def restrict_and_add(self, level, dst, src):
+        _check(load().stfem_restrict_and_add(self.h, level, _ptr(dst), _ptr(src)))
+
+    def gmg_vmult(self, dst, src):
+        _check(load().stfem_gmg_vmult(self.h, _ptr(dst), _ptr(src)))
+
+    def fgmres(self, x, b, hist_cap=256):
+        its = ctypes.c_int()
+        hist = np.zeros(hist_cap)
+        rc = load().stfem_fgmres_solve(self.h, _ptr(x), _ptr(b), ctypes.byref(its), _p(hist),
+                                       hist_cap)
+        return rc, its.value, hist[: its.value + 1]
+
+    def solve_slab(self, x, prev, time):
+        its = ctypes.c_int()
+        rc = load().stfem_solve_slab(self.h, _ptr(x), _ptr(prev), float(time), ctypes.byref(its))
+        return rc, its.value
+
+    def set_load_vector(self, h):
+        h = np.ascontiguousarray(h, np.float64)
+        _check(load().stfem_set_load_vector(self.h, _p(h)))
+
+    def bytes_model(self, what, level=0):
+        b = ctypes.c_double()
+        _check(load().stfem_bytes_model(self.h, what, level, ctypes.byref(b)))
+        return b.value
+
+    def time_kernel(self, what, level=0, warmup=3, reps=20):
+        ms = ctypes.c_double()
+        _check(load().stfem_time_kernel(self.h, what, level, warmup, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def time_slabs(self, n_slabs):
+        ms = ctypes.c_double()
+        its = ctypes.c_int()
+        rc = load().stfem_time_slabs(self.h, n_slabs, ctypes.byref(ms), ctypes.byref(its))
+        _check(rc)
+        return ms.value, its.value

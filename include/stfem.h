@@ -1,0 +1,144 @@
+/* C ABI of the MI355X space-time multigrid slab solver (libstfem_amd.so).
+ *
+ * Drop-in boundary for the hot path of nlsmrg/dealii-stfem: the space-time
+ * operator, the Vanka smoother, the space/time multigrid transfers, the GMG
+ * V-cycle and the FGMRES slab solve.  Each entry point names the reference
+ * interface it replaces (file:line in nlsmrg/dealii-stfem).
+ *
+ * Conventions
+ *   - Status: 0 on success, negative STFEM_ERR_* on failure; the message
+ *     of the last failure on the calling thread is stfem_last_error().
+ *   - Vectors passed as void* / double* to operator entry points are DEVICE
+ *     pointers on the problem's device, in node-major layout: the nt time
+ *     values of a node are contiguous, x[node * nt + t].  Nodes are the
+ *     Gauss-Lobatto points of FE_Q(p) on the structured mesh, numbered
+ *     lexicographically (x fastest).  stfem_blocks_to_nodes converts the
+ *     reference's BlockVector layout (block t holds all nodes).
+ *   - MG level vectors use the level precision (float when the spec sets
+ *     mg_float=1, double otherwise); level -1 denotes the FP64 fine system.
+ *   - Calls are synchronous with respect to the host: inputs are read after
+ *     all prior device work completes and outputs are complete on return.
+ *   - A problem handle must not be used from two threads at once (the
+ *     reference is single-threaded per MPI rank).
+ */
+#ifndef STFEM_H
+#define STFEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STFEM_OK 0
+#define STFEM_ERR_ARG -1
+#define STFEM_ERR_HIP -2
+#define STFEM_ERR_NO_CONVERGENCE -3
+#define STFEM_ERR_UNSUPPORTED -4
+#define STFEM_ERR_NOT_FINITE -5
+#define STFEM_ERR_INTERNAL -6
+
+#define STFEM_TIME_CGP 1
+#define STFEM_TIME_DG 2
+
+typedef struct stfem_problem_s *stfem_problem;
+
+int         stfem_version(void);
+const char *stfem_last_error(void);
+
+/* ---- time finite element algebra (host, row-major outputs) ---------- */
+/* get_fe_time_weights, tests/include/fe_time.h:292-349.  Sizes: alpha and
+ * beta nt x nt, gamma and zeta nt x 1, nt = (r+1)*nts (DG) or r*nts (CGP). */
+int stfem_fe_time_weights(int type, int r, double tau, int nts, double *alpha, double *beta,
+                          double *gamma, double *zeta);
+/* get_dg_weights / get_cg_weights, fe_time.h:473-555 */
+int stfem_dg_weights(int r, double *mass, double *der, double *jump);
+int stfem_cg_weights(int r, double *mass, double *der);
+/* get_fe_time_weights_wave (single-step weights of degree r, tau scaled),
+ * fe_time.h:98-246 */
+int stfem_fe_time_weights_wave(int type, int r, double tau, int nts, double *lhs_uK,
+                               double *lhs_uM, double *rhs_uK, double *rhs_uM, double *rhs_vM);
+/* get_time_prolongation_matrix / get_time_restriction_matrix,
+ * fe_time.h:625-716: P is (ni*nts) x (ni*nts/2), R its shape transposed */
+int stfem_time_prolongation(int type, int r, int nts, double *P);
+int stfem_time_restriction(int type, int r, int nts, double *R);
+/* get_time_projection_matrix, fe_time.h:567-623; P may be NULL to query
+ * the shape */
+int stfem_time_projection(int type, int r_src, int r_dst, int nts, double *P, int *rows,
+                          int *cols);
+/* kind 0 Gauss, 1 Gauss-Lobatto, 2 Gauss-Radau (right end point) on [0,1] */
+int stfem_quadrature(int kind, int n, double *points, double *weights);
+/* get_time_mg_sequence, fe_time.h:43-69 (1 tau, 2 k, 3 none); out may be
+ * NULL to query the length */
+int stfem_time_mg_sequence(int n_sp_lvl, int k_max, int k_min, int nts, int nts_min,
+                           int time_before_space, int *out, int *n);
+
+/* ---- problem: hierarchy + solver ------------------------------------ */
+/* Builds what tests/tp_01.cc:56-359 builds for one (refinement, degree):
+ * mesh hierarchy of a subdivided box refined `refine` times, FP64 system
+ * and rhs matrices (SystemMatrix, operators.h:71-256), MG levels with
+ * PreconditionVanka (gmg.h:402-540), transfers (gmg.h:35-400), GMG
+ * (gmg.h:652-906) and FGMRES (time_integrators.h:44, :113).
+ * spec: "key=value;..." with keys dim, subdiv (a,b,c), lo, hi, refine, p,
+ * type (DG|CGP), r, nts, tau, nts_min, r_min, space_time_mg,
+ * time_before_space, mg_space, mg_float, smoothing_steps, variable,
+ * estimate_relaxation, eig_iters, eig_safety, overwrite_transfer,
+ * vanka_factored, maxiter, abstol, reltol, basis, extrapolate, end_time,
+ * frequency, source (0 manufactured, 1 none, 2 load vector set by
+ * stfem_set_load_vector), dirichlet (face bit mask).
+ * vertices: finest-level vertex coordinates ((nc+1)^dim points, dim each,
+ * x fastest) or NULL for the uniform box; cell_coef: finest-level per-cell
+ * coefficient or NULL for 1. */
+int stfem_problem_create(const char *spec, const double *vertices, const double *cell_coef,
+                         int device, stfem_problem *out);
+int stfem_problem_destroy(stfem_problem p);
+/* info[0..5] = n_levels, nt, n_nodes(fine), n_cells(fine), sizeof(level
+ * number), n_timesteps_at_once */
+int stfem_problem_info(stfem_problem p, long *info);
+int stfem_problem_tau(stfem_problem p, double *tau);
+/* info[0..9] = n_nodes, nt, n_cells, r, nts, mg type to this level,
+ * cells x, y, z, sizeof(number) */
+int stfem_level_info(stfem_problem p, int level, long *info);
+/* relaxation parameter of the level (GMG::estimate_relaxation, gmg.h:792-844) */
+int stfem_level_omega(stfem_problem p, int level, double *omega);
+
+/* SystemMatrix::vmult / Tvmult, operators.h:94-152 (level -1: FP64 fine) */
+int stfem_st_vmult(stfem_problem p, int level, void *dst, const void *src);
+int stfem_st_tvmult(stfem_problem p, int level, void *dst, const void *src);
+/* PreconditionVanka::vmult, gmg.h:476-514 */
+int stfem_vanka_vmult(stfem_problem p, int level, void *dst, const void *src);
+/* STMGTransferBlockMatrixFree::prolongate_and_add / restrict_and_add,
+ * gmg.h:211-225: between `level`-1 (coarse) and `level` (fine) */
+int stfem_prolongate_and_add(stfem_problem p, int level, void *dst, const void *src);
+int stfem_restrict_and_add(stfem_problem p, int level, void *dst, const void *src);
+/* GMG::vmult, gmg.h:846-859 (FP64 in/out; levels in their precision) */
+int stfem_gmg_vmult(stfem_problem p, double *dst, const double *src);
+/* SolverFGMRES::solve with ReductionControl(maxiter, abstol, reltol),
+ * time_integrators.h:44, :174: x holds the initial guess.  its: iteration
+ * count (last_step()); hist: residual history (hist_cap entries max). */
+int stfem_fgmres_solve(stfem_problem p, double *x, const double *b, int *its, double *hist,
+                       int hist_cap);
+/* TimeIntegratorHeat::solve, time_integrators.h:157-180: rhs from prev
+ * (rhs_matrix, operators.h:154-187) + source, extrapolated start, FGMRES,
+ * Dirichlet nodes zeroed.  x: nt values per node; prev: one per node. */
+int stfem_solve_slab(stfem_problem p, double *x, const double *prev, double time, int *its);
+/* host array h (one value per fine node): source f(t,x) = h(x) (int phi_i)
+ * with unit time profile (source=2) */
+int stfem_set_load_vector(stfem_problem p, const double *h);
+/* algorithmic HBM bytes of one application: what 0 fine vmult, 1 Vanka
+ * sweep on `level` */
+int stfem_bytes_model(stfem_problem p, int what, int level, double *bytes);
+/* [t][node] <-> [node][t] on the device (double) */
+int stfem_blocks_to_nodes(long n_nodes, int nt, const double *blocks, double *nodes);
+int stfem_nodes_to_blocks(long n_nodes, int nt, const double *nodes, double *blocks);
+
+/* ---- measurement (HIP events on the library stream) ----------------- */
+/* average milliseconds of one application over `reps` after `warmup`:
+ * what 0 fine FP64 vmult, 1 level vmult, 2 level Vanka, 3 GMG V-cycle */
+int stfem_time_kernel(stfem_problem p, int what, int level, int warmup, int reps, double *ms);
+/* runs `n_slabs` heat slabs from a zero initial value; returns the summed
+ * slab-solve time (ms, HIP events) and the summed FGMRES iterations */
+int stfem_time_slabs(stfem_problem p, int n_slabs, double *ms, int *its);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

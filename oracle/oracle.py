@@ -180,7 +180,7 @@ class Problem:
     def level_info(self, l):
         info = np.zeros(9, dtype=np.int64)
         load().oracle_level_info(self.h, l, info.ctypes.data_as(_lp))
-        keys = ["ndofs", "nt", "ncells", "r", "nts", "mg_type", "nc0", "nc1", "nc2"]
+        keys = ["n_nodes", "nt", "n_cells", "r", "nts", "mg_type", "nc0", "nc1", "nc2"]
         return dict(zip(keys, [int(v) for v in info]))
 
     def omega(self, l):
@@ -203,13 +203,13 @@ class Problem:
         src = np.ascontiguousarray(src, np.float64)
         info = self.level_info(l)
         if what in (0, 1):
-            dst = np.zeros(info["ndofs"] * info["nt"])
+            dst = np.zeros(info["n_nodes"] * info["nt"])
         elif dst is None:
             if what == 2:
-                dst = np.zeros(info["ndofs"] * info["nt"])
+                dst = np.zeros(info["n_nodes"] * info["nt"])
             else:
                 ic = self.level_info(l - 1)
-                dst = np.zeros(ic["ndofs"] * ic["nt"])
+                dst = np.zeros(ic["n_nodes"] * ic["nt"])
         dst = np.ascontiguousarray(dst, np.float64)
         load().oracle_level_apply(self.h, l, what, _p(dst), _p(src))
         return dst
@@ -224,7 +224,7 @@ class Problem:
         """Assembled fine K (which=0) or M (1) in CSR (deal.II constraint convention)."""
         L = load()
         nnz = L.oracle_fine_matrix(self.h, which, None, None, None)
-        n = self.level_info(self.n_levels - 1)["ndofs"]
+        n = self.level_info(self.n_levels - 1)["n_nodes"]
         rp = np.zeros(n + 1, np.int64)
         ci = np.zeros(nnz, np.int64)
         v = np.zeros(nnz)
@@ -247,3 +247,19 @@ class Problem:
         return dict(rc=rc, slabs=n, total_its=int(stats[1]), L2=stats[2], Linf=stats[3],
                     H1=stats[4], ndofs=int(stats[5]), nblocks=int(stats[6]),
                     its=[per[i] for i in range(min(n, max_slabs))])
+
+
+def _solve_slab_sig():
+    L = load()
+    L.oracle_solve_slab.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _ip]
+    return L
+
+
+def solve_slab(problem, x, prev, time):
+    """One heat slab of the oracle; returns (rc, x_new, iterations)."""
+    x = np.ascontiguousarray(x, np.float64).copy()
+    prev = np.ascontiguousarray(prev, np.float64)
+    its = ctypes.c_int()
+    rc = _solve_slab_sig().oracle_solve_slab(problem.h, _p(x), _p(prev), float(time),
+                                             ctypes.byref(its))
+    return rc, x, its.value

@@ -2418,6 +2418,19 @@ extern "C"
     });
     return rc;
   }
+  // One heat slab (time_integrators.h:157-180): x (nt blocks) overwritten.
+  int oracle_solve_slab(void *h, double *x, const double *prev, double time, int *its)
+  {
+    int rc = 0;
+    ORC_DISPATCH(h, {
+      const long          N = P.fine.N();
+      std::vector<double> xv(x, x + (long)P.nt * N), pv(prev, prev + N);
+      rc = P.solve_slab(xv, pv, time);
+      std::copy(xv.begin(), xv.end(), x);
+      *its = P.last_its;
+    });
+    return rc;
+  }
   // Full heat run (tp_01.cc:630-683): stats = [slabs, total_its, L2, Linf,
   // H1semi, n_dofs, n_blocks]; its_per_slab optional (cap entries).
   int oracle_run_heat(void *h, double *stats, int *its_per_slab, int cap)

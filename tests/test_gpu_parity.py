@@ -1,0 +1,196 @@
+"""GPU parity: the HIP path (libstfem_amd.so via the C ABI) against the CPU
+oracle on the same seeded inputs.
+
+Operators are compared at the level precision (FP64 to 1e-12 relative, FP32
+to 2e-6); whole slab solves to 1e-10 relative in FP64 with identical
+iteration counts (north-star parity bar: 1e-12 on the solution difference
+up to the solver tolerance, iterations +-1).
+"""
+import numpy as np
+import pytest
+
+from helpers import blocks_to_nodes, box_vertices, log_uniform, nodes_to_blocks, rel_err
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import oracle as orc  # noqa: E402
+import stfem_amd as st  # noqa: E402
+
+DEV = "cuda:0"
+
+CASES = {
+    # tp_01 tf01 shape: 2D DG(1), 2 steps at once, Q2, space+tau+k levels
+    "2d_dg1_nts2": dict(spec=dict(dim=2, refine=2, type=2, r=1, nts=2, r_min=0), perturb=0.0,
+                        kappa=False),
+    # tp_01 tf02 shape: 2D CGP(2), 2 steps, FP32 multigrid
+    "2d_cg2_nts2_fp32": dict(spec=dict(dim=2, refine=2, type=1, r=2, nts=2, r_min=1, mg_float=1,
+                                       vanka_invert_T=0),
+                             perturb=0.0, kappa=False),
+    # distorted 2D mesh with a cell coefficient, DG(2), Q3
+    "2d_dg2_distorted": dict(spec=dict(dim=2, refine=3, type=2, r=2, nts=1, p=3, r_min=0),
+                             perturb=0.2, kappa=True),
+    # configs[1] shape (Q3, dG(2), perturbed 0.2h, log-uniform kappa), 4^3 cells
+    "3d_q3_dg2": dict(spec=dict(dim=3, refine=2, type=2, r=2, nts=1, p=3, r_min=0), perturb=0.2,
+                      kappa=True),
+    # the same with the factored Vanka inverse (eigen-decomposed patches)
+    "3d_q3_dg2_factored": dict(spec=dict(dim=3, refine=2, type=2, r=2, nts=1, p=3, r_min=0,
+                                         vanka_factored=1), perturb=0.2, kappa=True),
+    # configs[4] shape (Q4, dG(3)), 2^3 cells, FP32 multigrid
+    "3d_q4_dg3_fp32": dict(spec=dict(dim=3, refine=1, type=2, r=3, nts=1, p=4, r_min=2, vanka_invert_T=0,
+                                     mg_float=1), perturb=0.1, kappa=True),
+    # configs[2] time discretisation cG(2) with Q2 (heat operator), 4^3 cells
+    "3d_q2_cg2": dict(spec=dict(dim=3, refine=2, type=1, r=2, nts=2, p=2, r_min=1), perturb=0.0,
+                      kappa=False),
+}
+
+
+def _build(name):
+    c = CASES[name]
+    spec = dict(c["spec"])
+    dim = spec["dim"]
+    nc = [1 << spec["refine"]] * dim
+    X = box_vertices(dim, nc, perturb=c["perturb"], seed=7) if c["perturb"] > 0 else None
+    kap = log_uniform(int(np.prod(nc)), 1e-2, 1e2, seed=11) if c["kappa"] else None
+    o = orc.Problem(spec, coords=X, kappa=kap)
+    g = st.Problem(spec, vertices=X, kappa=kap)
+    return spec, o, g
+
+
+_cache = {}
+
+
+def problems(name):
+    if name not in _cache:
+        _cache.clear()
+        _cache[name] = _build(name)
+    return _cache[name]
+
+
+def dtype_of(g):
+    return torch.float32 if g.level_size == 4 else torch.float64
+
+
+def tol_of(g):
+    return 2e-6 if g.level_size == 4 else 1e-12
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_hierarchy_matches(name):
+    spec, o, g = problems(name)
+    print(f"[case {name}] levels {g.n_levels}", flush=True)
+    assert g.n_levels == o.n_levels
+    for l in range(g.n_levels):
+        a, b = g.level_info(l), o.level_info(l)
+        for k in ("n_nodes", "nt", "r", "nts", "mg_type"):
+            assert a[k] == b[k], (l, k, a, b)
+    assert abs(g.tau - o.tau) < 1e-15
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_level_operators(name):
+    spec, o, g = problems(name)
+    rng = np.random.default_rng(3)
+    dt = dtype_of(g)
+    for l in range(g.n_levels):
+        info = g.level_info(l)
+        n, nt = info["n_nodes"], info["nt"]
+        x = rng.standard_normal(n * nt)  # includes Dirichlet entries: read as zero
+        xg = torch.tensor(blocks_to_nodes(x, nt), dtype=dt, device=DEV)
+        y = torch.zeros_like(xg)
+        print(f"[case {name}] level {l} vmult n={n} nt={nt}", flush=True)
+        g.vmult(l, y, xg)
+        ref = o.level_apply(l, 0, x)
+        assert rel_err(nodes_to_blocks(y.cpu().double().numpy(), nt), ref) < tol_of(g) * 10
+        print(f"[case {name}] level {l} vanka", flush=True)
+        g.vanka(l, y, xg)
+        ref = o.level_apply(l, 1, x)
+        assert rel_err(nodes_to_blocks(y.cpu().double().numpy(), nt), ref) < tol_of(g) * 100
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_transfers(name):
+    spec, o, g = problems(name)
+    rng = np.random.default_rng(5)
+    dt = dtype_of(g)
+    for l in range(1, g.n_levels):
+        fi, ci = g.level_info(l), g.level_info(l - 1)
+        xc = rng.standard_normal(ci["n_nodes"] * ci["nt"])
+        yf0 = rng.standard_normal(fi["n_nodes"] * fi["nt"])
+        print(f"[case {name}] prolongate to {l}", flush=True)
+        dst = torch.tensor(blocks_to_nodes(yf0, fi["nt"]), dtype=dt, device=DEV)
+        src = torch.tensor(blocks_to_nodes(xc, ci["nt"]), dtype=dt, device=DEV)
+        g.prolongate_and_add(l, dst, src)
+        ref = o.level_apply(l, 2, xc, dst=yf0.copy())
+        assert rel_err(nodes_to_blocks(dst.cpu().double().numpy(), fi["nt"]), ref) < tol_of(g) * 10
+        xf = rng.standard_normal(fi["n_nodes"] * fi["nt"])
+        yc0 = rng.standard_normal(ci["n_nodes"] * ci["nt"])
+        print(f"[case {name}] restrict from {l}", flush=True)
+        dst = torch.tensor(blocks_to_nodes(yc0, ci["nt"]), dtype=dt, device=DEV)
+        src = torch.tensor(blocks_to_nodes(xf, fi["nt"]), dtype=dt, device=DEV)
+        g.restrict_and_add(l, dst, src)
+        ref = o.level_apply(l, 3, xf, dst=yc0.copy())
+        assert rel_err(nodes_to_blocks(dst.cpu().double().numpy(), ci["nt"]), ref) < tol_of(g) * 10
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_relaxation_parameters(name):
+    spec, o, g = problems(name)
+    for l in range(g.n_levels):
+        a, b = g.omega(l), o.omega(l)
+        tol = 1e-4 if g.level_size == 4 else 1e-8
+        assert abs(a - b) <= tol * abs(b), (l, a, b)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_fine_vmult_and_transpose(name):
+    spec, o, g = problems(name)
+    rng = np.random.default_rng(9)
+    n, nt = g.n_nodes, g.nt
+    x = rng.standard_normal(n * nt)
+    xg = torch.tensor(blocks_to_nodes(x, nt), dtype=torch.float64, device=DEV)
+    y = torch.zeros_like(xg)
+    for tr in (False, True):
+        print(f"[case {name}] fine vmult transpose={tr}", flush=True)
+        (g.Tvmult if tr else g.vmult)(st.FINE, y, xg)
+        ref = o.fine_vmult(x, transpose=tr)
+        assert rel_err(nodes_to_blocks(y.cpu().numpy(), nt), ref) < 1e-12
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_gmg_vcycle(name):
+    spec, o, g = problems(name)
+    rng = np.random.default_rng(13)
+    n, nt = g.n_nodes, g.nt
+    x = rng.standard_normal(n * nt)
+    xg = torch.tensor(blocks_to_nodes(x, nt), dtype=torch.float64, device=DEV)
+    y = torch.zeros_like(xg)
+    print(f"[case {name}] gmg vmult", flush=True)
+    g.gmg_vmult(y, xg)
+    ref = o.gmg_vmult(x)
+    tol = 1e-4 if g.level_size == 4 else 1e-10
+    assert rel_err(nodes_to_blocks(y.cpu().numpy(), nt), ref) < tol
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_slab_solves(name):
+    spec, o, g = problems(name)
+    n, nt = g.n_nodes, g.nt
+    xo = np.zeros(n * nt)
+    xg = torch.zeros(n * nt, dtype=torch.float64, device=DEV)
+    prev_g = torch.zeros(n, dtype=torch.float64, device=DEV)
+    time = 0.0
+    for k in range(3):
+        prev_o = xo[(nt - 1) * n:].copy()
+        prev_g.copy_(torch.tensor(prev_o, device=DEV))
+        rc_o, xo, its_o = orc.solve_slab(o, xo, prev_o, time)
+        print(f"[case {name}] slab {k}", flush=True)
+        rc_g, its_g = g.solve_slab(xg, prev_g, time)
+        assert rc_o == 0 and rc_g == 0
+        if g.level_size == 8:
+            assert its_g == its_o, (k, its_g, its_o)
+        else:
+            assert abs(its_g - its_o) <= 1, (k, its_g, its_o)
+        err = rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo)
+        assert err < (1e-9 if g.level_size == 8 else 1e-7), (k, err)
+        time += spec["nts"] * g.tau
