@@ -7,6 +7,7 @@
 // the Gram-Schmidt chain needs no host round trip.
 #include "kernels.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace stfem
@@ -144,8 +145,28 @@ namespace stfem
           x[i * nt + t] = T(0);
     }
 
+    template <typename T>
+    __global__ void k_fill_zero(T *p, size_t n)
+    {
+      for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+           i += (size_t)gridDim.x * blockDim.x)
+        p[i] = T(0);
+    }
+
     inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
   } // namespace
+
+  template <typename T>
+  void launch_fill_zero(T *p, size_t n, hipStream_t s)
+  {
+    if (n == 0)
+      return;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 4096);
+    k_fill_zero<T><<<dim3((unsigned)blocks), 256, 0, s>>>(p, n);
+    STFEM_HIP(hipGetLastError());
+  }
+  template void launch_fill_zero<double>(double *, size_t, hipStream_t);
+  template void launch_fill_zero<float>(float *, size_t, hipStream_t);
 
   int reduce_slots() { return RB; }
 

@@ -1,16 +1,22 @@
-// Cell-patch Vanka smoother apply for gfx950
-// (PreconditionVanka::vmult, tests/include/gmg.h:476-514 of nlsmrg/dealii-stfem).
+// Cell-patch Vanka smoother for gfx950
+// (PreconditionVanka::vmult, tests/include/gmg.h:476-514 of nlsmrg/dealii-stfem,
+// fused with the relaxation update of PreconditionRelaxation).
 //
-// dst = sum_c (1/valence) .* scatter_c( B_c^{-1} gather_c(src) ),
+//   u += omega * sum_c (1/valence) .* scatter_c( B_c^{-1} gather_c(rhs - Au) )
+//
 // B_c = Alpha (x) K[I_c,I_c] + Beta (x) M[I_c,I_c], local index k + i*n_c
 // (space fastest, time outer) as the reference builds it (gmg.h:460-468).
+// Like the reference, the patch gathers and scatters every node of the cell,
+// Dirichlet nodes included (their rows of B_c are decoupled).  With Au = null
+// and omega = 1 on a zeroed u this is PreconditionVanka::vmult.
 //
 // Explicit form: one workgroup per cell, B_c^{-1} stored column-major so a
 // column is read by consecutive lanes (coalesced); the gathered patch sits
 // in LDS.  Factored form: B_c^{-1} = (I x S)(blockdiag_k W_k)(I x S^T) with
-// S^T M_c S = I, S^T K_c S = diag(lambda), W_k = (lambda_k Alpha + Beta)^{-1}.
-// The valence (number of cells sharing a node) of a structured grid node is
-// a product of per-direction multiplicities and is computed on the fly.
+// S^T M_c S = I, S^T K_c S = diag(lambda), W_k = (lambda_k Alpha + Beta)^{-1};
+// S is staged into LDS with coalesced loads when it fits.  The valence
+// (number of cells sharing a node) of a structured-grid node is a product of
+// per-direction multiplicities and is computed on the fly.
 #include "kernels.h"
 
 namespace stfem
@@ -19,10 +25,9 @@ namespace stfem
   {
     struct PatchMap
     {
-      int  dim, p, n;     // n = p+1
+      int  dim, p, n; // n = p+1
       int  nc0, nc1;
       long nn0, nn1, nn2;
-      int  dir;
 
       __device__ void cell_xyz(long cell, int &cx, int &cy, int &cz) const
       {
@@ -39,24 +44,19 @@ namespace stfem
             cz = 0;
           }
       }
-      // global node of local node l (lexicographic within the cell); sets
-      // `cons` if Dirichlet and `inv_val` = 1/valence
-      __device__ long node(int cx, int cy, int cz, int l, bool &cons, double &inv_val) const
+      // global node of local node l (lexicographic within the cell) and its
+      // inverse valence
+      __device__ long node(int cx, int cy, int cz, int l, float &inv_val) const
       {
         const int  a  = l % n;
         const int  b  = (l / n) % n;
         const int  c  = (dim == 3) ? l / (n * n) : 0;
         const long gx = (long)cx * p + a, gy = (long)cy * p + b, gz = (long)cz * p + c;
-        cons = (gx == 0 && (dir & 1)) || (gx == nn0 - 1 && (dir & 2)) ||
-               (gy == 0 && (dir & 4)) || (gy == nn1 - 1 && (dir & 8));
         int m = ((gx % p) == 0 && gx > 0 && gx < nn0 - 1 ? 2 : 1) *
                 ((gy % p) == 0 && gy > 0 && gy < nn1 - 1 ? 2 : 1);
         if (dim == 3)
-          {
-            cons = cons || (gz == 0 && (dir & 16)) || (gz == nn2 - 1 && (dir & 32));
-            m *= ((gz % p) == 0 && gz > 0 && gz < nn2 - 1 ? 2 : 1);
-          }
-        inv_val = 1.0 / m;
+          m *= ((gz % p) == 0 && gz > 0 && gz < nn2 - 1 ? 2 : 1);
+        inv_val = 1.0f / float(m);
         return gx + nn0 * (gy + nn1 * gz);
       }
     };
@@ -72,14 +72,32 @@ namespace stfem
       m.nn0 = g.nn[0];
       m.nn1 = g.nn[1];
       m.nn2 = g.nn[2];
-      m.dir = g.dirichlet;
       return m;
+    }
+
+    template <typename T>
+    __device__ __forceinline__ void gather(const PatchMap &M, int cx, int cy, int cz, int nt,
+                                           int nloc, const T *__restrict__ rhs,
+                                           const T *__restrict__ ax, T *xs)
+    {
+      const int nb = nt * nloc;
+      for (int l = threadIdx.x; l < nb; l += blockDim.x)
+        {
+          const int  i = l / nloc, j = l - i * nloc;
+          float      iv;
+          const long g = M.node(cx, cy, cz, j, iv);
+          T          v = rhs[g * nt + i];
+          if (ax)
+            v -= ax[g * nt + i];
+          xs[l] = v;
+        }
     }
 
     template <typename T>
     __global__ void __launch_bounds__(256)
       k_vanka_explicit(const PatchMap M, int nt, int nloc, const T *__restrict__ binv,
-                       const T *__restrict__ src, T *__restrict__ dst)
+                       const T *__restrict__ rhs, const T *__restrict__ ax, T *__restrict__ u,
+                       T omega)
     {
       extern __shared__ unsigned char smem_raw[];
       T *const   xs   = reinterpret_cast<T *>(smem_raw);
@@ -87,14 +105,7 @@ namespace stfem
       const int  nb   = nt * nloc;
       int        cx, cy, cz;
       M.cell_xyz(cell, cx, cy, cz);
-      for (int l = threadIdx.x; l < nb; l += blockDim.x)
-        {
-          const int i = l / nloc, j = l - i * nloc;
-          bool      cons;
-          double    iv;
-          const long g = M.node(cx, cy, cz, j, cons, iv);
-          xs[l]        = cons ? T(0) : src[g * nt + i];
-        }
+      gather(M, cx, cy, cz, nt, nloc, rhs, ax, xs);
       __syncthreads();
       const T *B = binv + (size_t)cell * nb * nb;
       for (int row = threadIdx.x; row < nb; row += blockDim.x)
@@ -102,38 +113,35 @@ namespace stfem
           T acc = T(0);
           for (int col = 0; col < nb; ++col)
             acc += B[(size_t)col * nb + row] * xs[col];
-          const int i = row / nloc, j = row - i * nloc;
-          bool      cons;
-          double    iv;
-          const long g = M.node(cx, cy, cz, j, cons, iv);
-          if (!cons)
-            atomicAdd(&dst[g * nt + i], T(iv) * acc);
+          const int  i = row / nloc, j = row - i * nloc;
+          float      iv;
+          const long g = M.node(cx, cy, cz, j, iv);
+          atomicAdd(&u[g * nt + i], omega * T(iv) * acc);
         }
     }
 
-    // Factored apply, one workgroup per cell.  LDS: x (nt*nloc), y (nt*nloc).
-    template <typename T>
+    // Factored apply, one workgroup per cell.  LDS: x, y (nt*nloc each) and,
+    // when STAGE, S (nloc*nloc).
+    template <typename T, bool STAGE>
     __global__ void __launch_bounds__(256)
       k_vanka_factored(const PatchMap M, int nt, int nloc, const T *__restrict__ S,
-                       const T *__restrict__ W, const T *__restrict__ src, T *__restrict__ dst)
+                       const T *__restrict__ W, const T *__restrict__ rhs,
+                       const T *__restrict__ ax, T *__restrict__ u, T omega)
     {
       extern __shared__ unsigned char smem_raw[];
       T *const   xs   = reinterpret_cast<T *>(smem_raw);
       T *const   ys   = xs + nt * nloc;
+      T *const   ss   = ys + nt * nloc;
       const long cell = blockIdx.x;
-      const int  nb   = nt * nloc;
       int        cx, cy, cz;
       M.cell_xyz(cell, cx, cy, cz);
-      for (int l = threadIdx.x; l < nb; l += blockDim.x)
-        {
-          const int i = l / nloc, j = l - i * nloc;
-          bool      cons;
-          double    iv;
-          const long g = M.node(cx, cy, cz, j, cons, iv);
-          xs[l]        = cons ? T(0) : src[g * nt + i];
-        }
+      const T *Sg = S + (size_t)cell * nloc * nloc;
+      if constexpr (STAGE)
+        for (int l = threadIdx.x; l < nloc * nloc; l += blockDim.x)
+          ss[l] = Sg[l];
+      gather(M, cx, cy, cz, nt, nloc, rhs, ax, xs);
       __syncthreads();
-      const T *Sc = S + (size_t)cell * nloc * nloc;
+      const T *Sc = STAGE ? ss : Sg;
       const T *Wc = W + (size_t)cell * nloc * nt * nt;
       // y_{i,k} = sum_a S[a][k] x_{i,a}  then  z_{:,k} = W_k y_{:,k}
       for (int k = threadIdx.x; k < nloc; k += blockDim.x)
@@ -143,7 +151,7 @@ namespace stfem
             yk[i] = T(0);
           for (int a = 0; a < nloc; ++a)
             {
-              const T s = Sc[(size_t)a * nloc + k];
+              const T s = Sc[a * nloc + k];
               for (int i = 0; i < nt; ++i)
                 yk[i] += s * xs[i * nloc + a];
             }
@@ -163,26 +171,37 @@ namespace stfem
           T o[16];
           for (int i = 0; i < nt; ++i)
             o[i] = T(0);
-          const T *Sa = Sc + (size_t)a * nloc;
+          const T *Sa = Sc + a * nloc;
           for (int k = 0; k < nloc; ++k)
             {
               const T s = Sa[k];
               for (int i = 0; i < nt; ++i)
                 o[i] += s * ys[i * nloc + k];
             }
-          bool      cons;
-          double    iv;
-          const long g = M.node(cx, cy, cz, a, cons, iv);
-          if (!cons)
-            for (int i = 0; i < nt; ++i)
-              atomicAdd(&dst[g * nt + i], T(iv) * o[i]);
+          float      iv;
+          const long g = M.node(cx, cy, cz, a, iv);
+          for (int i = 0; i < nt; ++i)
+            atomicAdd(&u[g * nt + i], omega * T(iv) * o[i]);
         }
+    }
+
+    size_t max_lds_per_block()
+    {
+      static size_t v = 0;
+      if (!v)
+        {
+          int dev = 0, bytes = 0;
+          STFEM_HIP(hipGetDevice(&dev));
+          STFEM_HIP(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+          v = size_t(bytes);
+        }
+      return v;
     }
   } // namespace
 
   template <typename T>
-  void launch_vanka_explicit(const GridDesc &g, int nt, const T *binv, const T *src, T *dst,
-                             hipStream_t s)
+  void launch_vanka_explicit(const GridDesc &g, int nt, const T *binv, const T *rhs, const T *ax,
+                             T *u, T omega, hipStream_t s)
   {
     if (g.n_cells == 0)
       return;
@@ -192,32 +211,45 @@ namespace stfem
     const size_t sm = size_t(nb) * sizeof(T);
     STFEM_REQUIRE(sm <= 64 * 1024, "vanka: patch too large for LDS");
     k_vanka_explicit<T><<<dim3((unsigned)g.n_cells), dim3((unsigned)thr), sm, s>>>(
-      make_map(g), nt, nloc, binv, src, dst);
+      make_map(g), nt, nloc, binv, rhs, ax, u, omega);
     STFEM_HIP(hipGetLastError());
   }
 
   template <typename T>
-  void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *W, const T *src,
-                             T *dst, hipStream_t s)
+  void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *W, const T *rhs,
+                             const T *ax, T *u, T omega, hipStream_t s)
   {
     if (g.n_cells == 0)
       return;
     STFEM_REQUIRE(nt <= 16, "vanka factored: at most 16 time dofs per patch");
-    const int nloc = (g.dim == 3) ? (g.p + 1) * (g.p + 1) * (g.p + 1) : (g.p + 1) * (g.p + 1);
-    const int thr  = nloc >= 256 ? 256 : ((nloc + 63) / 64) * 64;
-    const size_t sm = 2 * size_t(nt) * nloc * sizeof(T);
-    k_vanka_factored<T><<<dim3((unsigned)g.n_cells), dim3((unsigned)thr), sm, s>>>(
-      make_map(g), nt, nloc, S, W, src, dst);
+    const int    nloc = (g.dim == 3) ? (g.p + 1) * (g.p + 1) * (g.p + 1) : (g.p + 1) * (g.p + 1);
+    const int    thr  = nloc >= 256 ? 256 : ((nloc + 63) / 64) * 64;
+    const size_t base = 2 * size_t(nt) * nloc * sizeof(T);
+    const size_t full = base + size_t(nloc) * nloc * sizeof(T);
+    const PatchMap M  = make_map(g);
+    if (full <= max_lds_per_block())
+      {
+        if (full > 64 * 1024)
+          STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored<T, true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(full)));
+        k_vanka_factored<T, true><<<dim3((unsigned)g.n_cells), dim3((unsigned)thr), full, s>>>(
+          M, nt, nloc, S, W, rhs, ax, u, omega);
+      }
+    else
+      k_vanka_factored<T, false><<<dim3((unsigned)g.n_cells), dim3((unsigned)thr), base, s>>>(
+        M, nt, nloc, S, W, rhs, ax, u, omega);
     STFEM_HIP(hipGetLastError());
   }
 
   template void launch_vanka_explicit<double>(const GridDesc &, int, const double *,
-                                              const double *, double *, hipStream_t);
-  template void launch_vanka_explicit<float>(const GridDesc &, int, const float *, const float *,
-                                             float *, hipStream_t);
-  template void launch_vanka_factored<double>(const GridDesc &, int, const double *,
-                                              const double *, const double *, double *,
+                                              const double *, const double *, double *, double,
                                               hipStream_t);
+  template void launch_vanka_explicit<float>(const GridDesc &, int, const float *, const float *,
+                                             const float *, float *, float, hipStream_t);
+  template void launch_vanka_factored<double>(const GridDesc &, int, const double *,
+                                              const double *, const double *, const double *,
+                                              double *, double, hipStream_t);
   template void launch_vanka_factored<float>(const GridDesc &, int, const float *, const float *,
-                                             const float *, float *, hipStream_t);
+                                             const float *, const float *, float *, float,
+                                             hipStream_t);
 } // namespace stfem

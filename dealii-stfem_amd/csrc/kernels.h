@@ -35,16 +35,20 @@ namespace stfem
   void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, hipStream_t s);
 
   // ------------------------------------------------------------ Vanka
-  // dst = sum_cells (1/valence) * scatter( Binv_c * gather(src) ) with
-  // explicit per-cell inverses stored column-major, nb = nt * (p+1)^dim.
+  // u += omega * sum_cells (1/valence) * scatter( Binv_c * gather(rhs - ax) )
+  // (ax may be null) with explicit per-cell inverses stored column-major,
+  // nb = nt * (p+1)^dim.
   template <typename T>
-  void launch_vanka_explicit(const GridDesc &g, int nt, const T *binv, const T *src, T *dst,
-                             hipStream_t s);
+  void launch_vanka_explicit(const GridDesc &g, int nt, const T *binv, const T *rhs, const T *ax,
+                             T *u, T omega, hipStream_t s);
   // Factored form: Binv_c = (I x S_c) blockdiag_k (W_{c,k}) (I x S_c^T),
   // S_c (n_c x n_c, row-major), W_{c,k} (nt x nt row-major) per mode k.
   template <typename T>
-  void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *W, const T *src,
-                             T *dst, hipStream_t s);
+  void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *W, const T *rhs,
+                             const T *ax, T *u, T omega, hipStream_t s);
+  // p[0..n) = 0 with a kernel (safe to record into a HIP graph)
+  template <typename T>
+  void launch_fill_zero(T *p, size_t n, hipStream_t s);
 
   // ------------------------------------------------------------ transfers
   // Per node: dst[node][a] (= or +=) sum_b M(a,b) src[node][b].

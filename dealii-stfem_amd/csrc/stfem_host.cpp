@@ -418,7 +418,7 @@ namespace stfem
                               bool accumulate) const
   {
     if (!accumulate)
-      STFEM_HIP(hipMemsetAsync(dst, 0, size_t(grid_.n_nodes) * nt_out_ * sizeof(T), ctx_.stream));
+      launch_fill_zero<T>(dst, size_t(grid_.n_nodes) * nt_out_, ctx_.stream);
     StOpArgs<T> args;
     args.g          = grid_;
     args.vertices   = verts_.get();
@@ -817,11 +817,16 @@ namespace stfem
   template <typename T>
   void PreconditionVanka<T>::vmult(T *dst, const T *src) const
   {
-    STFEM_HIP(hipMemsetAsync(dst, 0, size_t(grid_.n_nodes) * nt_ * sizeof(T), ctx_.stream));
+    launch_fill_zero<T>(dst, size_t(grid_.n_nodes) * nt_, ctx_.stream);
+    relax(dst, src, nullptr, T(1));
+  }
+  template <typename T>
+  void PreconditionVanka<T>::relax(T *u, const T *rhs, const T *ax, T omega) const
+  {
     if (form_ == Form::explicit_inverse)
-      launch_vanka_explicit<T>(grid_, nt_, binv_.get(), src, dst, ctx_.stream);
+      launch_vanka_explicit<T>(grid_, nt_, binv_.get(), rhs, ax, u, omega, ctx_.stream);
     else
-      launch_vanka_factored<T>(grid_, nt_, S_.get(), W_.get(), src, dst, ctx_.stream);
+      launch_vanka_factored<T>(grid_, nt_, S_.get(), W_.get(), rhs, ax, u, omega, ctx_.stream);
   }
   template <typename T>
   double PreconditionVanka<T>::sweep_bytes() const
@@ -1041,28 +1046,56 @@ namespace stfem
   }
 
   template <typename T>
+  GMG<T>::~GMG()
+  {
+    if (exec_)
+      (void)hipGraphExecDestroy(exec_);
+    if (graph_)
+      (void)hipGraphDestroy(graph_);
+  }
+
+  // One PreconditionRelaxation sweep with the Vanka preconditioner:
+  // first sweep from zero u = omega V rhs, later sweeps
+  // u += omega V (rhs - A u).  Two kernels per sweep (A u, fused Vanka
+  // update) plus a zero fill.
+  template <typename T>
   void GMG<T>::relax(int l, T *u, const T *rhs, bool first) const
+  {
+    relax_omega(l, u, rhs, first, T(lv_[l].omega));
+  }
+  template <typename T>
+  void GMG<T>::relax_omega(int l, T *u, const T *rhs, bool first, T omega) const
   {
     const auto &L  = lv_[l];
     const long  sz = L.A->m();
-    const T     om = T(L.omega);
     if (first)
       {
-        L.V->vmult(u, rhs);
-        if (L.omega != 1.0)
-          launch_axpby<T>(sz, om, u, T(0), u, ctx_.stream);
+        launch_fill_zero<T>(u, size_t(sz), ctx_.stream);
+        L.V->relax(u, rhs, nullptr, omega);
         return;
       }
-    T *t1 = t1_[l].get(), *t2 = t2_[l].get();
+    T *t1 = t1_[l].get();
     L.A->vmult(t1, u);
-    launch_axpby<T>(sz, T(1), rhs, T(-1), t1, ctx_.stream);
-    L.V->vmult(t2, t1);
-    launch_axpby<T>(sz, om, t2, T(1), u, ctx_.stream);
+    L.V->relax(u, rhs, t1, omega);
+  }
+  template <typename T>
+  bool GMG<T>::exact_level(int l) const
+  {
+    return opt_.exact_single_cell && lv_[l].V->n_cells() == 1;
+  }
+  // total sweeps s: on a one-cell level V = A^{-1}, so the error contracts
+  // by (1 - omega) per sweep and s sweeps equal one with 1 - (1 - omega)^s
+  template <typename T>
+  T GMG<T>::exact_omega(int l, int sweeps) const
+  {
+    return T(1.0 - std::pow(1.0 - lv_[l].omega, double(sweeps)));
   }
   template <typename T>
   void GMG<T>::smoother_apply(int l, T *u, const T *rhs) const
   {
     const int s2 = steps(l);
+    if (exact_level(l))
+      return relax_omega(l, u, rhs, true, exact_omega(l, s2 * opt_.smoothing_steps));
     for (int i = 0; i < opt_.smoothing_steps; ++i)
       relax(l, u, rhs, i == 0);
     for (int k = 1; k < s2; ++k)
@@ -1073,6 +1106,8 @@ namespace stfem
   void GMG<T>::smoother_smooth(int l, T *u, const T *rhs) const
   {
     const int s2 = steps(l);
+    if (exact_level(l))
+      return relax_omega(l, u, rhs, false, exact_omega(l, s2 * opt_.smoothing_steps));
     for (int k = 0; k < s2; ++k)
       for (int i = 1; i <= opt_.smoothing_steps; ++i)
         relax(l, u, rhs, false);
@@ -1090,9 +1125,8 @@ namespace stfem
     T         *t  = t1_[l].get();
     lv_[l].A->vmult(t, sol_[l].get());
     launch_axpby<T>(sz, T(1), def_[l].get(), T(-1), t, ctx_.stream);
-    def_[l - 1].zero(ctx_.stream);
+    launch_fill_zero<T>(def_[l - 1].get(), def_[l - 1].size(), ctx_.stream);
     tr_[l].restrict_and_add(def_[l - 1].get(), t);
-    sol_[l - 1].zero(ctx_.stream);
     level_v_step(l - 1);
     tr_[l].prolongate_and_add(sol_[l].get(), sol_[l - 1].get());
     smoother_smooth(l, sol_[l].get(), def_[l].get());
@@ -1112,7 +1146,21 @@ namespace stfem
                                ctx_.stream));
     else
       launch_convert<T, double>(sz, src, def_[L].get(), ctx_.stream);
-    vcycle();
+    if (opt_.use_graph)
+      {
+        // the V-cycle launches a fixed kernel sequence on fixed buffers:
+        // record it once, replay it afterwards
+        if (!exec_)
+          {
+            STFEM_HIP(hipStreamBeginCapture(ctx_.stream, hipStreamCaptureModeThreadLocal));
+            vcycle();
+            STFEM_HIP(hipStreamEndCapture(ctx_.stream, &graph_));
+            STFEM_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+          }
+        STFEM_HIP(hipGraphLaunch(exec_, ctx_.stream));
+      }
+    else
+      vcycle();
     if constexpr (std::is_same_v<T, double>)
       STFEM_HIP(hipMemcpyAsync(dst, sol_[L].get(), sz * sizeof(double), hipMemcpyDeviceToDevice,
                                ctx_.stream));
@@ -1314,6 +1362,8 @@ namespace stfem
     getd("eig_safety", ps.mg.eig_safety);
     getb("overwrite_transfer", ps.mg.overwrite_time_transfer);
     getb("vanka_factored", ps.mg.vanka_factored);
+    getb("exact_single_cell", ps.mg.exact_single_cell);
+    getb("use_graph", ps.mg.use_graph);
     geti("maxiter", ps.solver.maxiter);
     getd("abstol", ps.solver.abstol);
     getd("reltol", ps.solver.reltol);

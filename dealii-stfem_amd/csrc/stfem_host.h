@@ -187,6 +187,10 @@ namespace stfem
                       const DMat &Beta, Form form);
     // dst = sum_c (1/valence) B_c^{-1} src|_c   (gmg.h:476-514)
     void   vmult(T *dst, const T *src) const;
+    // u += omega * Vanka(rhs - ax); ax may be null.  One relaxation sweep of
+    // PreconditionRelaxation with this preconditioner, in one kernel.
+    void   relax(T *u, const T *rhs, const T *ax, T omega) const;
+    long   n_cells() const { return grid_.n_cells; }
     Form   form() const { return form_; }
     double sweep_bytes() const;
     size_t storage_bytes() const { return (binv_.size() + S_.size() + W_.size()) * sizeof(T); }
@@ -272,6 +276,12 @@ namespace stfem
     double eig_safety              = 1.15;
     bool   overwrite_time_transfer = false;
     bool   vanka_factored          = false;
+    // On a one-cell level the Vanka patch is the whole level, so the
+    // smoother is the exact inverse and s relaxation sweeps with omega equal
+    // one sweep with 1 - (1 - omega)^s.
+    bool   exact_single_cell       = true;
+    // replay the V-cycle from a captured HIP graph
+    bool   use_graph               = true;
   };
 
   // Level data of the hierarchy (precision T).
@@ -295,6 +305,7 @@ namespace stfem
   public:
     GMG(Context &ctx, std::vector<GMGLevel<T>> &&levels,
         std::vector<TwoLevelTransfer<T>> &&transfers, const GMGOptions &opt);
+    ~GMG();
     void   vmult(double *dst, const double *src) const;
     // one V-cycle on level vectors (def[L] given, sol[L] returned)
     void   vcycle() const;
@@ -307,6 +318,9 @@ namespace stfem
 
   private:
     void relax(int l, T *u, const T *rhs, bool first) const;
+    void relax_omega(int l, T *u, const T *rhs, bool first, T omega) const;
+    bool exact_level(int l) const;
+    T    exact_omega(int l, int sweeps) const;
     void smoother_apply(int l, T *u, const T *rhs) const;
     void smoother_smooth(int l, T *u, const T *rhs) const;
     void level_v_step(int l) const;
@@ -317,6 +331,8 @@ namespace stfem
     std::vector<TwoLevelTransfer<T>> tr_;
     GMGOptions                       opt_;
     mutable std::vector<DBuf<T>>     sol_, def_, t1_, t2_;
+    mutable hipGraph_t               graph_ = nullptr;
+    mutable hipGraphExec_t           exec_  = nullptr;
   };
 
   // ------------------------------------------------------------ FGMRES

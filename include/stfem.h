@@ -81,9 +81,10 @@ int stfem_time_mg_sequence(int n_sp_lvl, int k_max, int k_min, int nts, int nts_
  * type (DG|CGP), r, nts, tau, nts_min, r_min, space_time_mg,
  * time_before_space, mg_space, mg_float, smoothing_steps, variable,
  * estimate_relaxation, eig_iters, eig_safety, overwrite_transfer,
- * vanka_factored, maxiter, abstol, reltol, basis, extrapolate, end_time,
- * frequency, source (0 manufactured, 1 none, 2 load vector set by
- * stfem_set_load_vector), dirichlet (face bit mask).
+ * vanka_factored, exact_single_cell, use_graph, maxiter, abstol, reltol,
+ * basis, extrapolate, end_time, frequency, source (0 manufactured heat
+ * source, 1 none, 2 load vector set by stfem_set_load_vector, 3 seeded
+ * Gaussian bumps with n_bumps, seed), dirichlet (face bit mask).
  * vertices: finest-level vertex coordinates ((nc+1)^dim points, dim each,
  * x fastest) or NULL for the uniform box; cell_coef: finest-level per-cell
  * coefficient or NULL for 1. */
