@@ -31,8 +31,12 @@
 //   * PreconditionRelaxation step_operations, MGSmootherPrecondition (variable
 //     steps 2^(L-l)), MGCoarseGridApplySmoother, Multigrid::level_v_step with
 //     prolongate_and_add into the level solution.
-//   * PreconditionChebyshev power iteration (Rayleigh quotient, 20 its, start
-//     vector (i mod 11) - mean per block), omega = 2/(alpha + lambda_max).
+//   * PreconditionChebyshev power iteration (Rayleigh quotient, 20 its),
+//     omega = 2/(alpha + lambda_max), alpha = 0.9 lambda_max.  The start
+//     vector and any safety factor are deal.II-internal; the defaults here
+//     (constant start vector, factor 1.15 on the estimate) are the best fit
+//     to the golden iteration counts (other variants selectable by spec).
+//     This part is parity-unpinned beyond those counts.
 //   * SolverFGMRES (max basis 30, modified Gram-Schmidt, least squares on the
 //     j x (j-1) Hessenberg block, iteration counted per projected solve).
 // These are pinned only through the golden iteration counts and error norms
