@@ -105,6 +105,27 @@ def cpu_baseline(budget_s=25.0):
                        f"FGMRES its, {dt:.1f} s")
 
 
+def kernel_only(a, spec, X, kappa):
+    """Run a few launches of one kernel (for hardware-counter passes) after a
+    plain device copy of known size that calibrates the byte counters."""
+    import torch
+    import stfem_amd as st
+    n_copy = 64 << 20  # doubles: 512 MiB copied
+    src = torch.ones(n_copy, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"calibration copy: {n_copy * 8} bytes read, {n_copy * 8} bytes written per launch")
+    g = st.Problem(dict(spec, mg_float=1), vertices=X, kappa=kappa)
+    L = g.n_levels - 1
+    what = {"vanka": (2, L), "vmult32": (1, L), "vmult64": (0, L)}[a.kernel_only]
+    ms = g.time_kernel(what[0], what[1], warmup=1, reps=5)
+    byts = g.bytes_model(1, L) if a.kernel_only == "vanka" else g.bytes_model(0)
+    log(f"{a.kernel_only}: {ms * 1e3:.1f} us per launch, model bytes {byts:.4e}")
+    g.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +134,7 @@ def main():
     ap.add_argument("--refine", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--kernel-only", choices=["vanka", "vmult32", "vmult64"], default=None)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -122,6 +144,8 @@ def main():
     import torch
     torch.cuda.synchronize()
     spec, X, kappa = workload(a.refine)
+    if a.kernel_only:
+        return kernel_only(a, spec, X, kappa)
 
     # FP32 multigrid inside FP64 FGMRES (headline)
     s32 = dict(spec, mg_float=1)

@@ -19,6 +19,8 @@
 // per-direction multiplicities and is computed on the fly.
 #include "kernels.h"
 
+#include <type_traits>
+
 namespace stfem
 {
   namespace
@@ -185,6 +187,136 @@ namespace stfem
         }
     }
 
+    // Factored apply with compile-time patch size NL and time dofs NT: 256
+    // threads per cell in P = 256/G groups of G = 64*ceil(NL/64) lanes.  Each
+    // group handles a slice of the contraction index; partial sums meet in
+    // LDS.  S is staged into LDS with a padded row stride (NL+1) so that both
+    // the column reads of S^T x and the row reads of S z are conflict-free.
+    template <typename T, int NL, int NT>
+    __global__ void __launch_bounds__(256)
+      k_vanka_factored_v2(const PatchMap M, const T *__restrict__ S, const T *__restrict__ W,
+                          const T *__restrict__ rhs, const T *__restrict__ ax,
+                          T *__restrict__ u, T omega)
+    {
+      constexpr int G  = ((NL + 63) / 64) * 64;
+      constexpr int P  = 256 / G;
+      constexpr int LD = NL + 1;
+      constexpr int CH = (NL + P - 1) / P; // contraction slice per group
+      extern __shared__ unsigned char smem_raw[];
+      T *const ss  = reinterpret_cast<T *>(smem_raw); // NL * LD
+      T *const xs  = ss + NL * LD;                    // NT * NL
+      T *const ys  = xs + NT * NL;                    // NT * NL
+      T *const red = ys + NT * NL;                    // P * NT * NL
+
+      const long cell = blockIdx.x;
+      int        cx, cy, cz;
+      M.cell_xyz(cell, cx, cy, cz);
+      const T *Sg = S + (size_t)cell * NL * NL;
+      for (int l = threadIdx.x; l < NL * NL; l += 256)
+        {
+          const int a = l / NL, k = l - a * NL;
+          ss[a * LD + k] = Sg[l];
+        }
+      for (int l = threadIdx.x; l < NT * NL; l += 256)
+        {
+          const int  i = l / NL, j = l - i * NL;
+          float      iv;
+          const long g = M.node(cx, cy, cz, j, iv);
+          T          v = rhs[g * NT + i];
+          if (ax)
+            v -= ax[g * NT + i];
+          xs[l] = v;
+        }
+      __syncthreads();
+      const int part = threadIdx.x / G, lane = threadIdx.x - part * G;
+      const int a0 = part * CH, a1 = (a0 + CH < NL) ? a0 + CH : NL;
+      // partial y_{i,k} over a in [a0, a1)
+      if (lane < NL)
+        {
+          T y[NT];
+#pragma unroll
+          for (int i = 0; i < NT; ++i)
+            y[i] = T(0);
+          for (int a = a0; a < a1; ++a)
+            {
+              const T s = ss[a * LD + lane];
+#pragma unroll
+              for (int i = 0; i < NT; ++i)
+                y[i] += s * xs[i * NL + a];
+            }
+#pragma unroll
+          for (int i = 0; i < NT; ++i)
+            red[(part * NT + i) * NL + lane] = y[i];
+        }
+      __syncthreads();
+      // z_{:,k} = W_k sum_parts y_{:,k}
+      for (int k = threadIdx.x; k < NL; k += 256)
+        {
+          T y[NT];
+#pragma unroll
+          for (int i = 0; i < NT; ++i)
+            {
+              T s = T(0);
+#pragma unroll
+              for (int q = 0; q < P; ++q)
+                s += red[(q * NT + i) * NL + k];
+              y[i] = s;
+            }
+          const T *Wk = W + ((size_t)cell * NL + k) * NT * NT;
+#pragma unroll
+          for (int i = 0; i < NT; ++i)
+            {
+              T z = T(0);
+#pragma unroll
+              for (int j = 0; j < NT; ++j)
+                z += Wk[i * NT + j] * y[j];
+              ys[i * NL + k] = z;
+            }
+        }
+      __syncthreads();
+      // partial out_{i,a} over k in the group's slice
+      if (lane < NL)
+        {
+          T o[NT];
+#pragma unroll
+          for (int i = 0; i < NT; ++i)
+            o[i] = T(0);
+          for (int k = a0; k < a1; ++k)
+            {
+              const T s = ss[lane * LD + k];
+#pragma unroll
+              for (int i = 0; i < NT; ++i)
+                o[i] += s * ys[i * NL + k];
+            }
+#pragma unroll
+          for (int i = 0; i < NT; ++i)
+            red[(part * NT + i) * NL + lane] = o[i];
+        }
+      __syncthreads();
+      for (int l = threadIdx.x; l < NT * NL; l += 256)
+        {
+          const int i = l / NL, a = l - i * NL;
+          T         s = T(0);
+#pragma unroll
+          for (int q = 0; q < P; ++q)
+            s += red[(q * NT + i) * NL + a];
+          float      iv;
+          const long g = M.node(cx, cy, cz, a, iv);
+          atomicAdd(&u[g * NT + i], omega * T(iv) * s);
+        }
+    }
+
+    template <int NL>
+    constexpr int v2_groups()
+    {
+      return 256 / (((NL + 63) / 64) * 64);
+    }
+    template <typename T, int NL, int NT>
+    size_t v2_lds()
+    {
+      return sizeof(T) * (size_t(NL) * (NL + 1) + 2 * NT * NL + v2_groups<NL>() * NT * NL);
+    }
+
     size_t max_lds_per_block()
     {
       static size_t v = 0;
@@ -227,6 +359,41 @@ namespace stfem
     const size_t base = 2 * size_t(nt) * nloc * sizeof(T);
     const size_t full = base + size_t(nloc) * nloc * sizeof(T);
     const PatchMap M  = make_map(g);
+    // compile-time patch shapes first
+    bool done = false;
+    auto try_v2 = [&](auto nl_c, auto nt_c) {
+      constexpr int NL = decltype(nl_c)::value, NT = decltype(nt_c)::value;
+      if (done || nloc != NL || nt != NT)
+        return;
+      const size_t lds = v2_lds<T, NL, NT>();
+      if (lds > max_lds_per_block())
+        return;
+      if (lds > 64 * 1024)
+        STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v2<T, NL, NT>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+      k_vanka_factored_v2<T, NL, NT><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
+        M, S, W, rhs, ax, u, omega);
+      done = true;
+    };
+    auto for_nt = [&](auto nl_c) {
+      try_v2(nl_c, std::integral_constant<int, 1>{});
+      try_v2(nl_c, std::integral_constant<int, 2>{});
+      try_v2(nl_c, std::integral_constant<int, 3>{});
+      try_v2(nl_c, std::integral_constant<int, 4>{});
+      try_v2(nl_c, std::integral_constant<int, 6>{});
+      try_v2(nl_c, std::integral_constant<int, 8>{});
+    };
+    for_nt(std::integral_constant<int, 9>{});
+    for_nt(std::integral_constant<int, 16>{});
+    for_nt(std::integral_constant<int, 25>{});
+    for_nt(std::integral_constant<int, 27>{});
+    for_nt(std::integral_constant<int, 64>{});
+    for_nt(std::integral_constant<int, 125>{});
+    if (done)
+      {
+        STFEM_HIP(hipGetLastError());
+        return;
+      }
     if (full <= max_lds_per_block())
       {
         if (full > 64 * 1024)

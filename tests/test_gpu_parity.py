@@ -39,6 +39,10 @@ CASES = {
     # configs[4] shape (Q4, dG(3)), 2^3 cells, FP32 multigrid
     "3d_q4_dg3_fp32": dict(spec=dict(dim=3, refine=1, type=2, r=3, nts=1, p=4, r_min=2, vanka_invert_T=0,
                                      mg_float=1), perturb=0.1, kappa=True),
+    # FP32 multigrid with the factored Vanka (the benchmark configuration)
+    "3d_q3_dg2_fp32_factored": dict(spec=dict(dim=3, refine=2, type=2, r=2, nts=1, p=3, r_min=0,
+                                              mg_float=1, vanka_factored=1, vanka_invert_T=0),
+                                    perturb=0.2, kappa=True),
     # configs[2] time discretisation cG(2) with Q2 (heat operator), 4^3 cells
     "3d_q2_cg2": dict(spec=dict(dim=3, refine=2, type=1, r=2, nts=2, p=2, r_min=1), perturb=0.0,
                       kappa=False),
