@@ -105,12 +105,68 @@ def cpu_baseline(budget_s=25.0):
                        f"FGMRES its, {dt:.1f} s")
 
 
+COPY_BYTES = 512 << 20  # calibration copy of the counter passes
+
+
+def _counter_csv(d):
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                return os.path.join(root, f)
+    return None
+
+
+def pmc_traffic(kernel, name, refine, timeout_s=150):
+    """HBM bytes per launch of one kernel from two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE) over `bench.py --kernel-only`.  Each counter
+    is scaled by what it reads for a device copy of COPY_BYTES in the same
+    process; the finest-level launches (largest grid) are the ones reported."""
+    import csv
+    import shutil
+    import statistics
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not on PATH"
+    env = dict(os.environ, TMPDIR="/tmp")
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="stfem_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, os.path.abspath(__file__), "--kernel-only", kernel,
+               "--refine", str(refine)]
+        log(f"counter pass {counter}: {' '.join(cmd)}")
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.PIPE, timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            return None, f"{counter} pass timed out"
+        path = _counter_csv(d)
+        if path is None:
+            return None, f"{counter} pass wrote no counters (rc {r.returncode})"
+        rows = [x for x in csv.DictReader(open(path)) if x["Counter_Name"] == counter]
+        shutil.rmtree(d, ignore_errors=True)
+        copy = [float(x["Counter_Value"]) for x in rows if "copyBuffer" in x["Kernel_Name"]]
+        mine = [x for x in rows if name in x["Kernel_Name"]]
+        if not copy or not mine:
+            return None, f"{counter}: calibration copy or kernel missing from the counters"
+        gmax = max(int(x["Grid_Size"]) for x in mine)
+        vals = [float(x["Counter_Value"]) for x in mine if int(x["Grid_Size"]) == gmax]
+        scale = COPY_BYTES / (max(copy) * 1024.0)
+        res[counter] = (statistics.median(vals) * 1024.0 * scale, scale, len(vals))
+    fetch, fs, nf = res["FETCH_SIZE"]
+    write, ws, nw = res["WRITE_SIZE"]
+    return dict(bytes=fetch + write, fetch=fetch, write=write, fetch_scale=fs,
+                write_scale=ws, dispatches=[nf, nw]), None
+
+
 def kernel_only(a, spec, X, kappa):
     """Run a few launches of one kernel (for hardware-counter passes) after a
     plain device copy of known size that calibrates the byte counters."""
     import torch
     import stfem_amd as st
-    n_copy = 64 << 20  # doubles: 512 MiB copied
+    n_copy = COPY_BYTES // 8  # doubles
     src = torch.ones(n_copy, dtype=torch.float64, device="cuda:0")
     dst = torch.empty_like(src)
     for _ in range(3):
@@ -134,6 +190,8 @@ def main():
     ap.add_argument("--refine", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the two rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--kernel-only", choices=["vanka", "vmult32", "vmult64"], default=None)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -174,10 +232,17 @@ def main():
     # dominant kernel of an FGMRES iteration on the finest level: two Vanka
     # sweeps (pre/post), three level vmults and one FP64 vmult
     if 2 * vk_ms >= 3 * vm32_ms + vm64_ms:
-        dom = dict(kernel="vanka_factored_fp32", ms=vk_ms, bytes=vk_bytes)
+        dom = dict(kernel="vanka_factored_fp32", ms=vk_ms, bytes=vk_bytes, only="vanka",
+                   symbol="k_vanka_factored")
     else:
-        dom = dict(kernel="st_vmult_fp64", ms=vm64_ms, bytes=vm64_bytes)
+        dom = dict(kernel="st_vmult_fp64", ms=vm64_ms, bytes=vm64_bytes, only="vmult64",
+                   symbol="k_st_apply<3, 4, 3, double>")
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+
+    traffic, traffic_note = None, "counter passes skipped (--no-pmc)"
+    if not a.no_pmc:
+        traffic, traffic_note = pmc_traffic(dom["only"], dom["symbol"], a.refine)
+        log(f"traffic: {traffic if traffic else traffic_note}")
 
     cpu = None if a.no_cpu else cpu_baseline()
     out = {
@@ -210,7 +275,9 @@ def main():
         "vanka_fp32": {"ms": vk_ms, "GBps": vk_bytes / (vk_ms * 1e-3) / 1e9},
         "roofline": {"kernel": dom["kernel"], "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "algorithmic_bytes": dom["bytes"], "ms": dom["ms"],
+                     "traffic_detail": traffic if traffic else traffic_note},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
