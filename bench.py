@@ -192,6 +192,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the two rocprofv3 counter passes behind roofline.traffic")
+    ap.add_argument("--dense-coarse-dofs", type=int, default=None,
+                    help="largest coarse level collapsed into one dense matrix (0: off)")
     ap.add_argument("--kernel-only", choices=["vanka", "vmult32", "vmult64"], default=None)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -202,6 +204,8 @@ def main():
     import torch
     torch.cuda.synchronize()
     spec, X, kappa = workload(a.refine)
+    if a.dense_coarse_dofs is not None:
+        spec["dense_coarse_dofs"] = a.dense_coarse_dofs
     if a.kernel_only:
         return kernel_only(a, spec, X, kappa)
 

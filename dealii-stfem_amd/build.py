@@ -50,7 +50,8 @@ def build(jobs=None, verbose=False):
     objs = [o for o, _ in results]
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs + \
-        ["-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/llvm/lib"]
+        ["-L/opt/rocm/lib", "-lrocblas",
+         "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/llvm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed\n" + r.stdout + r.stderr)

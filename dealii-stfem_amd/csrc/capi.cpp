@@ -203,7 +203,7 @@ extern "C"
   {
     return guarded([&] {
       auto &pb = P(p);
-      long  li[10];
+      long  li[12];
       pb.level_info(pb.n_levels() - 1, li);
       info[0] = pb.n_levels();
       info[1] = pb.nt;
@@ -374,7 +374,7 @@ extern "C"
       STFEM_REQUIRE(reps >= 1 && warmup >= 0, "bad repetition counts");
       Context &ctx = pb.context();
       STFEM_HIP(hipSetDevice(ctx.device));
-      long li[10];
+      long li[12];
       if (what == 0 || what == 3)
         level = pb.n_levels() - 1;
       check_level(pb, level, false);

@@ -1,0 +1,569 @@
+// Device code of the fused matrix-free space-time operator (gfx950),
+// shared by the one-launch-per-level kernel and the coarse-level kernel.
+//
+//   dst_j = sum_i Alpha(j,i) K src_i + Beta(j,i) M src_i
+//
+// (SystemMatrix::vmult over MatrixFreeOperator K and M,
+// tests/include/operators.h:94-122 and :487-548 of nlsmrg/dealii-stfem.)
+// A cell is handled by N^(dim-1) threads, thread (a,b) owning the line of N
+// values along the last coordinate direction (in registers); the other
+// directions are swept through LDS.  All nt input components are evaluated
+// once (values and reference gradients at the Gauss points); the time mixing
+// with Alpha and Beta happens at the quadrature points, so each output
+// component costs one integration.  Geometry (Q1 mapping, coefficient) is
+// evaluated on the fly from the cell's 2^dim vertices.  Results are
+// accumulated with hardware FP atomics; Dirichlet nodes are read as zero and
+// never written.
+//
+// Every translation unit that includes this header owns its copy of the
+// constant-memory shape tables and uploads it with upload_tables_here().
+#pragma once
+
+#include "kernels.h"
+
+namespace stfem
+{
+  namespace
+  {
+    __constant__ double cSd[MAX_N + 1][MAX_N][MAX_N];
+    __constant__ double cDd[MAX_N + 1][MAX_N][MAX_N];
+    __constant__ double cWd[MAX_N + 1][MAX_N];
+    __constant__ double cXd[MAX_N + 1][MAX_N];
+    __constant__ float  cSf[MAX_N + 1][MAX_N][MAX_N];
+    __constant__ float  cDf[MAX_N + 1][MAX_N][MAX_N];
+    __constant__ float  cWf[MAX_N + 1][MAX_N];
+    __constant__ float  cXf[MAX_N + 1][MAX_N];
+
+    template <typename T>
+    struct Tab;
+    template <>
+    struct Tab<double>
+    {
+      static __device__ __forceinline__ double S(int n, int q, int a) { return cSd[n][q][a]; }
+      static __device__ __forceinline__ double D(int n, int q, int a) { return cDd[n][q][a]; }
+      static __device__ __forceinline__ double W(int n, int q) { return cWd[n][q]; }
+      static __device__ __forceinline__ double X(int n, int q) { return cXd[n][q]; }
+    };
+    template <>
+    struct Tab<float>
+    {
+      static __device__ __forceinline__ float S(int n, int q, int a) { return cSf[n][q][a]; }
+      static __device__ __forceinline__ float D(int n, int q, int a) { return cDf[n][q][a]; }
+      static __device__ __forceinline__ float W(int n, int q) { return cWf[n][q]; }
+      static __device__ __forceinline__ float X(int n, int q) { return cXf[n][q]; }
+    };
+
+    template <int DIM, int N>
+    struct Layout
+    {
+      static constexpr int TPC = (DIM == 3) ? N * N : N;       // threads per cell
+      static constexpr int CPB = 256 / TPC;                      // cells per block
+      static constexpr int NC  = (DIM == 3) ? N * N * N : N * N; // values per cell
+      static __device__ __forceinline__ int idx(int a, int b, int k)
+      {
+        return (DIM == 3) ? a + N * (b + N * k) : a + N * k;
+      }
+    };
+
+    // One group of threads applies the operator to the CPB cells
+    // [blk*CPB, blk*CPB + CPB) (cells past the end contribute nothing).
+    // tid in [0, 256).  When the group has more than CPB*TPC threads (256
+    // is not a multiple of TPC for N = 3, 5), the threads past CPB*TPC form
+    // an idle slot CPB: they touch no global memory, and lds0/lds1 must then
+    // hold (CPB+1)*NC values each so that their LDS traffic stays in that
+    // slot (CPB*NC suffices for groups of exactly CPB*TPC threads).  Every
+    // thread of the workgroup must call this with the same j0/jn: it
+    // synchronises the workgroup a fixed number of times.
+    template <int DIM, int N, int JB, typename T>
+    __device__ __forceinline__ void st_apply_cells(const StOpArgs<T> &A, const T *__restrict__ src,
+                                                   T *__restrict__ dst, int j0, int jn, long blk,
+                                                   int tid, T *__restrict__ lds0,
+                                                   T *__restrict__ lds1)
+    {
+      using Ly = Layout<DIM, N>;
+      using Tb = Tab<T>;
+      constexpr int TPC = Ly::TPC, CPB = Ly::CPB, NC = Ly::NC;
+
+      const int  lc    = tid / TPC;
+      const int  t     = tid - lc * TPC;
+      const int  ia    = t % N;
+      const int  ib    = (DIM == 3) ? t / N : 0;
+      const long cell  = blk * CPB + lc;
+      const bool valid = lc < CPB && cell < A.g.n_cells;
+      T *const   L0    = lds0 + lc * NC;
+      T *const   L1    = lds1 + lc * NC;
+      constexpr int p  = N - 1;
+
+      // --- cell position and the node line owned by this thread
+      const int nc0 = A.g.nc[0], nc1 = A.g.nc[1];
+      int       cx = 0, cy = 0, cz = 0;
+      if (valid)
+        {
+          cx     = int(cell % nc0);
+          long r = cell / nc0;
+          if (DIM == 3)
+            {
+              cy = int(r % nc1);
+              cz = int(r / nc1);
+            }
+          else
+            cy = int(r);
+        }
+      const long nn0 = A.g.nn[0], nn1 = A.g.nn[1], nn2 = A.g.nn[2];
+      const int  dir = A.g.dirichlet;
+      long       base, kstride;
+      unsigned   cmask = 0; // bit k: node k of the line is a Dirichlet node
+      {
+        const long gx = (long)cx * p + ia;
+        bool       bnd = (gx == 0 && (dir & 1)) || (gx == nn0 - 1 && (dir & 2));
+        if (DIM == 3)
+          {
+            const long gy = (long)cy * p + ib;
+            bnd = bnd || (gy == 0 && (dir & 4)) || (gy == nn1 - 1 && (dir & 8));
+            base    = gx + nn0 * (gy + nn1 * ((long)cz * p));
+            kstride = nn0 * nn1;
+            for (int k = 0; k < N; ++k)
+              {
+                const long gz = (long)cz * p + k;
+                if (bnd || (gz == 0 && (dir & 16)) || (gz == nn2 - 1 && (dir & 32)))
+                  cmask |= 1u << k;
+              }
+          }
+        else
+          {
+            base    = gx + nn0 * ((long)cy * p);
+            kstride = nn0;
+            for (int k = 0; k < N; ++k)
+              {
+                const long gy = (long)cy * p + k;
+                if (bnd || (gy == 0 && (dir & 4)) || (gy == nn1 - 1 && (dir & 8)))
+                  cmask |= 1u << k;
+              }
+          }
+      }
+      const int nti = A.nt_in, nto = A.nt_out;
+
+      // --- accumulators at the quadrature points of the owned line
+      T accV[JB][N];
+      T accG[JB][DIM][N];
+#pragma unroll
+      for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+          {
+            accV[jj][k] = T(0);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              accG[jj][d][k] = T(0);
+          }
+
+      for (int i = 0; i < nti; ++i)
+        {
+          T u[N];
+#pragma unroll
+          for (int k = 0; k < N; ++k)
+            u[k] = (valid && !((cmask >> k) & 1u)) ? src[(base + k * kstride) * nti + i] : T(0);
+          // interpolation along the owned line
+          T w[N];
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            {
+              T s = T(0);
+#pragma unroll
+              for (int c = 0; c < N; ++c)
+                s += Tb::S(N, q, c) * u[c];
+              w[q] = s;
+            }
+          T qv[N], g0[N], g1[N], g2[N];
+          __syncthreads();
+          if constexpr (DIM == 3)
+            {
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                L0[Ly::idx(ia, ib, k)] = w[k];
+              __syncthreads();
+              T v[N];
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  T s = T(0);
+#pragma unroll
+                  for (int b2 = 0; b2 < N; ++b2)
+                    s += Tb::S(N, ib, b2) * L0[Ly::idx(ia, b2, k)];
+                  v[k] = s;
+                }
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                L1[Ly::idx(ia, ib, k)] = v[k];
+              __syncthreads();
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  T s = T(0);
+#pragma unroll
+                  for (int a2 = 0; a2 < N; ++a2)
+                    s += Tb::S(N, ia, a2) * L1[Ly::idx(a2, ib, k)];
+                  qv[k] = s;
+                }
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                L0[Ly::idx(ia, ib, k)] = qv[k];
+              __syncthreads();
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  T sx = T(0), sy = T(0), sz = T(0);
+#pragma unroll
+                  for (int m = 0; m < N; ++m)
+                    {
+                      sx += Tb::D(N, ia, m) * L0[Ly::idx(m, ib, k)];
+                      sy += Tb::D(N, ib, m) * L0[Ly::idx(ia, m, k)];
+                      sz += Tb::D(N, k, m) * qv[m];
+                    }
+                  g0[k] = sx;
+                  g1[k] = sy;
+                  g2[k] = sz;
+                }
+            }
+          else
+            {
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                L0[Ly::idx(ia, 0, k)] = w[k];
+              __syncthreads();
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  T s = T(0);
+#pragma unroll
+                  for (int a2 = 0; a2 < N; ++a2)
+                    s += Tb::S(N, ia, a2) * L0[Ly::idx(a2, 0, k)];
+                  qv[k] = s;
+                }
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                L1[Ly::idx(ia, 0, k)] = qv[k];
+              __syncthreads();
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  T sx = T(0), sy = T(0);
+#pragma unroll
+                  for (int m = 0; m < N; ++m)
+                    {
+                      sx += Tb::D(N, ia, m) * L1[Ly::idx(m, 0, k)];
+                      sy += Tb::D(N, k, m) * qv[m];
+                    }
+                  g0[k] = sx;
+                  g1[k] = sy;
+                  g2[k] = T(0);
+                }
+            }
+          // time mixing at the quadrature points
+#pragma unroll
+          for (int jj = 0; jj < JB; ++jj)
+            {
+              const int j = j0 + jj;
+              if (j < jn)
+                {
+                  const T ca = A.alpha[j * nti + i];
+                  const T cb = A.beta[j * nti + i];
+#pragma unroll
+                  for (int k = 0; k < N; ++k)
+                    {
+                      accV[jj][k] += cb * qv[k];
+                      accG[jj][0][k] += ca * g0[k];
+                      accG[jj][1][k] += ca * g1[k];
+                      if constexpr (DIM == 3)
+                        accG[jj][2][k] += ca * g2[k];
+                    }
+                }
+            }
+        }
+
+      // --- geometry at the quadrature points: JxW and the metric
+      //     G = kappa * w * adj(J) adj(J)^T / |det J|
+      {
+        T Vx[1 << DIM][DIM];
+        if (valid)
+          {
+            const long nv0 = nc0 + 1, nv1 = nc1 + 1;
+#pragma unroll
+            for (int v = 0; v < (1 << DIM); ++v)
+              {
+                const long vx = cx + (v & 1), vy = cy + ((v >> 1) & 1);
+                const long vz = (DIM == 3) ? cz + ((v >> 2) & 1) : 0;
+                const long id = vx + nv0 * (vy + nv1 * vz);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+                  Vx[v][d] = T(A.vertices[id * DIM + d]);
+              }
+          }
+        else
+          {
+#pragma unroll
+            for (int v = 0; v < (1 << DIM); ++v)
+#pragma unroll
+              for (int d = 0; d < DIM; ++d)
+                Vx[v][d] = T((v >> d) & 1);
+          }
+        const T xa = Tb::X(N, ia), xb = Tb::X(N, ib);
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+          {
+            T J[DIM][DIM];
+            T wq;
+            if constexpr (DIM == 3)
+              {
+                const T xc = Tb::X(N, k);
+                const T wy[2] = {T(1) - xb, xb}, wz[2] = {T(1) - xc, xc}, wx[2] = {T(1) - xa, xa};
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                  {
+                    T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+                    for (int e = 0; e < 2; ++e)
+#pragma unroll
+                      for (int f = 0; f < 2; ++f)
+                        {
+                          s0 += (Vx[1 + 2 * e + 4 * f][d] - Vx[0 + 2 * e + 4 * f][d]) * wy[e] * wz[f];
+                          s1 += (Vx[e + 2 + 4 * f][d] - Vx[e + 4 * f][d]) * wx[e] * wz[f];
+                          s2 += (Vx[e + 2 * f + 4][d] - Vx[e + 2 * f][d]) * wx[e] * wy[f];
+                        }
+                    J[d][0] = s0;
+                    J[d][1] = s1;
+                    J[d][2] = s2;
+                  }
+                wq = Tb::W(N, ia) * Tb::W(N, ib) * Tb::W(N, k);
+              }
+            else
+              {
+                const T xc = Tb::X(N, k);
+                const T wy[2] = {T(1) - xc, xc}, wx[2] = {T(1) - xa, xa};
+#pragma unroll
+                for (int d = 0; d < 2; ++d)
+                  {
+                    J[d][0] = (Vx[1][d] - Vx[0][d]) * wy[0] + (Vx[3][d] - Vx[2][d]) * wy[1];
+                    J[d][1] = (Vx[2][d] - Vx[0][d]) * wx[0] + (Vx[3][d] - Vx[1][d]) * wx[1];
+                  }
+                wq = Tb::W(N, ia) * Tb::W(N, k);
+              }
+            T adj[DIM][DIM];
+            T det;
+            if constexpr (DIM == 3)
+              {
+                adj[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+                adj[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+                adj[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+                adj[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+                adj[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+                adj[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+                adj[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+                adj[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+                adj[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+                det = J[0][0] * adj[0][0] + J[0][1] * adj[1][0] + J[0][2] * adj[2][0];
+              }
+            else
+              {
+                adj[0][0] = J[1][1];
+                adj[0][1] = -J[0][1];
+                adj[1][0] = -J[1][0];
+                adj[1][1] = J[0][0];
+                det       = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+              }
+            const T adet = det < T(0) ? -det : det;
+            T       kap  = A.coef_value;
+            if (A.coef_mode == 1)
+              kap = valid ? A.coef[cell] : T(0);
+            else if (A.coef_mode == 2)
+              kap = valid ? A.coef[cell * NC + Ly::idx(ia, ib, k)] : T(0);
+            const T JxW = adet * wq;
+            const T gs  = kap * wq / adet;
+            T       G[DIM][DIM];
+#pragma unroll
+            for (int e = 0; e < DIM; ++e)
+#pragma unroll
+              for (int f = 0; f < DIM; ++f)
+                {
+                  T s = T(0);
+#pragma unroll
+                  for (int d = 0; d < DIM; ++d)
+                    s += adj[e][d] * adj[f][d];
+                  G[e][f] = s * gs;
+                }
+#pragma unroll
+            for (int jj = 0; jj < JB; ++jj)
+              {
+                accV[jj][k] *= JxW;
+                T gr[DIM];
+#pragma unroll
+                for (int e = 0; e < DIM; ++e)
+                  {
+                    T s = T(0);
+#pragma unroll
+                    for (int f = 0; f < DIM; ++f)
+                      s += G[e][f] * accG[jj][f][k];
+                    gr[e] = s;
+                  }
+#pragma unroll
+                for (int e = 0; e < DIM; ++e)
+                  accG[jj][e][k] = gr[e];
+              }
+          }
+      }
+
+      // --- integrate each output component and scatter
+#pragma unroll
+      for (int jj = 0; jj < JB; ++jj)
+        {
+          const int j = j0 + jj;
+          if (j >= jn)
+            break;
+          T out[N];
+          __syncthreads();
+          if constexpr (DIM == 3)
+            {
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  L0[Ly::idx(ia, ib, k)] = accG[jj][0][k];
+                  L1[Ly::idx(ia, ib, k)] = accG[jj][1][k];
+                }
+              __syncthreads();
+              T r[N];
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  T s = accV[jj][k];
+#pragma unroll
+                  for (int m = 0; m < N; ++m)
+                    {
+                      s += Tb::D(N, m, k) * accG[jj][2][m];
+                      s += Tb::D(N, m, ia) * L0[Ly::idx(m, ib, k)];
+                      s += Tb::D(N, m, ib) * L1[Ly::idx(ia, m, k)];
+                    }
+                  r[k] = s;
+                }
+              __syncthreads();
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                L0[Ly::idx(ia, ib, k)] = r[k];
+              __syncthreads();
+              T s1[N];
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  T s = T(0);
+#pragma unroll
+                  for (int m = 0; m < N; ++m)
+                    s += Tb::S(N, m, ia) * L0[Ly::idx(m, ib, k)];
+                  s1[k] = s;
+                }
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                L1[Ly::idx(ia, ib, k)] = s1[k];
+              __syncthreads();
+              T s2[N];
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  T s = T(0);
+#pragma unroll
+                  for (int m = 0; m < N; ++m)
+                    s += Tb::S(N, m, ib) * L1[Ly::idx(ia, m, k)];
+                  s2[k] = s;
+                }
+#pragma unroll
+              for (int c = 0; c < N; ++c)
+                {
+                  T s = T(0);
+#pragma unroll
+                  for (int k = 0; k < N; ++k)
+                    s += Tb::S(N, k, c) * s2[k];
+                  out[c] = s;
+                }
+            }
+          else
+            {
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                L0[Ly::idx(ia, 0, k)] = accG[jj][0][k];
+              __syncthreads();
+              T r[N];
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  T s = accV[jj][k];
+#pragma unroll
+                  for (int m = 0; m < N; ++m)
+                    {
+                      s += Tb::D(N, m, k) * accG[jj][1][m];
+                      s += Tb::D(N, m, ia) * L0[Ly::idx(m, 0, k)];
+                    }
+                  r[k] = s;
+                }
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                L1[Ly::idx(ia, 0, k)] = r[k];
+              __syncthreads();
+              T s1[N];
+#pragma unroll
+              for (int k = 0; k < N; ++k)
+                {
+                  T s = T(0);
+#pragma unroll
+                  for (int m = 0; m < N; ++m)
+                    s += Tb::S(N, m, ia) * L1[Ly::idx(m, 0, k)];
+                  s1[k] = s;
+                }
+#pragma unroll
+              for (int c = 0; c < N; ++c)
+                {
+                  T s = T(0);
+#pragma unroll
+                  for (int k = 0; k < N; ++k)
+                    s += Tb::S(N, k, c) * s1[k];
+                  out[c] = s;
+                }
+            }
+          if (valid)
+#pragma unroll
+            for (int c = 0; c < N; ++c)
+              if (!((cmask >> c) & 1u))
+                atomicAdd(&dst[(base + c * kstride) * nto + j], out[c]);
+        }
+    }
+
+    // copy the 1D tables of FE_Q(N-1) into this translation unit's constant memory
+    inline void upload_tables_here(const double *S, const double *D, const double *W,
+                                   const double *X, int N)
+    {
+      double Sd[MAX_N][MAX_N] = {}, Dd[MAX_N][MAX_N] = {}, Wd[MAX_N] = {}, Xd[MAX_N] = {};
+      float  Sf[MAX_N][MAX_N] = {}, Df[MAX_N][MAX_N] = {}, Wf[MAX_N] = {}, Xf[MAX_N] = {};
+      for (int q = 0; q < N; ++q)
+        {
+          Wd[q] = W[q];
+          Wf[q] = float(W[q]);
+          Xd[q] = X[q];
+          Xf[q] = float(X[q]);
+          for (int a = 0; a < N; ++a)
+            {
+              Sd[q][a] = S[q * N + a];
+              Dd[q][a] = D[q * N + a];
+              Sf[q][a] = float(S[q * N + a]);
+              Df[q][a] = float(D[q * N + a]);
+            }
+        }
+      const size_t o2 = size_t(N) * sizeof(Sd), o1 = size_t(N) * sizeof(Wd);
+      const size_t f2 = size_t(N) * sizeof(Sf), f1 = size_t(N) * sizeof(Wf);
+      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cSd), Sd, sizeof(Sd), o2, hipMemcpyHostToDevice));
+      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cDd), Dd, sizeof(Dd), o2, hipMemcpyHostToDevice));
+      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cWd), Wd, sizeof(Wd), o1, hipMemcpyHostToDevice));
+      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cXd), Xd, sizeof(Xd), o1, hipMemcpyHostToDevice));
+      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cSf), Sf, sizeof(Sf), f2, hipMemcpyHostToDevice));
+      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cDf), Df, sizeof(Df), f2, hipMemcpyHostToDevice));
+      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cWf), Wf, sizeof(Wf), f1, hipMemcpyHostToDevice));
+      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cXf), Xf, sizeof(Xf), f1, hipMemcpyHostToDevice));
+    }
+  } // namespace
+} // namespace stfem

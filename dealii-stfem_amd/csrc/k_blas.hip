@@ -154,6 +154,76 @@ namespace stfem
     }
 
     inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+    template <typename T>
+    __global__ void k_unit(long n, long j, T *v)
+    {
+      for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+           i += (long)gridDim.x * blockDim.x)
+        v[i] = (i == j) ? T(1) : T(0);
+    }
+    __global__ void k_add_diagonal(long n, double a, double *M)
+    {
+      const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (i < n)
+        M[i + i * n] += a;
+    }
+    __global__ void k_scale(long n, double a, double *x)
+    {
+      const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (i < n)
+        x[i] *= a;
+    }
+    template <typename T>
+    __global__ void k_store_column(long n, const T *__restrict__ v, double *__restrict__ M, long ld,
+                                   long j)
+    {
+      const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (i < n)
+        M[j * ld + i] = double(v[i]);
+    }
+    template <typename T>
+    __global__ void k_pack_rows(long n, const double *__restrict__ M, T *__restrict__ R, long ld)
+    {
+      const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (idx >= n * ld)
+        return;
+      const long i = idx / ld, j = idx - i * ld;
+      R[idx] = (j < n) ? T(M[i + j * n]) : T(0);
+    }
+
+    // y = R x, R row-major with rows of ld entries: one wavefront per row
+    // streams the row in 16-byte vectors (coalesced), sums in FP64 and
+    // reduces across the wavefront.
+    template <typename T>
+    __global__ void __launch_bounds__(256)
+      k_gemv_rows(long n, long ld, const T *__restrict__ R, const T *__restrict__ x,
+                  T *__restrict__ y)
+    {
+      constexpr int VW = 16 / sizeof(T);
+      typedef T     vec_t __attribute__((ext_vector_type(VW)));
+      const long    row  = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+      const int     lane = threadIdx.x & 63;
+      if (row >= n)
+        return; // uniform per wavefront
+      const T *r   = R + row * ld;
+      double   acc = 0.0;
+      const long nv = (n / VW) * VW;
+      for (long j = (long)lane * VW; j < nv; j += 64 * VW)
+        {
+          const vec_t m  = *reinterpret_cast<const vec_t *>(r + j);
+          const vec_t xv = *reinterpret_cast<const vec_t *>(x + j);
+#pragma unroll
+          for (int e = 0; e < VW; ++e)
+            acc += double(m[e]) * double(xv[e]);
+        }
+      for (long j = nv + lane; j < n; j += 64)
+        acc += double(r[j]) * double(x[j]);
+      for (int off = 32; off > 0; off >>= 1)
+        acc += __shfl_down(acc, off, 64);
+      if (lane == 0)
+        y[row] = T(acc);
+    }
   } // namespace
 
   template <typename T>
@@ -247,6 +317,69 @@ namespace stfem
     STFEM_HIP(hipGetLastError());
   }
 
+  template <typename T>
+  void launch_unit(long n, long j, T *v, hipStream_t s)
+  {
+    if (n == 0)
+      return;
+    const long blocks = std::min<long>((n + 255) / 256, 1024);
+    k_unit<T><<<dim3((unsigned)blocks), 256, 0, s>>>(n, j, v);
+    STFEM_HIP(hipGetLastError());
+  }
+  void launch_add_diagonal(long n, double a, double *M, hipStream_t s)
+  {
+    if (n == 0)
+      return;
+    k_add_diagonal<<<nblk(n), 256, 0, s>>>(n, a, M);
+    STFEM_HIP(hipGetLastError());
+  }
+  void launch_scale(long n, double a, double *x, hipStream_t s)
+  {
+    if (n == 0)
+      return;
+    k_scale<<<nblk(n), 256, 0, s>>>(n, a, x);
+    STFEM_HIP(hipGetLastError());
+  }
+  template <typename T>
+  void launch_store_column(long n, const T *v, double *M, long ld, long j, hipStream_t s)
+  {
+    if (n == 0)
+      return;
+    k_store_column<T><<<nblk(n), 256, 0, s>>>(n, v, M, ld, j);
+    STFEM_HIP(hipGetLastError());
+  }
+  template <typename T>
+  void launch_pack_rows(long n, const double *M, T *R, long ld, hipStream_t s)
+  {
+    STFEM_REQUIRE(ld >= n, "pack_rows: leading dimension below the order");
+    if (n == 0)
+      return;
+    k_pack_rows<T><<<nblk(n * ld), 256, 0, s>>>(n, M, R, ld);
+    STFEM_HIP(hipGetLastError());
+  }
+  template <typename T>
+  void launch_gemv_rows(long n, long ld, const T *R, const T *x, T *y, hipStream_t s)
+  {
+    STFEM_REQUIRE(ld >= n && ld % long(16 / sizeof(T)) == 0,
+                  "gemv_rows: rows must be padded to 16-byte multiples");
+    if (n == 0)
+      return;
+    k_gemv_rows<T><<<dim3((unsigned)((n + 3) / 4)), 256, 0, s>>>(n, ld, R, x, y);
+    STFEM_HIP(hipGetLastError());
+  }
+
+  template void launch_unit<double>(long, long, double *, hipStream_t);
+  template void launch_unit<float>(long, long, float *, hipStream_t);
+  template void launch_store_column<double>(long, const double *, double *, long, long,
+                                            hipStream_t);
+  template void launch_store_column<float>(long, const float *, double *, long, long,
+                                           hipStream_t);
+  template void launch_pack_rows<double>(long, const double *, double *, long, hipStream_t);
+  template void launch_pack_rows<float>(long, const double *, float *, long, hipStream_t);
+  template void launch_gemv_rows<double>(long, long, const double *, const double *, double *,
+                                         hipStream_t);
+  template void launch_gemv_rows<float>(long, long, const float *, const float *, float *,
+                                        hipStream_t);
   template void launch_dot<double>(long, const double *, const double *, double *, double *, int,
                                    hipStream_t);
   template void launch_dot<float>(long, const float *, const float *, double *, double *, int,

@@ -1,61 +1,32 @@
-// Multigrid transfer kernels for gfx950.
+// Multigrid transfer kernels for gfx950, one launch per transfer step.
 //
-// Time (tau / k) transfers: MGTwoLevelTransferST::prolongate_and_add /
-// restrict_and_add (tests/include/gmg.h:121-135 via tensorproduct_add,
-// operators.h:47-58 of nlsmrg/dealii-stfem) apply a small dense matrix to
-// the nt time values of every node: with the node-major layout this is one
-// streaming pass.
-//
-// Space (h) transfers: the embedding of Q_p on the coarse grid into Q_p on
-// the once-refined grid (deal.II MGTwoLevelTransfer as built at gmg.h:371-380)
-// is, on a structured grid, a Kronecker product of banded 1D operators with
-// zero-Dirichlet rows/columns removed.  It is applied as one 1D sweep per
-// direction; restriction uses the transposed 1D operators.
-#include "kernels.h"
+// Time transfers (MGTwoLevelTransferST, tests/include/gmg.h:121-135 of
+// nlsmrg/dealii-stfem) are one streaming pass over the node-major vector;
+// space transfers (deal.II MGTwoLevelTransfer built at gmg.h:371-380) are one
+// 1D banded sweep per direction.  Per-entry device code is in dev_transfer.h.
+#include "dev_transfer.h"
 
 namespace stfem
 {
   namespace
   {
     template <typename T>
-    __global__ void k_time_transfer(long n_nodes, const T *__restrict__ M, int ntd, int nts,
+    __global__ void k_time_transfer(long total, const T *__restrict__ M, int ntd, int nts,
                                     const T *__restrict__ src, T *__restrict__ dst, int add)
     {
       const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-      if (idx >= n_nodes * ntd)
-        return;
-      const long node = idx / ntd;
-      const int  a    = int(idx - node * ntd);
-      T          s    = T(0);
-      for (int b = 0; b < nts; ++b)
-        s += M[a * nts + b] * src[node * nts + b];
-      dst[idx] = add ? dst[idx] + s : s;
+      if (idx < total)
+        time_transfer_item(idx, M, ntd, nts, src, dst, add);
     }
 
     template <typename T>
-    __global__ void k_sweep(const T *__restrict__ in, T *__restrict__ out, int nt, int d, int n0,
-                            int n1, int n2, int m, const int *__restrict__ cnt,
+    __global__ void k_sweep(long total, const T *__restrict__ in, T *__restrict__ out, int nt,
+                            int d, int n0, int n1, int m, const int *__restrict__ cnt,
                             const int *__restrict__ col, const T *__restrict__ w, int K, int add)
     {
-      // output dims: n with n[d] replaced by m
-      const long o0 = (d == 0) ? m : n0, o1 = (d == 1) ? m : n1, o2 = (d == 2) ? m : n2;
-      const long total = o0 * o1 * o2 * nt;
-      const long idx   = (long)blockIdx.x * blockDim.x + threadIdx.x;
-      if (idx >= total)
-        return;
-      const long pt   = idx / nt;
-      const int  comp = int(idx - pt * nt);
-      const long i0 = pt % o0, r = pt / o0, i1 = r % o1, i2 = r / o1;
-      const long od = (d == 0) ? i0 : (d == 1) ? i1 : i2;
-      const long stride = (d == 0) ? 1 : (d == 1) ? (long)n0 : (long)n0 * n1;
-      // input index with the d coordinate set to 0
-      const long j0 = (d == 0) ? 0 : i0, j1 = (d == 1) ? 0 : i1, j2 = (d == 2) ? 0 : i2;
-      const long ib = j0 + (long)n0 * (j1 + (long)n1 * j2);
-      const int  c  = cnt[od];
-      T          s  = T(0);
-      for (int k = 0; k < c; ++k)
-        s += w[od * K + k] * in[(ib + (long)col[od * K + k] * stride) * nt + comp];
-      out[idx] = add ? out[idx] + s : s;
+      const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (idx < total)
+        sweep_item(idx, in, out, nt, d, n0, n1, m, cnt, col, w, K, add);
     }
   } // namespace
 
@@ -67,7 +38,7 @@ namespace stfem
     if (total == 0)
       return;
     k_time_transfer<T><<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(
-      n_nodes, M, ntd, nts, src, dst, add ? 1 : 0);
+      total, M, ntd, nts, src, dst, add ? 1 : 0);
     STFEM_HIP(hipGetLastError());
   }
 
@@ -75,13 +46,11 @@ namespace stfem
   void launch_sweep(const T *in, T *out, int nt, int d, const int n_in[3], int m_out,
                     const int *cnt, const int *col, const T *w, int K, bool add, hipStream_t s)
   {
-    long o[3] = {n_in[0], n_in[1], n_in[2]};
-    o[d]      = m_out;
-    const long total = o[0] * o[1] * o[2] * nt;
+    const long total = sweep_total(nt, d, n_in, m_out);
     if (total == 0)
       return;
     k_sweep<T><<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(
-      in, out, nt, d, n_in[0], n_in[1], n_in[2], m_out, cnt, col, w, K, add ? 1 : 0);
+      total, in, out, nt, d, n_in[0], n_in[1], m_out, cnt, col, w, K, add ? 1 : 0);
     STFEM_HIP(hipGetLastError());
   }
 

@@ -4,6 +4,10 @@
 
 #include "common.h"
 
+#include <string>
+#include <type_traits>
+#include <vector>
+
 namespace stfem
 {
   // ------------------------------------------------------------ shape data
@@ -90,4 +94,23 @@ namespace stfem
   // zero the Dirichlet nodes of an interleaved vector
   template <typename T>
   void launch_zero_dirichlet(const GridDesc &g, int nt, T *x, hipStream_t s);
+
+  // ------------------------------------------------- dense coarse operator
+  // v = e_j (length n)
+  template <typename T>
+  void launch_unit(long n, long j, T *v, hipStream_t s);
+  // M[i + i*n] += a (column-major n x n);  x[0..n) *= a
+  void launch_add_diagonal(long n, double a, double *M, hipStream_t s);
+  void launch_scale(long n, double a, double *x, hipStream_t s);
+  // column j of the column-major FP64 matrix M (leading dimension ld) = v
+  template <typename T>
+  void launch_store_column(long n, const T *v, double *M, long ld, long j, hipStream_t s);
+  // R[i*ld + j] = T(M[i + j*n]) for i, j < n (column-major n x n FP64 into
+  // row-major precision-T rows of ld >= n entries, padding zeroed)
+  template <typename T>
+  void launch_pack_rows(long n, const double *M, T *R, long ld, hipStream_t s);
+  // y = R x for the row-major R above (one wavefront per row, FP64 sums);
+  // ld must be a multiple of 16 bytes / sizeof(T)
+  template <typename T>
+  void launch_gemv_rows(long n, long ld, const T *R, const T *x, T *y, hipStream_t s);
 } // namespace stfem

@@ -4,6 +4,8 @@
 // time integrator) that launch the gfx950 kernels.
 #include "stfem_host.h"
 
+#include <rocblas/rocblas.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -1000,6 +1002,192 @@ namespace stfem
       }
     for (int l = 0; l < nl; ++l)
       lv_[l].omega = opt_.estimate_relaxation ? estimate_omega(l) : 1.0;
+    setup_dense_coarse();
+  }
+
+  namespace
+  {
+    // column-major FP64 products on the device (setup of the dense coarse
+    // operator only)
+    struct DeviceBlas
+    {
+      rocblas_handle h = nullptr;
+      explicit DeviceBlas(hipStream_t s)
+      {
+        check(rocblas_create_handle(&h));
+        check(rocblas_set_stream(h, s));
+      }
+      ~DeviceBlas()
+      {
+        if (h)
+          (void)rocblas_destroy_handle(h);
+      }
+      static void check(rocblas_status st)
+      {
+        if (st != rocblas_status_success)
+          throw Error(STATUS_ERR_INTERNAL, std::string("rocBLAS: ") + rocblas_status_to_string(st));
+      }
+      // C (m x n) = alpha A (m x k) B (k x n) + beta C
+      void gemm(long m, long n, long k, double alpha, const double *A, const double *B, double beta,
+                double *C)
+      {
+        check(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, rocblas_int(m),
+                            rocblas_int(n), rocblas_int(k), &alpha, A, rocblas_int(m), B,
+                            rocblas_int(k), &beta, C, rocblas_int(m)));
+      }
+    };
+  } // namespace
+
+  // The V-cycle on levels 0..lc is linear in def[lc]: sol[lc] = M_lc def[lc]
+  // with (level_v_step)
+  //   M_0 = S_0,
+  //   M_l = G_l^s (S_l + P_l M_{l-1} R_l (I - A_l S_l)) + S_l
+  //         (S_l dropped from the bracket when the prolongation overwrites),
+  // where one relaxation sweep is u <- G u + H def, G = I - w V A, H = w V,
+  // s = 2^(L-l) * smoothing_steps sweeps, and S_l = sum_{k<s} G^k H is the
+  // map of s sweeps from zero (one sweep with the exact-level weight on
+  // one-cell levels).  A, V, P, R are taken column by column from the level
+  // kernels applied to unit vectors, so M_lc is the V-cycle of the launched
+  // kernels up to rounding; the products run in FP64 and M_lc is stored in
+  // the level precision.
+  template <typename T>
+  void GMG<T>::setup_dense_coarse()
+  {
+    dense_level_ = -1;
+    const int L  = int(lv_.size()) - 1;
+    if (opt_.dense_coarse_dofs <= 0 || L < 1)
+      return;
+    int lc = -1;
+    for (int l = 0; l < L; ++l)
+      {
+        if (lv_[l].A->m() > opt_.dense_coarse_dofs)
+          break;
+        lc = l;
+      }
+    if (lc < 0)
+      return;
+    hipStream_t s = ctx_.stream;
+    DeviceBlas  blas(s);
+    // dense n_out x n_in matrix of a linear map given by its launcher
+    auto dense_of = [&](long n_out, long n_in, auto &&apply) {
+      DBuf<double> D(size_t(n_out) * n_in);
+      DBuf<T>      x(n_in), y(n_out);
+      for (long j = 0; j < n_in; ++j)
+        {
+          launch_unit<T>(n_in, j, x.get(), s);
+          apply(y.get(), x.get());
+          launch_store_column<T>(n_out, y.get(), D.get(), n_out, j, s);
+        }
+      return D;
+    };
+    auto copy_of = [&](const DBuf<double> &a) {
+      DBuf<double> b(a.size());
+      STFEM_HIP(hipMemcpyAsync(b.get(), a.get(), a.size() * sizeof(double),
+                               hipMemcpyDeviceToDevice, s));
+      return b;
+    };
+    auto identity = [&](long n) {
+      DBuf<double> I(size_t(n) * n);
+      launch_fill_zero<double>(I.get(), I.size(), s);
+      launch_add_diagonal(n, 1.0, I.get(), s);
+      return I;
+    };
+    DBuf<double> Mc;
+    long         nc = 0;
+    for (int l = 0; l <= lc; ++l)
+      {
+        const long n = lv_[l].A->m();
+        const auto &Lv = lv_[l];
+        DBuf<double> A = dense_of(n, n, [&](T *y, const T *x) { Lv.A->vmult(y, x); });
+        DBuf<double> V = dense_of(n, n, [&](T *y, const T *x) { Lv.V->vmult(y, x); });
+        const int sweeps = steps(l) * opt_.smoothing_steps;
+        // (Gs, S): s sweeps; G = I - w V A is the one-sweep iteration matrix
+        auto sweep_pair = [&](double w) {
+          DBuf<double> G = identity(n);
+          blas.gemm(n, n, n, -w, V.get(), A.get(), 1.0, G.get());
+          DBuf<double> H = copy_of(V);
+          launch_scale(long(H.size()), w, H.get(), s);
+          return std::make_pair(std::move(G), std::move(H));
+        };
+        DBuf<double> Gs, S;
+        if (exact_level(l))
+          {
+            auto gh = sweep_pair(double(exact_omega(l, sweeps)));
+            Gs      = std::move(gh.first);
+            S       = std::move(gh.second);
+          }
+        else
+          {
+            auto gh = sweep_pair(double(T(lv_[l].omega)));
+            DBuf<double> Gb = std::move(gh.first), Sb = std::move(gh.second);
+            bool         have = false;
+            for (int k = sweeps; k > 0; k >>= 1)
+              {
+                if (k & 1)
+                  {
+                    if (!have)
+                      {
+                        Gs   = copy_of(Gb);
+                        S    = copy_of(Sb);
+                        have = true;
+                      }
+                    else
+                      {
+                        // (Gs, S) followed by the base block (Gb, Sb)
+                        DBuf<double> S2 = copy_of(Sb);
+                        blas.gemm(n, n, n, 1.0, Gb.get(), S.get(), 1.0, S2.get());
+                        DBuf<double> G2(size_t(n) * n);
+                        blas.gemm(n, n, n, 1.0, Gb.get(), Gs.get(), 0.0, G2.get());
+                        S  = std::move(S2);
+                        Gs = std::move(G2);
+                      }
+                  }
+                if (k > 1)
+                  {
+                    DBuf<double> S2 = copy_of(Sb);
+                    blas.gemm(n, n, n, 1.0, Gb.get(), Sb.get(), 1.0, S2.get());
+                    DBuf<double> G2(size_t(n) * n);
+                    blas.gemm(n, n, n, 1.0, Gb.get(), Gb.get(), 0.0, G2.get());
+                    Sb = std::move(S2);
+                    Gb = std::move(G2);
+                  }
+              }
+          }
+        if (l == 0)
+          {
+            Mc = std::move(S);
+            nc = n;
+            continue;
+          }
+        const TwoLevelTransfer<T> &tr = tr_[l];
+        DBuf<double> P = dense_of(n, nc, [&](T *y, const T *x) {
+          launch_fill_zero<T>(y, size_t(n), s);
+          tr.prolongate_and_add(y, x);
+        });
+        DBuf<double> R = dense_of(nc, n, [&](T *y, const T *x) {
+          launch_fill_zero<T>(y, size_t(nc), s);
+          tr.restrict_and_add(y, x);
+        });
+        DBuf<double> X = identity(n); // I - A S
+        blas.gemm(n, n, n, -1.0, A.get(), S.get(), 1.0, X.get());
+        DBuf<double> Y(size_t(nc) * n); // R X
+        blas.gemm(nc, n, n, 1.0, R.get(), X.get(), 0.0, Y.get());
+        DBuf<double> Z(size_t(nc) * n); // M_{l-1} Y
+        blas.gemm(nc, n, nc, 1.0, Mc.get(), Y.get(), 0.0, Z.get());
+        DBuf<double> U = tr.prolongation_overwrites() ? DBuf<double>(size_t(n) * n) : copy_of(S);
+        blas.gemm(n, n, nc, 1.0, P.get(), Z.get(), tr.prolongation_overwrites() ? 0.0 : 1.0,
+                  U.get());
+        DBuf<double> M = copy_of(S); // Gs U + S
+        blas.gemm(n, n, n, 1.0, Gs.get(), U.get(), 1.0, M.get());
+        Mc = std::move(M);
+        nc = n;
+      }
+    const long vw = long(16 / sizeof(T));
+    dense_ld_     = (nc + vw - 1) / vw * vw;
+    dense_rows_.alloc(size_t(nc) * dense_ld_);
+    launch_pack_rows<T>(nc, Mc.get(), dense_rows_.get(), dense_ld_, s);
+    ctx_.sync();
+    dense_level_ = lc;
   }
 
   template <typename T>
@@ -1115,6 +1303,12 @@ namespace stfem
   template <typename T>
   void GMG<T>::level_v_step(int l) const
   {
+    if (l == dense_level_)
+      {
+        launch_gemv_rows<T>(lv_[l].A->m(), dense_ld_, dense_rows_.get(), def_[l].get(),
+                            sol_[l].get(), ctx_.stream);
+        return;
+      }
     if (l == 0)
       {
         smoother_apply(0, sol_[0].get(), def_[0].get());
@@ -1364,6 +1558,8 @@ namespace stfem
     getb("vanka_factored", ps.mg.vanka_factored);
     getb("exact_single_cell", ps.mg.exact_single_cell);
     getb("use_graph", ps.mg.use_graph);
+    if (kv.count("dense_coarse_dofs"))
+      ps.mg.dense_coarse_dofs = std::stol(kv["dense_coarse_dofs"]);
     geti("maxiter", ps.solver.maxiter);
     getd("abstol", ps.solver.abstol);
     getd("reltol", ps.solver.reltol);
@@ -1542,6 +1738,8 @@ namespace stfem
         info[7] = g.nc[1];
         info[8] = g.nc[2];
         info[9] = sizeof(T);
+        info[10] = l <= gmg_->dense_level() ? 1 : 0;
+        info[11] = l == gmg_->dense_level() ? g.n_nodes * L.A->n_blocks() : 0;
       }
       double level_omega(int l) const override { return gmg_->level(l).omega; }
       void level_vmult(int l, void *dst, const void *src, bool transpose) override

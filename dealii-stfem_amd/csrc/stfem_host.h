@@ -216,6 +216,7 @@ namespace stfem
     void restrict_and_add(T *dst, const T *src) const;
     int  nt_fine() const { return P_.m; }
     int  nt_coarse() const { return P_.n; }
+    bool overwrites() const { return overwrite_; }
     const DMat &prolongation() const { return P_; }
 
   private:
@@ -264,6 +265,8 @@ namespace stfem
     {
       time ? time->restrict_and_add(dst, src) : space->restrict_and_add(dst, src);
     }
+    // the prolongation replaces dst instead of adding to it
+    bool prolongation_overwrites() const { return time && time->overwrites(); }
   };
 
   // ------------------------------------------------------------ GMG
@@ -282,6 +285,12 @@ namespace stfem
     bool   exact_single_cell       = true;
     // replay the V-cycle from a captured HIP graph
     bool   use_graph               = true;
+    // The V-cycle restricted to the coarse levels is a fixed linear map of
+    // the restricted defect.  Below the finest level, the largest level with
+    // at most this many unknowns (and everything under it) is replaced by
+    // that map as one dense matrix, built at setup from the level kernels
+    // and applied as one matrix-vector product per V-cycle; 0 disables.
+    long   dense_coarse_dofs       = 8192;
   };
 
   // Level data of the hierarchy (precision T).
@@ -315,8 +324,11 @@ namespace stfem
     TwoLevelTransfer<T> &transfer(int l) { return tr_[l]; }
     T *level_defect(int l) const { return def_[l].get(); }
     T *level_solution(int l) const { return sol_[l].get(); }
+    // level whose V-cycle is one dense matrix-vector product (-1: none)
+    int dense_level() const { return dense_level_; }
 
   private:
+    void setup_dense_coarse();
     void relax(int l, T *u, const T *rhs, bool first) const;
     void relax_omega(int l, T *u, const T *rhs, bool first, T omega) const;
     bool exact_level(int l) const;
@@ -333,6 +345,9 @@ namespace stfem
     mutable std::vector<DBuf<T>>     sol_, def_, t1_, t2_;
     mutable hipGraph_t               graph_ = nullptr;
     mutable hipGraphExec_t           exec_  = nullptr;
+    int                              dense_level_ = -1;
+    long                             dense_ld_    = 0;
+    DBuf<T>                          dense_rows_; // row-major, rows padded to dense_ld_
   };
 
   // ------------------------------------------------------------ FGMRES

@@ -209,10 +209,10 @@ class Problem:
         self.close()
 
     def level_info(self, l):
-        info = np.zeros(10, np.int64)
+        info = np.zeros(12, np.int64)
         _check(load().stfem_level_info(self.h, l, info.ctypes.data_as(_lp)))
         keys = ["n_nodes", "nt", "n_cells", "r", "nts", "mg_type", "nc0", "nc1", "nc2",
-                "number_size"]
+                "number_size", "in_dense", "dense_dofs"]
         return dict(zip(keys, [int(x) for x in info]))
 
     def omega(self, l):

@@ -95,8 +95,11 @@ int stfem_problem_destroy(stfem_problem p);
  * number), n_timesteps_at_once */
 int stfem_problem_info(stfem_problem p, long *info);
 int stfem_problem_tau(stfem_problem p, double *tau);
-/* info[0..9] = n_nodes, nt, n_cells, r, nts, mg type to this level,
- * cells x, y, z, sizeof(number) */
+/* info[0..11] = n_nodes, nt, n_cells, r, nts, mg type to this level,
+ * cells x, y, z, sizeof(number), 1 if the level lies under the dense coarse
+ * operator (spec key dense_coarse_dofs: the V-cycle from that level down is
+ * one precomputed matrix) else 0, and on the level where that matrix is
+ * applied its order (else 0) */
 int stfem_level_info(stfem_problem p, int level, long *info);
 /* relaxation parameter of the level (GMG::estimate_relaxation, gmg.h:792-844) */
 int stfem_level_omega(stfem_problem p, int level, double *omega);

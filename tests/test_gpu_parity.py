@@ -223,6 +223,49 @@ def test_graph_replay_matches_eager():
     ge.close()
 
 
+DENSE_CASES = ["3d_q3_dg2", "3d_q3_dg2_fp32_factored", "3d_q4_dg3_fp32", "2d_dg1_nts2",
+               "3d_q2_cg2", "2d_cg2_nts2_fp32"]
+
+
+@pytest.mark.parametrize("dense", [0, 200])
+@pytest.mark.parametrize("name", DENSE_CASES)
+def test_dense_coarse_operator(name, dense):
+    """The V-cycle whose coarse part is one precomputed matrix (default: the
+    largest level below the finest with at most 8192 unknowns) equals the
+    V-cycle launched level by level (dense_coarse_dofs=0) and one collapsed
+    at a smaller level (200 unknowns); both match the oracle."""
+    spec, o, g = problems(name)
+    c = CASES[name]
+    dim = spec["dim"]
+    nc = [1 << spec["refine"]] * dim
+    X = box_vertices(dim, nc, perturb=c["perturb"], seed=7) if c["perturb"] > 0 else None
+    kap = log_uniform(int(np.prod(nc)), 1e-2, 1e2, seed=11) if c["kappa"] else None
+    h = st.Problem(dict(spec, dense_coarse_dofs=dense), vertices=X, kappa=kap)
+    full = [g.level_info(l) for l in range(g.n_levels)]
+    part = [h.level_info(l) for l in range(h.n_levels)]
+    L = g.n_levels - 1
+    assert full[L]["in_dense"] == 0 and full[L - 1]["in_dense"] == 1, full
+    for i in part:
+        m = i["n_nodes"] * i["nt"]
+        if dense == 0:
+            assert i["in_dense"] == 0, part
+        elif i["in_dense"]:
+            assert m <= dense
+    rng = np.random.default_rng(23)
+    n, nt = g.n_nodes, g.nt
+    x = rng.standard_normal(n * nt)
+    xg = torch.tensor(blocks_to_nodes(x, nt), dtype=torch.float64, device=DEV)
+    y1, y2 = torch.zeros_like(xg), torch.zeros_like(xg)
+    print(f"[case {name}] dense_coarse_dofs {dense}: gmg vmult", flush=True)
+    g.gmg_vmult(y1, xg)
+    h.gmg_vmult(y2, xg)
+    fp32 = g.level_size == 4
+    assert rel_err(y1.cpu().numpy(), y2.cpu().numpy()) < (1e-4 if fp32 else 1e-10)
+    ref = o.gmg_vmult(x)
+    assert rel_err(nodes_to_blocks(y2.cpu().numpy(), nt), ref) < (1e-4 if fp32 else 1e-10)
+    h.close()
+
+
 def test_single_cell_levels_shortcut():
     """s smoothing sweeps on a one-cell level (where the Vanka patch is the
     whole level) equal one sweep with 1-(1-omega)^s."""
