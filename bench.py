@@ -194,6 +194,8 @@ def main():
                     help="skip the two rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--dense-coarse-dofs", type=int, default=None,
                     help="largest coarse level collapsed into one dense matrix (0: off)")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="override a solver spec key (e.g. atomic_scatter=1)")
     ap.add_argument("--kernel-only", choices=["vanka", "vmult32", "vmult64"], default=None)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -206,6 +208,9 @@ def main():
     spec, X, kappa = workload(a.refine)
     if a.dense_coarse_dofs is not None:
         spec["dense_coarse_dofs"] = a.dense_coarse_dofs
+    for kv in a.set:
+        k, v = kv.split("=", 1)
+        spec[k] = int(v) if v.lstrip("-").isdigit() else float(v)
     if a.kernel_only:
         return kernel_only(a, spec, X, kappa)
 

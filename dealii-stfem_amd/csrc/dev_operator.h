@@ -65,6 +65,26 @@ namespace stfem
       }
     };
 
+    // Position of the cell-local value (cell (cx,cy,cz), local node
+    // (a,b,c), output component j) in the cell-major result buffer: for each
+    // row of cells along x, local (c,b) and component, the values of all
+    // cells of the row lie contiguous in x ((cx, a) with a fastest), so that
+    // both the per-cell stores and the per-node gathers are coalesced.  In
+    // 2D the local line index is b (y) and c is unused.
+    template <int DIM, int N>
+    __device__ __forceinline__ long cell_slot(const GridDesc &g, int nto, int cx, int cy, int cz,
+                                              int a, int b, int c, int j)
+    {
+      const long L = long(g.nc[0]) * N;
+      if constexpr (DIM == 3)
+        {
+          const long row = long(cz) * g.nc[1] + cy;
+          return (((row * N + c) * N + b) * nto + j) * L + long(cx) * N + a;
+        }
+      else
+        return ((long(cy) * N + b) * nto + j) * L + long(cx) * N + a;
+    }
+
     // One group of threads applies the operator to the CPB cells
     // [blk*CPB, blk*CPB + CPB) (cells past the end contribute nothing).
     // tid in [0, 256).  When the group has more than CPB*TPC threads (256
@@ -74,7 +94,11 @@ namespace stfem
     // slot (CPB*NC suffices for groups of exactly CPB*TPC threads).  Every
     // thread of the workgroup must call this with the same j0/jn: it
     // synchronises the workgroup a fixed number of times.
-    template <int DIM, int N, int JB, typename T>
+    // TO_CELLS: the cell results are stored into the cell-major buffer dst
+    // (cell_slot layout, every local node including Dirichlet ones);
+    // otherwise they are added to the node vector dst with atomics,
+    // skipping Dirichlet nodes.
+    template <int DIM, int N, int JB, typename T, bool TO_CELLS>
     __device__ __forceinline__ void st_apply_cells(const StOpArgs<T> &A, const T *__restrict__ src,
                                                    T *__restrict__ dst, int j0, int jn, long blk,
                                                    int tid, T *__restrict__ lds0,
@@ -527,10 +551,93 @@ namespace stfem
                 }
             }
           if (valid)
+            {
+              if constexpr (TO_CELLS)
+                {
 #pragma unroll
-            for (int c = 0; c < N; ++c)
-              if (!((cmask >> c) & 1u))
-                atomicAdd(&dst[(base + c * kstride) * nto + j], out[c]);
+                  for (int c = 0; c < N; ++c)
+                    {
+                      const long slot = (DIM == 3) ? cell_slot<DIM, N>(A.g, nto, cx, cy, cz, ia, ib, c, j)
+                                                   : cell_slot<DIM, N>(A.g, nto, cx, cy, cz, ia, c, 0, j);
+                      dst[slot] = out[c];
+                    }
+                }
+              else
+                {
+#pragma unroll
+                  for (int c = 0; c < N; ++c)
+                    if (!((cmask >> c) & 1u))
+                      atomicAdd(&dst[(base + c * kstride) * nto + j], out[c]);
+                }
+            }
+        }
+    }
+
+    // Second pass of the two-pass operator: node `node` of the level sums
+    // its (one to 2^dim) cell-local results from the cell-major buffer, in a
+    // fixed order, into dst (overwriting, or adding when accumulate);
+    // Dirichlet nodes get zero (overwrite) or stay unchanged (accumulate).
+    template <int DIM, int N, typename T>
+    __device__ __forceinline__ void gather_node(const GridDesc &g, int nto,
+                                                const T *__restrict__ cells, T *__restrict__ dst,
+                                                bool accumulate, long node)
+    {
+      constexpr int p   = N - 1;
+      const long    nn0 = g.nn[0], nn1 = g.nn[1];
+      const long    gx  = node % nn0, r = node / nn0;
+      const long    gy  = (DIM == 3) ? r % nn1 : r;
+      const long    gz  = (DIM == 3) ? r / nn1 : 0;
+      const int     d   = g.dirichlet;
+      bool bnd = (gx == 0 && (d & 1)) || (gx == nn0 - 1 && (d & 2)) || (gy == 0 && (d & 4)) ||
+                 (gy == nn1 - 1 && (d & 8));
+      if (DIM == 3)
+        bnd = bnd || (gz == 0 && (d & 16)) || (gz == g.nn[2] - 1 && (d & 32));
+      T *const out = dst + node * nto;
+      if (bnd)
+        {
+          if (!accumulate)
+            for (int j = 0; j < nto; ++j)
+              out[j] = T(0);
+          return;
+        }
+      // cells (and local indices) sharing the node, per direction
+      int cs[3][2], ls[3][2], nsh[3];
+      const long gc[3] = {gx, gy, gz};
+#pragma unroll
+      for (int e = 0; e < DIM; ++e)
+        {
+          int c = int(gc[e] / p), l = int(gc[e] - long(c) * p);
+          if (c == g.nc[e])
+            c = g.nc[e] - 1, l = p;
+          cs[e][0] = c;
+          ls[e][0] = l;
+          nsh[e]   = 1;
+          if (l == 0 && c > 0)
+            {
+              cs[e][1] = c - 1;
+              ls[e][1] = p;
+              nsh[e]   = 2;
+            }
+        }
+      for (int j = 0; j < nto; ++j)
+        {
+          T s = T(0);
+          if constexpr (DIM == 3)
+            {
+              for (int kz = 0; kz < nsh[2]; ++kz)
+                for (int ky = 0; ky < nsh[1]; ++ky)
+                  for (int kx = 0; kx < nsh[0]; ++kx)
+                    s += cells[cell_slot<DIM, N>(g, nto, cs[0][kx], cs[1][ky], cs[2][kz], ls[0][kx],
+                                                 ls[1][ky], ls[2][kz], j)];
+            }
+          else
+            {
+              for (int ky = 0; ky < nsh[1]; ++ky)
+                for (int kx = 0; kx < nsh[0]; ++kx)
+                  s += cells[cell_slot<DIM, N>(g, nto, cs[0][kx], cs[1][ky], 0, ls[0][kx], ls[1][ky],
+                                               0, j)];
+            }
+          out[j] = accumulate ? out[j] + s : s;
         }
     }
 

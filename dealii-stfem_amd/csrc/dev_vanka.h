@@ -240,11 +240,29 @@ namespace stfem
         {
           M.cell_xyz(cell, cx, cy, cz);
           const T *Sg = S + (size_t)cell * NL * NL;
-          for (int l = tid; l < NL * NL; l += 256)
+          constexpr int VW = 16 / int(sizeof(T));
+          if constexpr ((NL * NL) % VW == 0)
             {
-              const int a = l / NL, k = l - a * NL;
-              ss[a * LD + k] = Sg[l];
+              // 16-byte loads: every cell's block starts 16-byte aligned
+              typedef T vec_t __attribute__((ext_vector_type(VW)));
+              const vec_t *Sv = reinterpret_cast<const vec_t *>(Sg);
+              for (int l4 = tid; l4 < NL * NL / VW; l4 += 256)
+                {
+                  const vec_t v = Sv[l4];
+#pragma unroll
+                  for (int e = 0; e < VW; ++e)
+                    {
+                      const int l = l4 * VW + e, a = l / NL, k = l - a * NL;
+                      ss[a * LD + k] = v[e];
+                    }
+                }
             }
+          else
+            for (int l = tid; l < NL * NL; l += 256)
+              {
+                const int a = l / NL, k = l - a * NL;
+                ss[a * LD + k] = Sg[l];
+              }
           for (int l = tid; l < NT * NL; l += 256)
             {
               const int  i = l / NL, j = l - i * NL;

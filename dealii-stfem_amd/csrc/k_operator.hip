@@ -5,15 +5,19 @@
 // calls and the nt^2 block axpys of SystemMatrix::vmult
 // (tests/include/operators.h:94-122 and :487-548 of nlsmrg/dealii-stfem).
 // Work decomposition: a 256-thread workgroup holds CPB cells; the per-cell
-// device code (sum factorization, on-the-fly geometry, atomic scatter) is in
-// dev_operator.h.
+// device code (sum factorization, on-the-fly geometry) is in dev_operator.h.
+// Two-pass form (default): the cell kernel stores each cell's results into a
+// cell-major buffer with coalesced plain stores and a node kernel sums the
+// (up to 2^dim) contributions of every node in a fixed order.  This avoids
+// the device-scope float atomics of the one-pass form, which are performed
+// one at a time at the memory side and cost more than the whole cell pass.
 #include "dev_operator.h"
 
 namespace stfem
 {
   namespace
   {
-    template <int DIM, int N, int JB, typename T>
+    template <int DIM, int N, int JB, typename T, bool TO_CELLS>
     __global__ void __launch_bounds__(256)
       k_st_apply(const StOpArgs<T> A, const T *__restrict__ src, T *__restrict__ dst, int j0,
                  int jn)
@@ -21,11 +25,21 @@ namespace stfem
       using Ly = Layout<DIM, N>;
       __shared__ T lds0[Ly::CPB * Ly::NC];
       __shared__ T lds1[Ly::CPB * Ly::NC];
-      st_apply_cells<DIM, N, JB, T>(A, src, dst, j0, jn, (long)blockIdx.x, (int)threadIdx.x, lds0,
-                                    lds1);
+      st_apply_cells<DIM, N, JB, T, TO_CELLS>(A, src, dst, j0, jn, (long)blockIdx.x,
+                                              (int)threadIdx.x, lds0, lds1);
     }
 
     template <int DIM, int N, typename T>
+    __global__ void __launch_bounds__(256)
+      k_gather_nodes(const GridDesc g, int nto, const T *__restrict__ cells, T *__restrict__ dst,
+                     int accumulate)
+    {
+      const long node = (long)blockIdx.x * 256 + threadIdx.x;
+      if (node < g.n_nodes)
+        gather_node<DIM, N, T>(g, nto, cells, dst, accumulate != 0, node);
+    }
+
+    template <int DIM, int N, typename T, bool TO_CELLS>
     void dispatch_jb(const StOpArgs<T> &a, const T *src, T *dst, hipStream_t s)
     {
       using Ly              = Layout<DIM, N>;
@@ -41,16 +55,16 @@ namespace stfem
           switch (jb)
             {
               case 1:
-                k_st_apply<DIM, N, 1, T><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
+                k_st_apply<DIM, N, 1, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
                 break;
               case 2:
-                k_st_apply<DIM, N, 2, T><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
+                k_st_apply<DIM, N, 2, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
                 break;
               case 3:
-                k_st_apply<DIM, N, 3, T><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
+                k_st_apply<DIM, N, 3, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
                 break;
               default:
-                k_st_apply<DIM, N, 4, T><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
+                k_st_apply<DIM, N, 4, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
                 break;
             }
         }
@@ -65,8 +79,28 @@ namespace stfem
     upload_tables_here(S, D, W, X, N);
   }
 
+  namespace
+  {
+    template <int DIM, int N, typename T>
+    void apply_dn(const StOpArgs<T> &a, const T *src, T *dst, T *cells, bool accumulate,
+                  hipStream_t s)
+    {
+      if (!cells)
+        {
+          if (!accumulate)
+            launch_fill_zero<T>(dst, size_t(a.g.n_nodes) * a.nt_out, s);
+          return dispatch_jb<DIM, N, T, false>(a, src, dst, s);
+        }
+      dispatch_jb<DIM, N, T, true>(a, src, cells, s);
+      k_gather_nodes<DIM, N, T><<<dim3((unsigned)((a.g.n_nodes + 255) / 256)), 256, 0, s>>>(
+        a.g, a.nt_out, cells, dst, accumulate ? 1 : 0);
+      STFEM_HIP(hipGetLastError());
+    }
+  } // namespace
+
   template <typename T>
-  void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, hipStream_t s)
+  void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, T *cells, bool accumulate,
+                       hipStream_t s)
   {
     STFEM_REQUIRE(a.nt_in >= 1 && a.nt_out >= 1, "st_apply: bad number of time components");
     if (a.g.n_cells == 0)
@@ -75,26 +109,26 @@ namespace stfem
     if (a.g.dim == 2)
       switch (N)
         {
-          case 2: return dispatch_jb<2, 2, T>(a, src, dst, s);
-          case 3: return dispatch_jb<2, 3, T>(a, src, dst, s);
-          case 4: return dispatch_jb<2, 4, T>(a, src, dst, s);
-          case 5: return dispatch_jb<2, 5, T>(a, src, dst, s);
+          case 2: return apply_dn<2, 2, T>(a, src, dst, cells, accumulate, s);
+          case 3: return apply_dn<2, 3, T>(a, src, dst, cells, accumulate, s);
+          case 4: return apply_dn<2, 4, T>(a, src, dst, cells, accumulate, s);
+          case 5: return apply_dn<2, 5, T>(a, src, dst, cells, accumulate, s);
           default: break;
         }
     else if (a.g.dim == 3)
       switch (N)
         {
-          case 2: return dispatch_jb<3, 2, T>(a, src, dst, s);
-          case 3: return dispatch_jb<3, 3, T>(a, src, dst, s);
-          case 4: return dispatch_jb<3, 4, T>(a, src, dst, s);
-          case 5: return dispatch_jb<3, 5, T>(a, src, dst, s);
+          case 2: return apply_dn<3, 2, T>(a, src, dst, cells, accumulate, s);
+          case 3: return apply_dn<3, 3, T>(a, src, dst, cells, accumulate, s);
+          case 4: return apply_dn<3, 4, T>(a, src, dst, cells, accumulate, s);
+          case 5: return apply_dn<3, 5, T>(a, src, dst, cells, accumulate, s);
           default: break;
         }
     throw Error(STATUS_ERR_UNSUPPORTED, "st_apply: unsupported dim/degree");
   }
 
   template void launch_st_apply<double>(const StOpArgs<double> &, const double *, double *,
-                                        hipStream_t);
-  template void launch_st_apply<float>(const StOpArgs<float> &, const float *, float *,
-                                       hipStream_t);
+                                        double *, bool, hipStream_t);
+  template void launch_st_apply<float>(const StOpArgs<float> &, const float *, float *, float *,
+                                       bool, hipStream_t);
 } // namespace stfem

@@ -21,8 +21,8 @@ namespace stfem
   // ------------------------------------------------------- space-time op
   // dst[node][j] (+)= sum_i Alpha(j,i) (K src_i)[node] + Beta(j,i) (M src_i)[node]
   // K: stiffness int kappa grad phi . grad phi (Q1 mapping), M: mass.
-  // Dirichlet nodes are read as zero and never written.  dst must be
-  // zeroed by the caller (atomic accumulation).
+  // Dirichlet nodes are read as zero; their dst entries are set to zero
+  // (or left unchanged when accumulating).
   template <typename T>
   struct StOpArgs
   {
@@ -35,8 +35,19 @@ namespace stfem
     const T      *beta;      // nt_out x nt_in (mass weights), device
     int           nt_in, nt_out;
   };
+  // cells != null: two-pass form with the cell-major scratch `cells` of
+  // st_cells_size() values; cells == null: one pass with atomic adds.
+  // accumulate: dst += result instead of dst = result.
   template <typename T>
-  void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, hipStream_t s);
+  void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, T *cells, bool accumulate,
+                       hipStream_t s);
+  inline size_t st_cells_size(const GridDesc &g, int nt_out)
+  {
+    size_t nc = 1;
+    for (int d = 0; d < g.dim; ++d)
+      nc *= size_t(g.p + 1);
+    return size_t(g.n_cells) * nc * size_t(nt_out);
+  }
 
   // ------------------------------------------------------------ Vanka
   // u += omega * sum_cells (1/valence) * scatter( Binv_c * gather(rhs - ax) )

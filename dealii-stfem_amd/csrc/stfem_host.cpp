@@ -385,7 +385,7 @@ namespace stfem
   // ========================================================== SystemMatrix
   template <typename T>
   SystemMatrix<T>::SystemMatrix(Context &ctx, const MeshLevel &m, const DMat &Alpha,
-                                const DMat &Beta)
+                                const DMat &Beta, bool two_pass)
     : ctx_(ctx)
     , grid_(m.g)
     , coef_mode_(m.coef_mode)
@@ -397,6 +397,8 @@ namespace stfem
   {
     STFEM_REQUIRE(Alpha.m == Beta.m && Alpha.n == Beta.n, "SystemMatrix: Alpha/Beta shapes differ");
     verts_.upload(m.X.data(), m.X.size(), ctx.stream);
+    if (two_pass)
+      cells_.alloc(st_cells_size(grid_, nt_out_));
     if (coef_mode_ != 0)
       {
         std::vector<T> c(m.coef.begin(), m.coef.end());
@@ -419,8 +421,6 @@ namespace stfem
   void SystemMatrix<T>::apply(T *dst, const T *src, const DBuf<T> &a, const DBuf<T> &b,
                               bool accumulate) const
   {
-    if (!accumulate)
-      launch_fill_zero<T>(dst, size_t(grid_.n_nodes) * nt_out_, ctx_.stream);
     StOpArgs<T> args;
     args.g          = grid_;
     args.vertices   = verts_.get();
@@ -431,7 +431,7 @@ namespace stfem
     args.beta       = b.get();
     args.nt_in      = nt_in_;
     args.nt_out     = nt_out_;
-    launch_st_apply<T>(args, src, dst, ctx_.stream);
+    launch_st_apply<T>(args, src, dst, cells_.get(), accumulate, ctx_.stream);
   }
   template <typename T>
   void SystemMatrix<T>::vmult(T *dst, const T *src) const
@@ -1549,6 +1549,7 @@ namespace stfem
     geti("mg_space", ps.mg_space);
     geti("mg_float", ps.mg_float);
     geti("dirichlet", ps.dirichlet);
+    geti("atomic_scatter", ps.atomic_scatter);
     geti("smoothing_steps", ps.mg.smoothing_steps);
     getb("variable", ps.mg.variable);
     getb("estimate_relaxation", ps.mg.estimate_relaxation);
@@ -1647,7 +1648,8 @@ namespace stfem
           {
             const TimeWeights w = get_fe_time_weights(spec.type, rl[l], tl[l], nl[l]);
             const MeshLevel  &m = mesh_[sref[l]];
-            levels[l].A  = std::make_unique<SystemMatrix<T>>(ctx_, m, w.Alpha, w.Beta);
+            levels[l].A  = std::make_unique<SystemMatrix<T>>(ctx_, m, w.Alpha, w.Beta,
+                                                             spec.atomic_scatter == 0);
             levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form);
             levels[l].r  = rl[l];
             levels[l].nts = nl[l];
@@ -1670,11 +1672,12 @@ namespace stfem
         const MeshLevel  &F = mesh_.back();
         const TimeWeights w = get_fe_time_weights(spec.type, spec.r, tau, spec.nts);
         nt                  = w.Alpha.m;
-        fine_ = std::make_unique<SystemMatrix<double>>(ctx_, F, w.Alpha, w.Beta);
+        fine_ = std::make_unique<SystemMatrix<double>>(ctx_, F, w.Alpha, w.Beta,
+                                                       spec.atomic_scatter == 0);
         DMat zero(w.Gamma.m, 1);
         const DMat rK = (spec.type == TimeStepType::CGP) ? w.Gamma : zero;
         const DMat rM = (spec.type == TimeStepType::CGP) ? w.Zeta : w.Gamma;
-        rhs_ = std::make_unique<SystemMatrix<double>>(ctx_, F, rK, rM);
+        rhs_ = std::make_unique<SystemMatrix<double>>(ctx_, F, rK, rM, spec.atomic_scatter == 0);
         const TimeWeights w1 = get_fe_time_weights(spec.type, spec.r, tau, 1);
         alpha1_ = w1.Alpha;
         gamma1_ = w1.Gamma;

@@ -143,7 +143,10 @@ namespace stfem
   class SystemMatrix
   {
   public:
-    SystemMatrix(Context &ctx, const MeshLevel &m, const DMat &Alpha, const DMat &Beta);
+    // two_pass: cell results through a cell-major scratch buffer and a
+    // per-node gather (default); otherwise one pass with atomic adds
+    SystemMatrix(Context &ctx, const MeshLevel &m, const DMat &Alpha, const DMat &Beta,
+                 bool two_pass = true);
     // dst = (Alpha x K + Beta x M) src   (operators.h:94-122)
     void vmult(T *dst, const T *src) const;
     // dst = (Alpha^T x K + Beta^T x M) src   (operators.h:124-152)
@@ -171,6 +174,7 @@ namespace stfem
     DMat         A_, B_;
     int          nt_in_, nt_out_;
     DBuf<T>      dA_, dB_, dAt_, dBt_;
+    mutable DBuf<T> cells_; // cell-major results of the two-pass form
   };
 
   // ------------------------------------------------------ PreconditionVanka
@@ -402,6 +406,7 @@ namespace stfem
     int          mg_space = 1;
     int          mg_float = 0;
     int          dirichlet = 63;
+    int          atomic_scatter = 0; // 1: one-pass operator with atomic adds
     GMGOptions   mg;
     SolverControlData solver;
     int          extrapolate = 1;
