@@ -575,12 +575,15 @@ namespace stfem
 
     // Second pass of the two-pass operator: node `node` of the level sums
     // its (one to 2^dim) cell-local results from the cell-major buffer, in a
-    // fixed order, into dst (overwriting, or adding when accumulate);
-    // Dirichlet nodes get zero (overwrite) or stay unchanged (accumulate).
-    template <int DIM, int N, typename T>
+    // fixed order (cells by increasing x, then y, then z), and stores
+    //   mode 0: dst = sum,  mode 1: dst += sum,  mode 2: dst = rhs - sum.
+    // Dirichlet nodes get zero (modes 0, 2) or stay unchanged (mode 1).
+    // NT > 0: compile-time number of components (all loads issued before
+    // the sums); NT == 0: nto at run time.
+    template <int DIM, int N, int NT, typename T>
     __device__ __forceinline__ void gather_node(const GridDesc &g, int nto,
                                                 const T *__restrict__ cells, T *__restrict__ dst,
-                                                bool accumulate, long node)
+                                                const T *__restrict__ rhs, int mode, long node)
     {
       constexpr int p   = N - 1;
       const long    nn0 = g.nn[0], nn1 = g.nn[1];
@@ -592,16 +595,17 @@ namespace stfem
                  (gy == nn1 - 1 && (d & 8));
       if (DIM == 3)
         bnd = bnd || (gz == 0 && (d & 16)) || (gz == g.nn[2] - 1 && (d & 32));
-      T *const out = dst + node * nto;
+      const int nt  = NT > 0 ? NT : nto;
+      T *const  out = dst + node * nt;
       if (bnd)
         {
-          if (!accumulate)
-            for (int j = 0; j < nto; ++j)
+          if (mode != 1)
+            for (int j = 0; j < nt; ++j)
               out[j] = T(0);
           return;
         }
-      // cells (and local indices) sharing the node, per direction
-      int cs[3][2], ls[3][2], nsh[3];
+      // the cells sharing the node and its local index in each, per direction
+      int cs[3][2] = {}, ls[3][2] = {}, nsh[3] = {1, 1, 1};
       const long gc[3] = {gx, gy, gz};
 #pragma unroll
       for (int e = 0; e < DIM; ++e)
@@ -609,36 +613,66 @@ namespace stfem
           int c = int(gc[e] / p), l = int(gc[e] - long(c) * p);
           if (c == g.nc[e])
             c = g.nc[e] - 1, l = p;
-          cs[e][0] = c;
-          ls[e][0] = l;
-          nsh[e]   = 1;
+          // lower neighbour first: fixed summation order
           if (l == 0 && c > 0)
             {
-              cs[e][1] = c - 1;
-              ls[e][1] = p;
+              cs[e][0] = c - 1;
+              ls[e][0] = p;
+              cs[e][1] = c;
+              ls[e][1] = 0;
               nsh[e]   = 2;
-            }
-        }
-      for (int j = 0; j < nto; ++j)
-        {
-          T s = T(0);
-          if constexpr (DIM == 3)
-            {
-              for (int kz = 0; kz < nsh[2]; ++kz)
-                for (int ky = 0; ky < nsh[1]; ++ky)
-                  for (int kx = 0; kx < nsh[0]; ++kx)
-                    s += cells[cell_slot<DIM, N>(g, nto, cs[0][kx], cs[1][ky], cs[2][kz], ls[0][kx],
-                                                 ls[1][ky], ls[2][kz], j)];
             }
           else
             {
-              for (int ky = 0; ky < nsh[1]; ++ky)
-                for (int kx = 0; kx < nsh[0]; ++kx)
-                  s += cells[cell_slot<DIM, N>(g, nto, cs[0][kx], cs[1][ky], 0, ls[0][kx], ls[1][ky],
-                                               0, j)];
+              cs[e][0] = c;
+              ls[e][0] = l;
             }
-          out[j] = accumulate ? out[j] + s : s;
         }
+      constexpr int MAXS = 1 << DIM;
+      long          slot[MAXS];
+      int           ns = 0;
+#pragma unroll
+      for (int kz = 0; kz < (DIM == 3 ? 2 : 1); ++kz)
+#pragma unroll
+        for (int ky = 0; ky < 2; ++ky)
+#pragma unroll
+          for (int kx = 0; kx < 2; ++kx)
+            {
+              const bool use = kx < nsh[0] && ky < nsh[1] && (DIM == 2 || kz < nsh[2]);
+              if (use)
+                slot[ns] = (DIM == 3) ? cell_slot<DIM, N>(g, nt, cs[0][kx], cs[1][ky], cs[2][kz],
+                                                          ls[0][kx], ls[1][ky], ls[2][kz], 0)
+                                      : cell_slot<DIM, N>(g, nt, cs[0][kx], cs[1][ky], 0,
+                                                          ls[0][kx], ls[1][ky], 0, 0);
+              ns += use ? 1 : 0;
+            }
+      const long L = long(g.nc[0]) * N; // component stride of cell_slot
+      if constexpr (NT > 0)
+        {
+          T v[MAXS][NT];
+#pragma unroll
+          for (int k = 0; k < MAXS; ++k)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+              v[k][j] = k < ns ? cells[slot[k] + j * L] : T(0);
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            {
+              T s = v[0][j];
+#pragma unroll
+              for (int k = 1; k < MAXS; ++k)
+                s += v[k][j];
+              out[j] = mode == 0 ? s : mode == 1 ? out[j] + s : rhs[node * NT + j] - s;
+            }
+        }
+      else
+        for (int j = 0; j < nt; ++j)
+          {
+            T s = T(0);
+            for (int k = 0; k < ns; ++k)
+              s += cells[slot[k] + j * L];
+            out[j] = mode == 0 ? s : mode == 1 ? out[j] + s : rhs[node * nt + j] - s;
+          }
     }
 
     // copy the 1D tables of FE_Q(N-1) into this translation unit's constant memory

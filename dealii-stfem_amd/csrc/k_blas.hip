@@ -1,10 +1,11 @@
 // Vector kernels of the Krylov and multigrid drivers (gfx950).
 //
 // Reductions are two-stage with a fixed grid so results are bitwise
-// reproducible run to run: stage 1 writes one partial sum per workgroup,
-// stage 2 (one workgroup) sums the partials in a fixed order and may post-
-// process (sqrt) and store to a device scalar, e.g. a Hessenberg entry, so
-// the Gram-Schmidt chain needs no host round trip.
+// reproducible run to run: stage 1 writes one partial sum per workgroup;
+// stage 2 sums the partials in a fixed order, either in a one-workgroup
+// kernel, or at the start of the kernel that consumes the result (the
+// Gram-Schmidt chain: each step re-reduces the previous step's partials, so
+// the chain needs no extra launches and no host round trip), or on the host.
 #include "kernels.h"
 
 #include <algorithm>
@@ -55,22 +56,38 @@ namespace stfem
         out[0] = (op == 1) ? sqrt(s) : s;
     }
 
+    // x += factor * c * y, c = sum of the previous reduction's partials
+    // `pin` (every workgroup sums them in the same order, so all use the
+    // same c; workgroup 0 also stores c to *hout), then the partial sums of
+    // dot(x, z) (z null: dot(x, x)) into `pout`.
     __global__ void __launch_bounds__(RT)
-      k_axpy_dot_partial(long n, double *__restrict__ x, const double *coef, double factor,
-                         const double *__restrict__ y, const double *__restrict__ z,
-                         double *partial)
+      k_axpy_dot_chain(long n, double *__restrict__ x, const double *__restrict__ pin,
+                       double factor, const double *__restrict__ y,
+                       const double *__restrict__ z, double *__restrict__ pout, double *hout)
     {
-      const double c = (coef ? coef[0] : 1.0) * factor;
+      __shared__ double cs;
+      double v = 0.0;
+      for (int i = threadIdx.x; i < RB; i += RT)
+        v += pin[i];
+      v = block_sum(v);
+      if (threadIdx.x == 0)
+        {
+          cs = v;
+          if (blockIdx.x == 0 && hout)
+            *hout = v;
+        }
+      __syncthreads();
+      const double c = cs * factor;
       double       s = 0.0;
       for (long i = (long)blockIdx.x * RT + threadIdx.x; i < n; i += (long)RB * RT)
         {
-          const double v = x[i] + c * y[i];
-          x[i]           = v;
-          s += v * (z ? z[i] : v);
+          const double w = x[i] + c * y[i];
+          x[i]           = w;
+          s += w * (z ? z[i] : w);
         }
       s = block_sum(s);
       if (threadIdx.x == 0)
-        partial[blockIdx.x] = s;
+        pout[blockIdx.x] = s;
     }
 
     template <typename T>
@@ -249,12 +266,45 @@ namespace stfem
     STFEM_HIP(hipGetLastError());
   }
 
-  void launch_axpy_dot(long n, double *x, const double *coef, double factor, const double *y,
-                       const double *z, double *partial, double *out, int op, hipStream_t s)
+  template <typename T>
+  void launch_dot_partials(long n, const T *x, const T *y, double *partial, hipStream_t s)
   {
-    k_axpy_dot_partial<<<RB, RT, 0, s>>>(n, x, coef, factor, y, z, partial);
-    k_finish<<<1, RT, 0, s>>>(partial, out, op);
+    k_dot_partial<T><<<RB, RT, 0, s>>>(n, x, y, partial);
     STFEM_HIP(hipGetLastError());
+  }
+
+  void launch_axpy_dot_chain(long n, double *x, const double *pin, double factor, const double *y,
+                             const double *z, double *pout, double *hout, hipStream_t s)
+  {
+    k_axpy_dot_chain<<<RB, RT, 0, s>>>(n, x, pin, factor, y, z, pout, hout);
+    STFEM_HIP(hipGetLastError());
+  }
+
+  double sum_partials(const double *h)
+  {
+    // the stage-2 order of k_finish: thread t sums t, t+RT, ...; then the
+    // wavefront tree and the four wavefront sums
+    double part[RT];
+    for (int t = 0; t < RT; ++t)
+      {
+        double v = 0.0;
+        for (int i = t; i < RB; i += RT)
+          v += h[i];
+        part[t] = v;
+      }
+    double w[RT / 64];
+    for (int g = 0; g < RT / 64; ++g)
+      {
+        double *p = part + 64 * g;
+        for (int off = 32; off > 0; off >>= 1)
+          for (int l = 0; l < off; ++l)
+            p[l] += p[l + off];
+        w[g] = p[0];
+      }
+    double r = 0.0;
+    for (int g = 0; g < RT / 64; ++g)
+      r += w[g];
+    return r;
   }
 
   template <typename T>
@@ -382,6 +432,10 @@ namespace stfem
                                         hipStream_t);
   template void launch_dot<double>(long, const double *, const double *, double *, double *, int,
                                    hipStream_t);
+  template void launch_dot_partials<double>(long, const double *, const double *, double *,
+                                            hipStream_t);
+  template void launch_dot_partials<float>(long, const float *, const float *, double *,
+                                           hipStream_t);
   template void launch_dot<float>(long, const float *, const float *, double *, double *, int,
                                   hipStream_t);
   template void launch_axpby<double>(long, double, const double *, double, double *, hipStream_t);

@@ -29,14 +29,14 @@ namespace stfem
                                               (int)threadIdx.x, lds0, lds1);
     }
 
-    template <int DIM, int N, typename T>
+    template <int DIM, int N, int NT, typename T>
     __global__ void __launch_bounds__(256)
       k_gather_nodes(const GridDesc g, int nto, const T *__restrict__ cells, T *__restrict__ dst,
-                     int accumulate)
+                     const T *__restrict__ rhs, int mode)
     {
       const long node = (long)blockIdx.x * 256 + threadIdx.x;
       if (node < g.n_nodes)
-        gather_node<DIM, N, T>(g, nto, cells, dst, accumulate != 0, node);
+        gather_node<DIM, N, NT, T>(g, nto, cells, dst, rhs, mode, node);
     }
 
     template <int DIM, int N, typename T, bool TO_CELLS>
@@ -81,26 +81,39 @@ namespace stfem
 
   namespace
   {
+    // mode 0: dst = A src, 1: dst += A src, 2: dst = rhs - A src
     template <int DIM, int N, typename T>
-    void apply_dn(const StOpArgs<T> &a, const T *src, T *dst, T *cells, bool accumulate,
+    void apply_dn(const StOpArgs<T> &a, const T *src, T *dst, T *cells, const T *rhs, int mode,
                   hipStream_t s)
     {
+      const long m = a.g.n_nodes * a.nt_out;
       if (!cells)
         {
-          if (!accumulate)
-            launch_fill_zero<T>(dst, size_t(a.g.n_nodes) * a.nt_out, s);
+          if (mode == 2)
+            throw Error(STATUS_ERR_UNSUPPORTED, "st_apply: residual form needs the two-pass operator");
+          if (mode == 0)
+            launch_fill_zero<T>(dst, size_t(m), s);
           return dispatch_jb<DIM, N, T, false>(a, src, dst, s);
         }
       dispatch_jb<DIM, N, T, true>(a, src, cells, s);
-      k_gather_nodes<DIM, N, T><<<dim3((unsigned)((a.g.n_nodes + 255) / 256)), 256, 0, s>>>(
-        a.g, a.nt_out, cells, dst, accumulate ? 1 : 0);
+      const dim3 grid{(unsigned)((a.g.n_nodes + 255) / 256), 1, 1};
+      switch (a.nt_out)
+        {
+          case 1: k_gather_nodes<DIM, N, 1, T><<<grid, 256, 0, s>>>(a.g, 1, cells, dst, rhs, mode); break;
+          case 2: k_gather_nodes<DIM, N, 2, T><<<grid, 256, 0, s>>>(a.g, 2, cells, dst, rhs, mode); break;
+          case 3: k_gather_nodes<DIM, N, 3, T><<<grid, 256, 0, s>>>(a.g, 3, cells, dst, rhs, mode); break;
+          case 4: k_gather_nodes<DIM, N, 4, T><<<grid, 256, 0, s>>>(a.g, 4, cells, dst, rhs, mode); break;
+          default:
+            k_gather_nodes<DIM, N, 0, T><<<grid, 256, 0, s>>>(a.g, a.nt_out, cells, dst, rhs, mode);
+            break;
+        }
       STFEM_HIP(hipGetLastError());
     }
   } // namespace
 
   template <typename T>
-  void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, T *cells, bool accumulate,
-                       hipStream_t s)
+  void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, T *cells, int mode,
+                       const T *rhs, hipStream_t s)
   {
     STFEM_REQUIRE(a.nt_in >= 1 && a.nt_out >= 1, "st_apply: bad number of time components");
     if (a.g.n_cells == 0)
@@ -109,26 +122,26 @@ namespace stfem
     if (a.g.dim == 2)
       switch (N)
         {
-          case 2: return apply_dn<2, 2, T>(a, src, dst, cells, accumulate, s);
-          case 3: return apply_dn<2, 3, T>(a, src, dst, cells, accumulate, s);
-          case 4: return apply_dn<2, 4, T>(a, src, dst, cells, accumulate, s);
-          case 5: return apply_dn<2, 5, T>(a, src, dst, cells, accumulate, s);
+          case 2: return apply_dn<2, 2, T>(a, src, dst, cells, rhs, mode, s);
+          case 3: return apply_dn<2, 3, T>(a, src, dst, cells, rhs, mode, s);
+          case 4: return apply_dn<2, 4, T>(a, src, dst, cells, rhs, mode, s);
+          case 5: return apply_dn<2, 5, T>(a, src, dst, cells, rhs, mode, s);
           default: break;
         }
     else if (a.g.dim == 3)
       switch (N)
         {
-          case 2: return apply_dn<3, 2, T>(a, src, dst, cells, accumulate, s);
-          case 3: return apply_dn<3, 3, T>(a, src, dst, cells, accumulate, s);
-          case 4: return apply_dn<3, 4, T>(a, src, dst, cells, accumulate, s);
-          case 5: return apply_dn<3, 5, T>(a, src, dst, cells, accumulate, s);
+          case 2: return apply_dn<3, 2, T>(a, src, dst, cells, rhs, mode, s);
+          case 3: return apply_dn<3, 3, T>(a, src, dst, cells, rhs, mode, s);
+          case 4: return apply_dn<3, 4, T>(a, src, dst, cells, rhs, mode, s);
+          case 5: return apply_dn<3, 5, T>(a, src, dst, cells, rhs, mode, s);
           default: break;
         }
     throw Error(STATUS_ERR_UNSUPPORTED, "st_apply: unsupported dim/degree");
   }
 
   template void launch_st_apply<double>(const StOpArgs<double> &, const double *, double *,
-                                        double *, bool, hipStream_t);
+                                        double *, int, const double *, hipStream_t);
   template void launch_st_apply<float>(const StOpArgs<float> &, const float *, float *, float *,
-                                       bool, hipStream_t);
+                                       int, const float *, hipStream_t);
 } // namespace stfem

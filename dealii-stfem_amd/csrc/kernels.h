@@ -37,10 +37,11 @@ namespace stfem
   };
   // cells != null: two-pass form with the cell-major scratch `cells` of
   // st_cells_size() values; cells == null: one pass with atomic adds.
-  // accumulate: dst += result instead of dst = result.
+  // mode 0: dst = A src;  1: dst += A src;  2: dst = rhs - A src (two-pass
+  // form only).
   template <typename T>
-  void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, T *cells, bool accumulate,
-                       hipStream_t s);
+  void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, T *cells, int mode,
+                       const T *rhs, hipStream_t s);
   inline size_t st_cells_size(const GridDesc &g, int nt_out)
   {
     size_t nc = 1;
@@ -85,10 +86,16 @@ namespace stfem
   template <typename T>
   void launch_dot(long n, const T *x, const T *y, double *partial, double *out, int op,
                   hipStream_t s);
-  // x += coef[0] * factor * y ; then out = dot(x, z) (z may equal x; then
-  // out = sqrt(dot) when op == 1)
-  void launch_axpy_dot(long n, double *x, const double *coef, double factor, const double *y,
-                       const double *z, double *partial, double *out, int op, hipStream_t s);
+  // stage 1 only: partial[0..reduce_slots()) of dot(x, y)
+  template <typename T>
+  void launch_dot_partials(long n, const T *x, const T *y, double *partial, hipStream_t s);
+  // One modified Gram-Schmidt step: c = sum of the partials `pin` (stored to
+  // *hout when hout is not null), x += factor * c * y, then the partials of
+  // dot(x, z) (z null: dot(x, x)) into `pout` (pout != pin).
+  void launch_axpy_dot_chain(long n, double *x, const double *pin, double factor, const double *y,
+                             const double *z, double *pout, double *hout, hipStream_t s);
+  // stage 2 on the host, in the same order as the device stage 2
+  double sum_partials(const double *partial_host);
   template <typename T>
   void launch_axpby(long n, T a, const T *x, T b, T *y, hipStream_t s); // y = a x + b y
   void launch_multi_axpy(long n, int m, const double *const *z, const double *coef,

@@ -26,7 +26,7 @@ namespace stfem
   {
     STFEM_HIP(hipSetDevice(dev));
     STFEM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    partial.alloc(reduce_slots());
+    partial.alloc(2 * size_t(reduce_slots()));
     scalars.alloc(64);
     // shape tables for every supported degree, once per device
     static std::mutex           mtx;
@@ -418,8 +418,8 @@ namespace stfem
   }
 
   template <typename T>
-  void SystemMatrix<T>::apply(T *dst, const T *src, const DBuf<T> &a, const DBuf<T> &b,
-                              bool accumulate) const
+  void SystemMatrix<T>::apply(T *dst, const T *src, const DBuf<T> &a, const DBuf<T> &b, int mode,
+                              const T *rhs) const
   {
     StOpArgs<T> args;
     args.g          = grid_;
@@ -431,24 +431,32 @@ namespace stfem
     args.beta       = b.get();
     args.nt_in      = nt_in_;
     args.nt_out     = nt_out_;
-    launch_st_apply<T>(args, src, dst, cells_.get(), accumulate, ctx_.stream);
+    launch_st_apply<T>(args, src, dst, cells_.get(), mode, rhs, ctx_.stream);
+  }
+  template <typename T>
+  void SystemMatrix<T>::residual(T *dst, const T *rhs, const T *src) const
+  {
+    if (cells_.get())
+      return apply(dst, src, dA_, dB_, 2, rhs);
+    apply(dst, src, dA_, dB_, 0);
+    launch_axpby<T>(m(), T(1), rhs, T(-1), dst, ctx_.stream);
   }
   template <typename T>
   void SystemMatrix<T>::vmult(T *dst, const T *src) const
   {
-    apply(dst, src, dA_, dB_, false);
+    apply(dst, src, dA_, dB_, 0);
   }
   template <typename T>
   void SystemMatrix<T>::Tvmult(T *dst, const T *src) const
   {
     STFEM_REQUIRE(nt_in_ == nt_out_, "Tvmult needs square time matrices");
-    apply(dst, src, dAt_, dBt_, false);
+    apply(dst, src, dAt_, dBt_, 0);
   }
   template <typename T>
   void SystemMatrix<T>::vmult_add_nx1(T *dst, const T *src) const
   {
     STFEM_REQUIRE(nt_in_ == 1, "vmult_add_nx1 needs nt x 1 time matrices");
-    apply(dst, src, dA_, dB_, true);
+    apply(dst, src, dA_, dB_, 1);
   }
   template <typename T>
   double SystemMatrix<T>::vmult_bytes() const
@@ -1315,10 +1323,8 @@ namespace stfem
         return;
       }
     smoother_apply(l, sol_[l].get(), def_[l].get());
-    const long sz = lv_[l].A->m();
-    T         *t  = t1_[l].get();
-    lv_[l].A->vmult(t, sol_[l].get());
-    launch_axpby<T>(sz, T(1), def_[l].get(), T(-1), t, ctx_.stream);
+    T *t = t1_[l].get();
+    lv_[l].A->residual(t, def_[l].get(), sol_[l].get());
     launch_fill_zero<T>(def_[l - 1].get(), def_[l - 1].size(), ctx_.stream);
     tr_[l].restrict_and_add(def_[l - 1].get(), t);
     level_v_step(l - 1);
@@ -1402,14 +1408,23 @@ namespace stfem
       return 0;
     };
     std::vector<double> hcol(m + 1);
+    // two partial-sum buffers: each Gram-Schmidt step reads the previous
+    // step's partials and writes its own
+    const int           ns = reduce_slots();
+    std::vector<double> hpart(ns);
+    double *const       pA = prt, *const pB = prt + ns;
+    auto                host_sum = [&](const double *dpart) {
+      STFEM_HIP(hipMemcpyAsync(hpart.data(), dpart, ns * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx_.stream));
+      ctx_.sync();
+      return sum_partials(hpart.data());
+    };
     do
       {
         A.vmult(aux, x);
         launch_axpby<double>(n_, 1.0, b, -1.0, aux, ctx_.stream);
-        launch_dot<double>(n_, aux, aux, prt, dH, 1, ctx_.stream);
-        double beta;
-        STFEM_HIP(hipMemcpyAsync(&beta, dH, sizeof(double), hipMemcpyDeviceToHost, ctx_.stream));
-        ctx_.sync();
+        launch_dot_partials<double>(n_, aux, aux, pA, ctx_.stream);
+        const double beta = std::sqrt(host_sum(pA));
         state = check(acc, beta);
         if (state != 0)
           break;
@@ -1424,20 +1439,25 @@ namespace stfem
               v_[j].zero(ctx_.stream);
             P.vmult(z_[j].get(), v_[j].get());
             A.vmult(aux, z_[j].get());
-            // modified Gram-Schmidt with fused update + dot; coefficients
-            // stay on the device
-            launch_dot<double>(n_, aux, v_[0].get(), prt, dH, 0, ctx_.stream);
-            for (int i = 1; i <= j; ++i)
-              launch_axpy_dot(n_, aux, dH + i - 1, -1.0, v_[i - 1].get(), v_[i].get(), prt,
-                              dH + i, 0, ctx_.stream);
-            launch_axpy_dot(n_, aux, dH + j, -1.0, v_[j].get(), nullptr, prt, dH + j + 1, 1,
-                            ctx_.stream);
-            STFEM_HIP(hipMemcpyAsync(hcol.data(), dH, (j + 2) * sizeof(double),
+            // modified Gram-Schmidt: every step takes its coefficient from
+            // the previous step's partial sums (all on the device) and
+            // stores it to dH
+            launch_dot_partials<double>(n_, aux, v_[0].get(), pA, ctx_.stream);
+            double *pin = pA, *pout = pB;
+            for (int i = 1; i <= j + 1; ++i)
+              {
+                launch_axpy_dot_chain(n_, aux, pin, -1.0, v_[i - 1].get(),
+                                      i <= j ? v_[i].get() : nullptr, pout, dH + i - 1,
+                                      ctx_.stream);
+                std::swap(pin, pout);
+              }
+            STFEM_HIP(hipMemcpyAsync(hcol.data(), dH, (j + 1) * sizeof(double),
                                      hipMemcpyDeviceToHost, ctx_.stream));
-            ctx_.sync();
-            for (int i = 0; i <= j + 1; ++i)
+            const double hn = std::sqrt(host_sum(pin)); // also waits for hcol
+            for (int i = 0; i <= j; ++i)
               H[i][j] = hcol[i];
-            a = hcol[j + 1];
+            H[j + 1][j] = hn;
+            a           = hn;
             const int ncol = j; // projected solve once a second vector exists
             if (ncol > 0)
               {

@@ -154,6 +154,8 @@ namespace stfem
     // dst(all blocks) += (Alpha(:,0) x K + Beta(:,0) x M) src(one block)
     // for an nt x 1 weight pair (operators.h:154-187)
     void vmult_add_nx1(T *dst, const T *src) const;
+    // dst = rhs - (Alpha x K + Beta x M) src (the residual of the V-cycle)
+    void residual(T *dst, const T *rhs, const T *src) const;
     long        n_nodes() const { return grid_.n_nodes; }
     int         n_blocks() const { return nt_out_; }
     long        m() const { return grid_.n_nodes * nt_out_; }
@@ -164,7 +166,8 @@ namespace stfem
     double      vmult_bytes() const;
 
   private:
-    void apply(T *dst, const T *src, const DBuf<T> &a, const DBuf<T> &b, bool accumulate) const;
+    void apply(T *dst, const T *src, const DBuf<T> &a, const DBuf<T> &b, int mode,
+               const T *rhs = nullptr) const;
     Context     &ctx_;
     GridDesc     grid_;
     DBuf<double> verts_;
