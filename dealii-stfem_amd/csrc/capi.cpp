@@ -315,6 +315,26 @@ extern "C"
     });
     return g != STFEM_OK ? g : rc;
   }
+  int stfem_slab_error(stfem_problem p, const double *x, const double *prev, double time,
+                       double *out)
+  {
+    return guarded([&] {
+      auto &pb = P(p);
+      device_barrier(pb);
+      pb.slab_error(x, prev, time, out);
+    });
+  }
+  int stfem_run_heat(stfem_problem p, double *stats, int *its_per_slab, int cap)
+  {
+    int rc = STFEM_OK;
+    int g  = guarded([&] {
+      auto &pb = P(p);
+      device_barrier(pb);
+      rc = pb.run_heat(stats, its_per_slab, cap);
+      pb.context().sync();
+    });
+    return g != STFEM_OK ? g : rc;
+  }
   int stfem_solve_slab(stfem_problem p, double *x, const double *prev, double time, int *its)
   {
     int rc = STFEM_OK;

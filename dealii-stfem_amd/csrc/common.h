@@ -50,7 +50,7 @@ namespace stfem
   while (0)
 
   // Largest 1D node count (space degree + 1) with compiled kernels.
-  constexpr int MAX_N = 5;
+  constexpr int MAX_N = 6;
 
   // Structured level description passed to kernels.  Nodes of the
   // Gauss-Lobatto grid are numbered lexicographically (x fastest); a vector

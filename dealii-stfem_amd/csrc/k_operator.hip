@@ -126,6 +126,7 @@ namespace stfem
           case 3: return apply_dn<2, 3, T>(a, src, dst, cells, rhs, mode, s);
           case 4: return apply_dn<2, 4, T>(a, src, dst, cells, rhs, mode, s);
           case 5: return apply_dn<2, 5, T>(a, src, dst, cells, rhs, mode, s);
+          case 6: return apply_dn<2, 6, T>(a, src, dst, cells, rhs, mode, s);
           default: break;
         }
     else if (a.g.dim == 3)
@@ -135,6 +136,7 @@ namespace stfem
           case 3: return apply_dn<3, 3, T>(a, src, dst, cells, rhs, mode, s);
           case 4: return apply_dn<3, 4, T>(a, src, dst, cells, rhs, mode, s);
           case 5: return apply_dn<3, 5, T>(a, src, dst, cells, rhs, mode, s);
+          case 6: return apply_dn<3, 6, T>(a, src, dst, cells, rhs, mode, s);
           default: break;
         }
     throw Error(STATUS_ERR_UNSUPPORTED, "st_apply: unsupported dim/degree");

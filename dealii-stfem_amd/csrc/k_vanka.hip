@@ -126,6 +126,7 @@ namespace stfem
     for_nt(std::integral_constant<int, 27>{});
     for_nt(std::integral_constant<int, 64>{});
     for_nt(std::integral_constant<int, 125>{});
+    for_nt(std::integral_constant<int, 36>{});
     if (done)
       {
         STFEM_HIP(hipGetLastError());

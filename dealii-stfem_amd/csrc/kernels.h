@@ -113,6 +113,17 @@ namespace stfem
   template <typename T>
   void launch_zero_dirichlet(const GridDesc &g, int nt, T *x, hipStream_t s);
 
+  // ------------------------------------------------------- slab error
+  // Per-cell {L2^2, Linf, H1semi^2} errors of a slab against the
+  // manufactured heat solution: nq Gauss points (qx, qw on [0,1]) with the
+  // FE_Q(p) values S and derivatives D there (nq x (p+1), row-major); tpts
+  // holds npts rows [time, weight, coefficients of the nt blocks of x, of
+  // prev]; cell_err gets 3 values per cell.
+  void launch_slab_error(const GridDesc &g, const double *vertices, int nt, const double *x,
+                         const double *prev, int nq, const double *qx, const double *qw,
+                         const double *S, const double *D, int npts, const double *tpts,
+                         double freq, double *cell_err, hipStream_t s);
+
   // ------------------------------------------------- dense coarse operator
   // v = e_j (length n)
   template <typename T>

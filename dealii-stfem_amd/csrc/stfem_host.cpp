@@ -79,7 +79,7 @@ namespace stfem
                                            int dirichlet_mask)
   {
     STFEM_REQUIRE(dim == 2 || dim == 3, "mesh: dim must be 2 or 3");
-    STFEM_REQUIRE(p >= 1 && p < MAX_N, "mesh: space degree must be in [1, 4]");
+    STFEM_REQUIRE(p >= 1 && p < MAX_N, "mesh: space degree must be in [1, 5]");
     STFEM_REQUIRE(refine >= 0 && refine < 12, "mesh: refinement out of range");
     int fnc[3] = {1, 1, 1};
     for (int d = 0; d < dim; ++d)
@@ -1603,7 +1603,7 @@ namespace stfem
     if (ps.p < 0)
       ps.p = ps.r + 1;
     STFEM_REQUIRE(ps.dim == 2 || ps.dim == 3, "spec: dim must be 2 or 3");
-    STFEM_REQUIRE(ps.p >= 1 && ps.p < MAX_N, "spec: space degree must be in [1, 4]");
+    STFEM_REQUIRE(ps.p >= 1 && ps.p < MAX_N, "spec: space degree must be in [1, 5]");
     STFEM_REQUIRE(ps.r >= (ps.type == TimeStepType::DG ? 0 : 1), "spec: bad time degree");
     STFEM_REQUIRE(ps.nts >= 1 && (ps.nts & (ps.nts - 1)) == 0,
                   "spec: n_timesteps_at_once must be a power of two");
@@ -1839,6 +1839,103 @@ namespace stfem
           *its = solver_->last_step();
         return rc;
       }
+      void slab_error(const double *x, const double *prev, double time, double *out) override
+      {
+        STFEM_REQUIRE(spec.source == 0,
+                      "slab error: needs the manufactured solution (source=0)");
+        const GridDesc &g  = fine_->grid();
+        const int       nq = spec.r + 1, n = sh_.n;
+        const Quad1D    sq = quad_gauss(nq);
+        std::vector<double> S(size_t(nq) * n), D(size_t(nq) * n);
+        for (int q = 0; q < nq; ++q)
+          for (int a = 0; a < n; ++a)
+            {
+              S[q * n + a] = sh_.L.value(a, sq.x[q]);
+              D[q * n + a] = sh_.L.deriv(a, sq.x[q]);
+            }
+        // time points: the numerical solution is a combination of the
+        // slab's blocks and, for cG, of the previous slab's last block
+        const LagrangeBasis tb = get_time_basis(spec.type, spec.r);
+        const Quad1D        tq = quad_gauss(spec.r + 1);
+        const int           ni = n_dofs_per_interval(spec.type, spec.r);
+        std::vector<double> rows;
+        for (int it = 0; it < spec.nts; ++it)
+          for (size_t q = 0; q < tq.x.size(); ++q)
+            {
+              std::vector<double> row(size_t(nt) + 3, 0.0);
+              row[0] = time + tau * it + tq.x[q] * tau;
+              row[1] = tau * tq.w[q];
+              for (int i = 0; i < tb.size(); ++i)
+                {
+                  const double v = tb.value(i, tq.x[q]);
+                  int          k;
+                  if (spec.type == TimeStepType::DG)
+                    k = it * ni + i;
+                  else
+                    k = it * ni + i - 1; // -1: the previous slab's last block
+                  row[2 + (k < 0 ? nt : k)] += v;
+                }
+              rows.insert(rows.end(), row.begin(), row.end());
+            }
+        const int npts = int(rows.size() / (size_t(nt) + 3));
+        err_tp_.upload(rows.data(), rows.size(), ctx_.stream);
+        if (err_cells_.size() < size_t(g.n_cells) * 3)
+          err_cells_.alloc(size_t(g.n_cells) * 3);
+        launch_slab_error(g, fine_->vertices(), nt, x, prev, nq, sq.x.data(), sq.w.data(),
+                          S.data(), D.data(), npts, err_tp_.get(), spec.frequency,
+                          err_cells_.get(), ctx_.stream);
+        std::vector<double> e(size_t(g.n_cells) * 3);
+        STFEM_HIP(hipMemcpyAsync(e.data(), err_cells_.get(), e.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx_.stream));
+        ctx_.sync();
+        out[0] = 0;
+        out[1] = -1;
+        out[2] = 0;
+        for (long k = 0; k < g.n_cells; ++k)
+          {
+            out[0] += e[k * 3];
+            out[1] = std::max(out[1], e[k * 3 + 1]);
+            out[2] += e[k * 3 + 2];
+          }
+      }
+      int run_heat(double *stats, int *its_per_slab, int cap) override
+      {
+        const long   N = fine_->n_nodes();
+        DBuf<double> x{size_t(N) * nt}, prev{size_t(N)}, last{size_t(nt)};
+        launch_fill_zero<double>(x.get(), x.size(), ctx_.stream);
+        std::vector<double> sel(nt, 0.0);
+        sel[nt - 1] = 1.0; // picks the last block of a node-major vector
+        last.upload(sel.data(), sel.size(), ctx_.stream);
+        double time = 0.0, l2 = 0.0, l8 = -1.0, h1 = 0.0;
+        int    slab = 0, total = 0, rc = 0;
+        while (time < spec.end_time)
+          {
+            launch_time_transfer<double>(N, last.get(), 1, nt, x.get(), prev.get(), false,
+                                         ctx_.stream);
+            int       its = 0;
+            const int r   = solve_slab(x.get(), prev.get(), time, &its);
+            if (r != 0)
+              rc = r;
+            if (its_per_slab && slab < cap)
+              its_per_slab[slab] = its;
+            total += its;
+            double e[3];
+            slab_error(x.get(), prev.get(), time, e);
+            l2 += e[0];
+            l8 = std::max(l8, e[1]);
+            h1 += e[2];
+            time += spec.nts * tau;
+            ++slab;
+          }
+        stats[0] = slab;
+        stats[1] = total;
+        stats[2] = std::sqrt(l2);
+        stats[3] = l8;
+        stats[4] = std::sqrt(h1);
+        stats[5] = double(N);
+        stats[6] = nt;
+        return rc;
+      }
       void set_load_vector(const double *h) override
       {
         load_.upload(h, size_t(fine_->n_nodes()), ctx_.stream);
@@ -1854,6 +1951,7 @@ namespace stfem
       const std::vector<double> &history() const { return solver_->history(); }
 
     private:
+      DBuf<double> err_tp_, err_cells_; // slab error: time table, per-cell errors
       // time profile g(t) of the source f(t,x) = g(t) h(x): the manufactured
       // heat source of exact_solution.h:53-75 splits this way; the other
       // sources are constant in time

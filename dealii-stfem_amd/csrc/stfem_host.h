@@ -158,6 +158,7 @@ namespace stfem
     void residual(T *dst, const T *rhs, const T *src) const;
     long        n_nodes() const { return grid_.n_nodes; }
     int         n_blocks() const { return nt_out_; }
+    const double *vertices() const { return verts_.get(); }
     long        m() const { return grid_.n_nodes * nt_out_; }
     const GridDesc &grid() const { return grid_; }
     const DMat &alpha() const { return A_; }
@@ -440,6 +441,11 @@ namespace stfem
     virtual void gmg_vmult(double *dst, const double *src) = 0;
     virtual int  fgmres(double *x, const double *b, int *its, std::vector<double> *hist) = 0;
     virtual int  solve_slab(double *x, const double *prev, double time, int *its) = 0;
+    // ErrorCalculator::evaluate_error of one slab: out = {L2^2, Linf, H1semi^2}
+    virtual void slab_error(const double *x, const double *prev, double time, double *out) = 0;
+    // the heat time loop of tp_01.cc:630-683 from zero initial values:
+    // stats = {slabs, total iterations, L2, Linf, H1semi, n_nodes, nt}
+    virtual int  run_heat(double *stats, int *its_per_slab, int cap) = 0;
     virtual void set_load_vector(const double *h_host) = 0;
     virtual long n_nodes() const = 0;
     virtual Context &context() = 0;

@@ -54,6 +54,8 @@ _SIGS = {
     "stfem_gmg_vmult": ([_vp, _vp, _vp], ctypes.c_int),
     "stfem_fgmres_solve": ([_vp, _vp, _vp, _ip, _dp, ctypes.c_int], ctypes.c_int),
     "stfem_solve_slab": ([_vp, _vp, _vp, ctypes.c_double, _ip], ctypes.c_int),
+    "stfem_slab_error": ([_vp, _vp, _vp, ctypes.c_double, _dp], ctypes.c_int),
+    "stfem_run_heat": ([_vp, _dp, _ip, ctypes.c_int], ctypes.c_int),
     "stfem_set_load_vector": ([_vp, _dp], ctypes.c_int),
     "stfem_bytes_model": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
     "stfem_blocks_to_nodes": ([ctypes.c_long, ctypes.c_int, _vp, _vp], ctypes.c_int),
@@ -249,6 +251,24 @@ class Problem:
         its = ctypes.c_int()
         rc = load().stfem_solve_slab(self.h, _ptr(x), _ptr(prev), float(time), ctypes.byref(its))
         return rc, its.value
+
+    def slab_error(self, x, prev, time):
+        """{L2^2, Linf, H1semi^2} of one slab against the manufactured solution."""
+        out = np.zeros(3)
+        _check(load().stfem_slab_error(self.h, _ptr(x), _ptr(prev), float(time), _p(out)))
+        return out
+
+    def run_heat(self, max_slabs=4096):
+        """The heat time loop with errors (same record as the oracle's run_heat)."""
+        stats = np.zeros(7)
+        per = (ctypes.c_int * max_slabs)()
+        rc = load().stfem_run_heat(self.h, _p(stats), per, max_slabs)
+        if rc not in (0, -3, -5):
+            _check(rc)
+        n = int(stats[0])
+        return dict(rc=rc, slabs=n, total_its=int(stats[1]), L2=stats[2], Linf=stats[3],
+                    H1=stats[4], ndofs=int(stats[5]), nblocks=int(stats[6]),
+                    its=[per[i] for i in range(min(n, max_slabs))])
 
     def set_load_vector(self, h):
         h = np.ascontiguousarray(h, np.float64)

@@ -124,6 +124,19 @@ int stfem_fgmres_solve(stfem_problem p, double *x, const double *b, int *its, do
  * (rhs_matrix, operators.h:154-187) + source, extrapolated start, FGMRES,
  * Dirichlet nodes zeroed.  x: nt values per node; prev: one per node. */
 int stfem_solve_slab(stfem_problem p, double *x, const double *prev, double time, int *its);
+/* ErrorCalculator::evaluate_error (exact_solution.h:218-299) for one slab
+ * against the manufactured heat solution (source=0): Gauss(r+1) points in
+ * space and time; out[0..2] = {squared L2 error, L-infinity error, squared
+ * H1-seminorm error} of the slab starting at `time` (device x and prev as for
+ * stfem_solve_slab). */
+int stfem_slab_error(stfem_problem p, const double *x, const double *prev, double time,
+                     double *out);
+/* The heat driver of tp_01.cc:630-683 (TimeIntegratorHeat + ErrorCalculator)
+ * from zero initial values to end_time: stats[0..6] = {slabs, total FGMRES
+ * iterations, L2, Linf, H1-seminorm error of the whole run, fine nodes, time
+ * dofs per slab}; its_per_slab (optional, cap entries) gets the iterations
+ * of every slab.  Returns the first non-OK solver status. */
+int stfem_run_heat(stfem_problem p, double *stats, int *its_per_slab, int cap);
 /* host array h (one value per fine node): source f(t,x) = h(x) (int phi_i)
  * with unit time profile (source=2) */
 int stfem_set_load_vector(stfem_problem p, const double *h);

@@ -255,6 +255,17 @@ def _solve_slab_sig():
     return L
 
 
+def slab_error(problem, x, prev, time):
+    """{L2^2, Linf, H1semi^2} of one slab (x: nt blocks, prev: one block)."""
+    L = load()
+    L.oracle_slab_error.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, _dp]
+    x = np.ascontiguousarray(x, np.float64)
+    prev = np.ascontiguousarray(prev, np.float64)
+    out = np.zeros(3)
+    L.oracle_slab_error(problem.h, _p(x), _p(prev), float(time), _p(out))
+    return out
+
+
 def solve_slab(problem, x, prev, time):
     """One heat slab of the oracle; returns (rc, x_new, iterations)."""
     x = np.ascontiguousarray(x, np.float64).copy()

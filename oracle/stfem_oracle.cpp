@@ -2435,6 +2435,17 @@ extern "C"
     });
     return rc;
   }
+  // ErrorCalculator::evaluate_error of one slab (exact_solution.h:218-299):
+  // out = {L2^2, Linf, H1semi^2}; x: nt blocks, prev: one block.
+  int oracle_slab_error(void *h, const double *x, const double *prev, double time, double *out)
+  {
+    ORC_DISPATCH(h, {
+      const long          N = P.fine.N();
+      std::vector<double> xv(x, x + (long)P.nt * N), pv(prev, prev + N);
+      P.slab_error(time, xv, pv, out);
+    });
+    return 0;
+  }
   // Full heat run (tp_01.cc:630-683): stats = [slabs, total_its, L2, Linf,
   // H1semi, n_dofs, n_blocks]; its_per_slab optional (cap entries).
   int oracle_run_heat(void *h, double *stats, int *its_per_slab, int cap)

@@ -1,0 +1,84 @@
+"""The HIP path against the reference's own golden transcripts
+(tests/transfer_01.output and tests/tp_01.output of nlsmrg/dealii-stfem, as
+JSON fixtures) and the device error calculator against the oracle's.
+
+The whole heat driver runs on the GPU (stfem_run_heat: slab solves with
+FGMRES + space-time multigrid, extrapolated starts, ErrorCalculator), and its
+DoF counts, level counts, slab counts, space-time error norms and average
+iterations per slab are checked with the tolerances the oracle's own golden
+tests use.
+"""
+import numpy as np
+import pytest
+
+from helpers import blocks_to_nodes
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import oracle as orc  # noqa: E402
+import stfem_amd as st  # noqa: E402
+from test_oracle_goldens import load, tp01_heat_runs, transfer01_runs  # noqa: E402
+
+DEV = "cuda:0"
+
+
+@pytest.mark.parametrize("spec", [dict(dim=2, refine=2, type=2, r=1, nts=2),
+                                  dict(dim=2, refine=2, type=1, r=2, nts=2),
+                                  dict(dim=3, refine=1, type=2, r=2, nts=1, p=3)])
+def test_slab_error_matches_oracle(spec):
+    o, g = orc.Problem(spec), st.Problem(spec)
+    n, nt = g.n_nodes, g.nt
+    rng = np.random.default_rng(29)
+    x = rng.standard_normal(n * nt)
+    prev = rng.standard_normal(n)
+    ref = orc.slab_error(o, x, prev, 0.3)
+    xg = torch.tensor(blocks_to_nodes(x, nt), dtype=torch.float64, device=DEV)
+    pg = torch.tensor(prev, dtype=torch.float64, device=DEV)
+    print(f"[slab error {spec}]", flush=True)
+    got = g.slab_error(xg, pg, 0.3)
+    assert np.allclose(got, ref, rtol=1e-12, atol=0), (got, ref)
+    g.close()
+
+
+def test_transfer01_transcript_on_gpu():
+    gold = load("transfer_01")
+    table_rows = [row for t in gold["tables"] for row in t]
+    exact = 0
+    for k, (spec, g, row) in enumerate(zip(transfer01_runs(), gold["runs"], table_rows)):
+        print(f"[transfer_01 run {k}] {spec}", flush=True)
+        P = st.Problem(spec)
+        res = P.run_heat()
+        assert res["rc"] == 0, (k, res)
+        assert res["ndofs"] == g["dofs"] == row["s_dofs"], k
+        assert P.n_levels - 1 == g["max_level"], k
+        assert res["slabs"] == g["slabs"], k
+        assert res["nblocks"] == row["t_dofs"], k
+        for key in ("L2", "Linf", "H1"):
+            assert abs(res[key] - row[key]) <= 6e-6 * abs(row[key]), (k, key, res[key], row[key])
+        avg = res["total_its"] / res["slabs"]
+        assert abs(avg - g["avg_its"]) <= 0.25, (k, avg, g["avg_its"])
+        exact += abs(avg - g["avg_its"]) < 1e-12
+        P.close()
+    assert exact >= 30, exact
+
+
+def test_tp01_transcript_on_gpu():
+    exact, n = 0, 0
+    for spec, g, row in tp01_heat_runs(3):
+        print(f"[tp_01] {spec}", flush=True)
+        P = st.Problem(spec)
+        res = P.run_heat()
+        assert res["rc"] == 0, (spec, res)
+        assert res["ndofs"] == g["dofs"] == row["s_dofs"]
+        assert P.n_levels - 1 == g["max_level"]
+        assert res["slabs"] == g["slabs"]
+        assert res["nblocks"] == row["t_dofs"]
+        for key in ("L2", "Linf", "H1"):
+            assert abs(res[key] - row[key]) <= 2e-5 * abs(row[key]), (spec, key, res[key], row[key])
+        avg = res["total_its"] / res["slabs"]
+        assert abs(avg - g["avg_its"]) <= 0.25, (spec, avg, g["avg_its"])
+        exact += abs(avg - g["avg_its"]) < 1e-12
+        n += 1
+        P.close()
+    assert exact >= int(0.7 * n), (exact, n)
