@@ -55,6 +55,23 @@ def workload(refine):
     return spec, X, kappa
 
 
+def workload_large():
+    """Spec, vertices and coefficient of the large 3D heat workload: 64^3
+    cells perturbed by up to 0.1h, Q4 space, dG(3) time (4 time dofs per
+    slab, 67.9M space-time dofs per slab), T = 1 in 4 slabs, cellwise
+    log-uniform coefficient in [1e-3, 1e3], seeded Gaussian bumps."""
+    import numpy as np
+    dim, refine = 3, 6
+    nc = [1 << refine] * dim
+    X = box_vertices(dim, nc, 0.1, seed=2026)
+    rng = np.random.default_rng(2027)
+    kappa = np.exp(rng.uniform(np.log(1e-3), np.log(1e3), int(np.prod(nc))))
+    spec = dict(dim=3, refine=refine, p=4, type=2, r=3, nts=1, tau=0.25, end_time=1.0,
+                source=3, n_bumps=4, seed=7, reltol=1e-8, abstol=1e-300, maxiter=200,
+                basis=30, vanka_factored=1)
+    return spec, X, kappa
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -67,10 +84,11 @@ def run_gpu(spec, X, kappa, steps, warmup):
     log(f"setup mg_float={spec.get('mg_float')} {setup_s:.1f} s, {g.n_levels} levels, "
         f"{g.n_nodes * g.nt} space-time dofs")
     if warmup > 0:
-        g.time_slabs(warmup)
-    ms, its = g.time_slabs(steps)
-    log(f"timed {steps} slabs: {ms:.1f} ms, {its} FGMRES iterations")
-    return g, setup_s, ms, its
+        g.time_slabs(warmup, allow_unconverged=True)
+    ms, its, ok = g.time_slabs(steps, allow_unconverged=True)
+    log(f"timed {steps} slabs: {ms:.1f} ms, {its} FGMRES iterations"
+        f"{'' if ok else ' (iteration limit reached)'}")
+    return g, setup_s, ms, its, ok
 
 
 def cpu_baseline(budget_s=25.0):
@@ -122,6 +140,7 @@ def pmc_traffic(kernel, name, refine, timeout_s=150):
     is scaled by what it reads for a device copy of COPY_BYTES in the same
     process; the finest-level launches (largest grid) are the ones reported."""
     import csv
+    import re
     import shutil
     import statistics
     import subprocess
@@ -148,7 +167,7 @@ def pmc_traffic(kernel, name, refine, timeout_s=150):
         rows = [x for x in csv.DictReader(open(path)) if x["Counter_Name"] == counter]
         shutil.rmtree(d, ignore_errors=True)
         copy = [float(x["Counter_Value"]) for x in rows if "copyBuffer" in x["Kernel_Name"]]
-        mine = [x for x in rows if name in x["Kernel_Name"]]
+        mine = [x for x in rows if re.search(name, x["Kernel_Name"])]
         if not copy or not mine:
             return None, f"{counter}: calibration copy or kernel missing from the counters"
         gmax = max(int(x["Grid_Size"]) for x in mine)
@@ -188,6 +207,8 @@ def main():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--refine", type=int, default=5)
+    ap.add_argument("--workload", choices=["default", "large"], default="default",
+                    help="large: 64^3 cells, Q4 space, dG(3) time (67.9M dofs per slab)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-pmc", action="store_true",
@@ -205,7 +226,7 @@ def main():
 
     import torch
     torch.cuda.synchronize()
-    spec, X, kappa = workload(a.refine)
+    spec, X, kappa = workload(a.refine) if a.workload == "default" else workload_large()
     if a.dense_coarse_dofs is not None:
         spec["dense_coarse_dofs"] = a.dense_coarse_dofs
     for kv in a.set:
@@ -216,7 +237,7 @@ def main():
 
     # FP32 multigrid inside FP64 FGMRES (headline)
     s32 = dict(spec, mg_float=1)
-    g32, setup32, ms32, its32 = run_gpu(s32, X, kappa, a.steps, a.warmup)
+    g32, setup32, ms32, its32, ok32 = run_gpu(s32, X, kappa, a.steps, a.warmup)
     nst = g32.n_nodes * g32.nt
     dofs32 = a.steps * nst / (ms32 * 1e-3)
 
@@ -231,10 +252,10 @@ def main():
     vk_bytes = g32.bytes_model(1, L)
     g32.close()
 
-    dofs64, its64, ms64, setup64 = None, None, None, None
+    dofs64, its64, ms64, setup64, ok64 = None, None, None, None, None
     if not a.no_fp64:
         s64 = dict(spec, mg_float=0)
-        g64, setup64, ms64, its64 = run_gpu(s64, X, kappa, a.steps, a.warmup)
+        g64, setup64, ms64, its64, ok64 = run_gpu(s64, X, kappa, a.steps, a.warmup)
         dofs64 = a.steps * nst / (ms64 * 1e-3)
         g64.close()
 
@@ -242,10 +263,10 @@ def main():
     # sweeps (pre/post), three level vmults and one FP64 vmult
     if 2 * vk_ms >= 3 * vm32_ms + vm64_ms:
         dom = dict(kernel="vanka_factored_fp32", ms=vk_ms, bytes=vk_bytes, only="vanka",
-                   symbol="k_vanka_factored")
+                   symbol=r"k_vanka_factored")
     else:
         dom = dict(kernel="st_vmult_fp64", ms=vm64_ms, bytes=vm64_bytes, only="vmult64",
-                   symbol="k_st_apply<3, 4, 3, double>")
+                   symbol=r"k_st_apply<3, \d, \d, double")
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
 
     traffic, traffic_note = None, "counter passes skipped (--no-pmc)"
@@ -267,15 +288,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 (FGMRES) + f32 (multigrid)",
         "data": "synthetic",
-        "config": {"workload": f"3D heat, {1 << a.refine}^3 cells perturbed 0.2h, Q3 space, "
-                               f"dG(2) time, log-uniform kappa [1e-2,1e2], Gaussian bumps, "
-                               f"tol 1e-8",
+        "config": {"workload": (f"3D heat, {1 << a.refine}^3 cells perturbed 0.2h, Q3 space, "
+                                f"dG(2) time, log-uniform kappa [1e-2,1e2], Gaussian bumps, "
+                                f"tol 1e-8") if a.workload == "default" else
+                               ("3D heat, 64^3 cells perturbed 0.1h, Q4 space, dG(3) time, "
+                                "log-uniform kappa [1e-3,1e3], Gaussian bumps, tol 1e-8"),
                    "space_time_dofs_per_slab": nst, "n_levels": g32.n_levels,
                    "vanka": "factored"},
         "fp64_dofs_per_s": dofs64,
         "speedup_fp32mg_vs_fp64": (dofs32 / dofs64) if dofs64 else None,
         "fgmres_its_fp32mg": its32,
         "fgmres_its_fp64": its64,
+        "converged": {"fp32mg": ok32, "fp64": ok64},
+        "ms_per_fgmres_iteration": {"fp32mg": ms32 / its32 if its32 else None,
+                                    "fp64": (ms64 / its64) if its64 else None},
         "setup_s": {"fp32mg": setup32, "fp64": setup64},
         "vmult_fp64": {"ms": vm64_ms, "GBps": vm64_bytes / (vm64_ms * 1e-3) / 1e9,
                        "hbm_frac": vm64_bytes / (vm64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,

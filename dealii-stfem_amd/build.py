@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libstfem_amd.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["k_operator.hip", "k_error.hip", "k_vanka.hip", "k_transfer.hip", "k_blas.hip",
+SOURCES = ["k_operator.hip", "k_error.hip", "k_setup.hip", "k_vanka.hip", "k_transfer.hip", "k_blas.hip",
            "time_fe.cpp", "stfem_host.cpp", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -50,7 +50,7 @@ def build(jobs=None, verbose=False):
     objs = [o for o, _ in results]
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs + \
-        ["-L/opt/rocm/lib", "-lrocblas",
+        ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas",
          "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/llvm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -485,11 +485,12 @@ extern "C"
           total += t;
           itsum += it;
           time += pb.spec.nts * pb.tau;
-          if (r != STFEM_OK)
-            {
-              rc = r;
-              break;
-            }
+          // a slab that hits the iteration limit still counts (its time and
+          // iterations); the status is reported at the end
+          if (r != STFEM_OK && rc == STFEM_OK)
+            rc = r;
+          if (r != STFEM_OK && r != STFEM_ERR_NO_CONVERGENCE)
+            break;
         }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);

@@ -254,6 +254,7 @@ namespace stfem
   }
   template void launch_fill_zero<double>(double *, size_t, hipStream_t);
   template void launch_fill_zero<float>(float *, size_t, hipStream_t);
+  template void launch_fill_zero<int>(int *, size_t, hipStream_t);
 
   int reduce_slots() { return RB; }
 

@@ -113,6 +113,29 @@ namespace stfem
   template <typename T>
   void launch_zero_dirichlet(const GridDesc &g, int nt, T *x, hipStream_t s);
 
+  // ------------------------------------------------------- Vanka setup
+  // Element mass and stiffness matrices (row-major nl x nl, FP64) of cells
+  // [c0, c0 + nc): Gauss(p+1) rule (qx, qw on [0,1]) with phi_a(x_q) = S and
+  // phi_a'(x_q) = Sd ([q][a]), Q1 mapping from the vertices X, coefficient
+  // as in StOpArgs (FP64).
+  void launch_element_matrices(const GridDesc &g, const double *X, int coef_mode,
+                               double coef_value, const double *coef, const double *S,
+                               const double *Sd, const double *qx, const double *qw, long c0,
+                               long nc, double *Me, double *Ke, hipStream_t s);
+  // Patch matrices K[I_c,I_c], M[I_c,I_c] of the assembled matrices for
+  // cells [p0, p0 + np) (column-major nl x nl; constrained rows/columns
+  // reduced to the diagonal) from the element matrices of cells
+  // [e0, e0 + ne), which must contain every neighbour of the patches.
+  void launch_patch_matrices(const GridDesc &g, long p0, long np, long e0, const double *Me,
+                             const double *Ke, double *Mp, double *Kp, hipStream_t s);
+  // S_c (row-major, eigenvector k = column k) from the column-major
+  // eigenvectors Z, and W_{c,k} = (lam_k Alpha + Beta)^{-1} (nt <= 8); sets
+  // *singular when a time block cannot be inverted.
+  template <typename T>
+  void launch_pack_factors(long ncell, int nl, int nt, const double *Z, const double *lam,
+                           const double *Alpha, const double *Beta, T *S, T *W, int *singular,
+                           hipStream_t s);
+
   // ------------------------------------------------------- slab error
   // Per-cell {L2^2, Linf, H1semi^2} errors of a slab against the
   // manufactured heat solution: nq Gauss points (qx, qw on [0,1]) with the

@@ -5,6 +5,7 @@
 #include "stfem_host.h"
 
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -13,6 +14,9 @@
 #include <mutex>
 #include <set>
 #include <array>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <random>
 #include <sstream>
 
@@ -20,6 +24,28 @@
 
 namespace stfem
 {
+  namespace
+  {
+    // setup phase timings on stderr when STFEM_VERBOSE is set
+    struct PhaseLog
+    {
+      bool                                  on = std::getenv("STFEM_VERBOSE") != nullptr;
+      std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+      void operator()(const char *what, long a = -1)
+      {
+        if (!on)
+          return;
+        const auto   t1 = std::chrono::steady_clock::now();
+        const double s  = std::chrono::duration<double>(t1 - t0).count();
+        if (a >= 0)
+          std::fprintf(stderr, "[stfem] %s %ld: %.3f s\n", what, a, s);
+        else
+          std::fprintf(stderr, "[stfem] %s: %.3f s\n", what, s);
+        t0 = t1;
+      }
+    };
+  } // namespace
+
   // =============================================================== Context
   Context::Context(int dev)
     : device(dev)
@@ -656,12 +682,18 @@ namespace stfem
 
   template <typename T>
   PreconditionVanka<T>::PreconditionVanka(Context &ctx, const MeshLevel &m, const Shape1D &sh,
-                                          const DMat &Alpha, const DMat &Beta, Form form)
+                                          const DMat &Alpha, const DMat &Beta, Form form,
+                                          bool device_setup)
     : ctx_(ctx)
     , grid_(m.g)
     , nt_(Alpha.m)
     , form_(form)
   {
+    if (form_ == Form::factored && device_setup && nt_ <= 8)
+      {
+        setup_factored_on_device(m, sh, Alpha, Beta);
+        return;
+      }
     std::vector<double> Kp, Mp;
     cell_patch_matrices(m, sh, Kp, Mp);
     const int    nl = (m.g.dim == 3) ? sh.n * sh.n * sh.n : sh.n * sh.n;
@@ -822,6 +854,83 @@ namespace stfem
                                 sW.size() * sizeof(T), hipMemcpyHostToDevice));
           }
       }
+  }
+
+  // Factored Vanka on the device, in chunks of whole layers of cells (rows in
+  // 2D) so that element matrices of a chunk and its neighbouring layers,
+  // the patch matrices and the eigensolver workspace stay bounded:
+  // element matrices -> patch submatrices K_c, M_c -> K_c Z = M_c Z Lambda
+  // (rocSOLVER sygvj, Z^T M_c Z = I) -> S = Z, W_k = (lambda_k Alpha +
+  // Beta)^{-1} in the level precision.
+  template <typename T>
+  void PreconditionVanka<T>::setup_factored_on_device(const MeshLevel &m, const Shape1D &sh,
+                                                      const DMat &Alpha, const DMat &Beta)
+  {
+    const GridDesc &g  = m.g;
+    const int       nl = (g.dim == 3) ? sh.n * sh.n * sh.n : sh.n * sh.n;
+    const int       nt = nt_;
+    const long      nc = g.n_cells;
+    const size_t    bs = size_t(nl) * nl;
+    hipStream_t     s  = ctx_.stream;
+    S_.alloc(size_t(nc) * bs);
+    W_.alloc(size_t(nc) * nl * nt * nt);
+    DBuf<double> X, coef, dA, dB;
+    X.upload(m.X.data(), m.X.size(), s);
+    if (m.coef_mode != 0)
+      coef.upload(m.coef.data(), m.coef.size(), s);
+    dA.upload(Alpha.a.data(), Alpha.a.size(), s);
+    dB.upload(Beta.a.data(), Beta.a.size(), s);
+    DBuf<int> flag(1);
+    launch_fill_zero<int>(flag.get(), 1, s);
+    const long layer = (g.dim == 3) ? long(g.nc[0]) * g.nc[1] : long(g.nc[0]);
+    const int  nlay  = (g.dim == 3) ? g.nc[2] : g.nc[1];
+    // about 8 GB of FP64 work arrays per chunk, at most 16384 patches
+    const long per_cell = long(bs) * 8 * 5;
+    const long target   = std::min<long>(16384, std::max<long>(1, (8L << 30) / per_cell));
+    const int  lper     = int(std::max<long>(1, std::min<long>(nlay, target / layer)));
+    const long emax     = std::min<long>(nlay, lper + 2) * layer, pmax = long(lper) * layer;
+    DBuf<double> Me{size_t(emax) * bs}, Ke{size_t(emax) * bs}, Mp{size_t(pmax) * bs},
+      Kp{size_t(pmax) * bs}, lam{size_t(pmax) * nl}, res{size_t(pmax)};
+    DBuf<int>     sweeps{size_t(pmax)}, info{size_t(pmax)};
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+      throw Error(STATUS_ERR_INTERNAL, "rocBLAS: handle creation failed");
+    std::unique_ptr<std::remove_pointer_t<rocblas_handle>, rocblas_status (*)(rocblas_handle)>
+      guard(h, rocblas_destroy_handle);
+    rocblas_set_stream(h, s);
+    std::vector<int> hinfo(static_cast<size_t>(pmax));
+    for (int z0 = 0; z0 < nlay; z0 += lper)
+      {
+        const int  z1 = std::min(nlay, z0 + lper);
+        const int  e_lo = std::max(0, z0 - 1), e_hi = std::min(nlay, z1 + 1);
+        const long e0 = long(e_lo) * layer, ne = long(e_hi - e_lo) * layer;
+        const long p0 = long(z0) * layer, np = long(z1 - z0) * layer;
+        launch_element_matrices(g, X.get(), m.coef_mode, m.coef_value, coef.get(), sh.S.data(),
+                                sh.Sd.data(), sh.qx.data(), sh.qw.data(), e0, ne, Me.get(),
+                                Ke.get(), s);
+        launch_patch_matrices(g, p0, np, e0, Me.get(), Ke.get(), Mp.get(), Kp.get(), s);
+        const rocblas_status st = rocsolver_dsygvj_strided_batched(
+          h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, nl, Kp.get(), nl,
+          rocblas_stride(bs), Mp.get(), nl, rocblas_stride(bs), 0.0, res.get(), 100,
+          sweeps.get(), lam.get(), nl, info.get(), rocblas_int(np));
+        if (st != rocblas_status_success)
+          throw Error(STATUS_ERR_INTERNAL,
+                      std::string("rocSOLVER sygvj: ") + rocblas_status_to_string(st));
+        launch_pack_factors<T>(np, nl, nt, Kp.get(), lam.get(), dA.get(), dB.get(),
+                               S_.get() + size_t(p0) * bs, W_.get() + size_t(p0) * nl * nt * nt,
+                               flag.get(), s);
+        STFEM_HIP(hipMemcpyAsync(hinfo.data(), info.get(), size_t(np) * sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+        ctx_.sync();
+        for (long k = 0; k < np; ++k)
+          if (hinfo[k] != 0)
+            throw Error(STATUS_ERR_INTERNAL, "Vanka factored: patch not positive definite");
+      }
+    int hf = 0;
+    STFEM_HIP(hipMemcpyAsync(&hf, flag.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    ctx_.sync();
+    if (hf)
+      throw Error(STATUS_ERR_INTERNAL, "Vanka factored: singular time block");
   }
 
   template <typename T>
@@ -1570,6 +1679,7 @@ namespace stfem
     geti("mg_float", ps.mg_float);
     geti("dirichlet", ps.dirichlet);
     geti("atomic_scatter", ps.atomic_scatter);
+    geti("vanka_device_setup", ps.vanka_device_setup);
     geti("smoothing_steps", ps.mg.smoothing_steps);
     getb("variable", ps.mg.variable);
     getb("estimate_relaxation", ps.mg.estimate_relaxation);
@@ -1622,8 +1732,10 @@ namespace stfem
         , sh_(s.p)
       {
         spec = s;
+        PhaseLog plog;
         mesh_ = build_mesh_levels(spec.dim, spec.subdiv, spec.refine, spec.p, vertices, spec.lo,
                                   spec.hi, kappa, spec.dirichlet);
+        plog("mesh levels");
         // time step (tp_01.cc:87, 124-127) unless given
         if (spec.tau > 0)
           tau = spec.tau;
@@ -1670,10 +1782,13 @@ namespace stfem
             const MeshLevel  &m = mesh_[sref[l]];
             levels[l].A  = std::make_unique<SystemMatrix<T>>(ctx_, m, w.Alpha, w.Beta,
                                                              spec.atomic_scatter == 0);
-            levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form);
+            levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form,
+                                                                  spec.vanka_device_setup != 0);
             levels[l].r  = rl[l];
             levels[l].nts = nl[l];
             levels[l].type_to_here = l > 0 ? seq[l - 1] : TimeMGType::none;
+            ctx_.sync();
+            plog("level operator and Vanka", l);
           }
         std::vector<TwoLevelTransfer<T>> tr(nlev);
         for (int l = 1; l < nlev; ++l)
@@ -1687,7 +1802,9 @@ namespace stfem
                 ctx_, spec.type, rl[l], nl[l], t, mesh_[sref[l]].g.n_nodes,
                 spec.mg.overwrite_time_transfer);
           }
+        plog("transfers");
         gmg_ = std::make_unique<GMG<T>>(ctx_, std::move(levels), std::move(tr), spec.mg);
+        plog("relaxation estimates and dense coarse operator");
         // FP64 fine system and rhs matrices (tp_01.cc:152-198, heat)
         const MeshLevel  &F = mesh_.back();
         const TimeWeights w = get_fe_time_weights(spec.type, spec.r, tau, spec.nts);

@@ -191,8 +191,11 @@ namespace stfem
       explicit_inverse,
       factored
     };
+    // device_setup: the factored form is set up on the GPU (element and
+    // patch matrices, batched generalized eigensolver); otherwise, and for
+    // the explicit inverse, on the host
     PreconditionVanka(Context &ctx, const MeshLevel &m, const Shape1D &sh, const DMat &Alpha,
-                      const DMat &Beta, Form form);
+                      const DMat &Beta, Form form, bool device_setup = true);
     // dst = sum_c (1/valence) B_c^{-1} src|_c   (gmg.h:476-514)
     void   vmult(T *dst, const T *src) const;
     // u += omega * Vanka(rhs - ax); ax may be null.  One relaxation sweep of
@@ -204,6 +207,8 @@ namespace stfem
     size_t storage_bytes() const { return (binv_.size() + S_.size() + W_.size()) * sizeof(T); }
 
   private:
+    void     setup_factored_on_device(const MeshLevel &m, const Shape1D &sh, const DMat &Alpha,
+                                      const DMat &Beta);
     Context &ctx_;
     GridDesc grid_;
     int      nt_;
@@ -411,6 +416,7 @@ namespace stfem
     int          mg_float = 0;
     int          dirichlet = 63;
     int          atomic_scatter = 0; // 1: one-pass operator with atomic adds
+    int          vanka_device_setup = 1; // factored Vanka set up on the GPU
     GMGOptions   mg;
     SolverControlData solver;
     int          extrapolate = 1;

@@ -284,9 +284,14 @@ class Problem:
         _check(load().stfem_time_kernel(self.h, what, level, warmup, reps, ctypes.byref(ms)))
         return ms.value
 
-    def time_slabs(self, n_slabs):
+    def time_slabs(self, n_slabs, allow_unconverged=False):
+        """(ms, iterations) of n_slabs heat slabs; with allow_unconverged,
+        (ms, iterations, converged) and slabs stopped by the iteration limit
+        are accepted."""
         ms = ctypes.c_double()
         its = ctypes.c_int()
         rc = load().stfem_time_slabs(self.h, n_slabs, ctypes.byref(ms), ctypes.byref(its))
+        if allow_unconverged and rc == -3:
+            return ms.value, its.value, False
         _check(rc)
-        return ms.value, its.value
+        return (ms.value, its.value, True) if allow_unconverged else (ms.value, its.value)

@@ -152,7 +152,9 @@ int stfem_nodes_to_blocks(long n_nodes, int nt, const double *nodes, double *blo
  * what 0 fine FP64 vmult, 1 level vmult, 2 level Vanka, 3 GMG V-cycle */
 int stfem_time_kernel(stfem_problem p, int what, int level, int warmup, int reps, double *ms);
 /* runs `n_slabs` heat slabs from a zero initial value; returns the summed
- * slab-solve time (ms, HIP events) and the summed FGMRES iterations */
+ * slab-solve time (ms, HIP events) and the summed FGMRES iterations.  Slabs
+ * that reach the iteration limit are timed and counted, and the run goes on
+ * (status STFEM_ERR_NO_CONVERGENCE at the end); any other failure stops it. */
 int stfem_time_slabs(stfem_problem p, int n_slabs, double *ms, int *its);
 
 #ifdef __cplusplus

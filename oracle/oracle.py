@@ -255,6 +255,21 @@ def _solve_slab_sig():
     return L
 
 
+def solve_slab_wave(problem, x, prev, prev_v, time):
+    """One wave slab of the oracle; returns (rc, u_new, v_new, iterations)."""
+    L = load()
+    L.oracle_solve_slab_wave.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                         _ip]
+    x = np.ascontiguousarray(x, np.float64).copy()
+    v = np.zeros_like(x)
+    prev = np.ascontiguousarray(prev, np.float64)
+    prev_v = np.ascontiguousarray(prev_v, np.float64)
+    its = ctypes.c_int()
+    rc = L.oracle_solve_slab_wave(problem.h, _p(x), _p(v), _p(prev), _p(prev_v), float(time),
+                                  ctypes.byref(its))
+    return rc, x, v, its.value
+
+
 def slab_error(problem, x, prev, time):
     """{L2^2, Linf, H1semi^2} of one slab (x: nt blocks, prev: one block)."""
     L = load()

@@ -1388,7 +1388,8 @@ namespace orc
     int                 extrapolate = 1;
     double              end_time = 1.0;
     double              frequency = 1.0;
-    int                 rhs = 0; // 0: manufactured heat (exact_solution.h:53-75), 1: zero
+    int                 rhs = 0; // 0: manufactured solution's source, 1: zero
+    int                 problem = 0; // 0: heat, 1: wave (tp_01.cc:167-197, time_integrators.h:183-280)
     int                 fgmres_variant = 0; // 0: deal.II 9.5 projected solve at j>0
     int                 dealii_order = 0;   // power-iteration start vector in deal.II dof order
     double              omega_scale = 1.0;  // diagnostic multiplier on estimated omegas
@@ -1454,6 +1455,8 @@ namespace orc
     getd("end_time", sp.end_time);
     getd("frequency", sp.frequency);
     geti("rhs", sp.rhs);
+    if (kv.count("problem"))
+      sp.problem = (kv["problem"] == "wave" || kv["problem"] == "1") ? 1 : 0;
     geti("fgmres_variant", sp.fgmres_variant);
     geti("dealii_order", sp.dealii_order);
     getd("omega_scale", sp.omega_scale);
@@ -1486,6 +1489,9 @@ namespace orc
     Level<double>              fine;     // FP64 system matrix (fine level)
     Mat                        rhsK, rhsM; // rhs_matrix weights
     Mat                        Alpha1, Gamma1;
+    // wave: rhs_matrix_v weights (mass only) and the velocity recovery
+    // matrices of TimeIntegratorWave (time_integrators.h:219-233)
+    Mat                        rhsV, AixB, AixG, AixZ;
     int                        nt = 1;
     double                     tau = 0.0;
     std::vector<double>        omega_log;
@@ -1524,6 +1530,16 @@ namespace orc
       L.Kv.assign(L.K.v.begin(), L.K.v.end());
       L.Mv.assign(L.M.v.begin(), L.M.v.end());
       auto w    = fe_time_weights(type, r, tau_l, nts);
+      if (spec.problem == 1)
+        {
+          // level weights of the wave problem: the Schur-in-time transform
+          // of the level's (multi-step) weights taken as one interval
+          // (fe_time.h:383-412, which calls the transform with its default
+          // n_timesteps_at_once = 1)
+          auto ww = fe_time_weights_wave(type, w[0], w[1], w[2], w[3], 1);
+          w[0]    = ww[0];
+          w[1]    = ww[1];
+        }
       L.Alpha   = w[0];
       L.Beta    = w[1];
       L.nt      = w[0].m;
@@ -1642,18 +1658,41 @@ namespace orc
         F.Kv = F.K.v;
         F.Mv = F.M.v;
         auto w   = fe_time_weights(spec.type, spec.r, tau, spec.nts);
+        auto w1  = fe_time_weights(spec.type, spec.r, tau, 1);
+        Mat  zero(w[2].m, 1);
+        if (spec.problem == 1)
+          {
+            // tp_01.cc:167-190: lhs = (Alpha_lhs, Beta_lhs), rhs_matrix =
+            // (rhs_uK, rhs_uM) on prev_u, rhs_matrix_v = (0, rhs_vM) on prev_v
+            auto ww = fe_time_weights_wave(spec.type, w1[0], w1[1], w1[2], w1[3], spec.nts);
+            w[0]    = ww[0];
+            w[1]    = ww[1];
+            rhsK    = ww[2];
+            rhsM    = ww[3];
+            rhsV    = ww[4];
+            // TimeIntegratorWave: AixB = Alpha^-1 Beta, AixG = Alpha^-1 Gamma
+            // (negated for dG), AixZ = Alpha^-1 Zeta (negated for cG)
+            const Mat Ai = inverse(w1[0]);
+            AixB         = matmul(Ai, w1[1]);
+            AixG         = matmul(Ai, w1[2]);
+            AixZ         = matmul(Ai, w1[3]);
+            if (spec.type == DG)
+              for (auto &e : AixG.a)
+                e = -e;
+            else
+              for (auto &e : AixZ.a)
+                e = -e;
+          }
+        else if (spec.type == CGP)
+          rhsK = w[2], rhsM = w[3];
+        else
+          rhsK = zero, rhsM = w[2];
         F.Alpha  = w[0];
         F.Beta   = w[1];
         F.nt     = w[0].m;
         F.At     = F.Alpha.a;
         F.Bt     = F.Beta.a;
         nt       = F.nt;
-        Mat zero(w[2].m, 1);
-        if (spec.type == CGP)
-          rhsK = w[2], rhsM = w[3];
-        else
-          rhsK = zero, rhsM = w[2];
-        auto w1 = fe_time_weights(spec.type, spec.r, tau, 1);
         Alpha1  = w1[0];
         Gamma1  = w1[2];
       }
@@ -1929,6 +1968,15 @@ namespace orc
     {
       if (spec.rhs == 1)
         return 0.0;
+      if (spec.problem == 1)
+        {
+          // wave::RHSFunction (exact_solution.h:166-180)
+          const double f = spec.frequency, PI = M_PI;
+          double v = std::pow(2.0, spec.dim) * (PI * f) * (PI * f) * std::sin(2 * PI * f * t);
+          for (int d = 0; d < spec.dim; ++d)
+            v *= std::sin(2 * PI * f * x[d]);
+          return v;
+        }
       const double f = spec.frequency, PI = M_PI;
       double       v = spec.dim * 4 * PI * PI * f * f * std::sin(2 * PI * f * t) +
                  2 * PI * f * std::cos(2 * PI * f * t);
@@ -1936,6 +1984,37 @@ namespace orc
         v *= std::sin(2 * PI * f * x[d]);
       return v;
     }
+    // VectorTools::interpolate of wave::ExactSolutionV at t (exact_solution.h:
+    // 143-163) onto the fine nodes (support points through the Q1 mapping)
+    std::vector<double> interpolate_exact_v(double t) const
+    {
+      const Space        &sp = fine.sp;
+      const int           dim = sp.dim, n = sp.n;
+      std::vector<double> out(sp.ndofs, 0.0);
+      std::vector<long>   idx;
+      const double        f = spec.frequency, PI = M_PI;
+      for (long c = 0; c < sp.g.n_cells(); ++c)
+        {
+          sp.cell_dofs(c, idx);
+          double V[8][3];
+          sp.cell_vertices(c, V);
+          int l = 0;
+          for (int c3 = 0; c3 < (dim == 3 ? n : 1); ++c3)
+            for (int b = 0; b < n; ++b)
+              for (int a = 0; a < n; ++a, ++l)
+                {
+                  double xi[3] = {sh.gll[a], sh.gll[b], dim == 3 ? sh.gll[c3] : 0.0};
+                  double x[3], J[3][3];
+                  q1_map(dim, V, xi, x, J);
+                  double v = 2 * PI * f * std::cos(2 * PI * f * t);
+                  for (int d = 0; d < dim; ++d)
+                    v *= std::sin(2 * PI * f * x[d]);
+                  out[idx[l]] = sp.constrained[idx[l]] ? 0.0 : v;
+                }
+        }
+      return out;
+    }
+
     double exact_value(double t, const double *x) const
     {
       const double f = spec.frequency, PI = M_PI;
@@ -2092,10 +2171,65 @@ namespace orc
     }
 
     // TimeIntegratorHeat::solve (time_integrators.h:157-180)
-    int solve_slab(std::vector<double> &x, const std::vector<double> &prev, double time)
+    // TimeIntegratorWave::solve (time_integrators.h:237-280): the
+    // displacement slab (heat path with the wave weights plus
+    // rhs_matrix_v * prev_v), then the velocities
+    //   v_it = AixB u_it + AixG prev_u_   (dG)
+    //   v_it = AixB u_it + AixG prev_v_ + AixZ prev_u_   (cG)
+    // with prev_u_/prev_v_ the previous step's last block.
+    int solve_slab_wave(std::vector<double> &x, std::vector<double> &v,
+                        const std::vector<double> &prev, const std::vector<double> &prev_v,
+                        double time)
+    {
+      const int  rc = solve_slab(x, prev, time, &prev_v);
+      const long N  = fine.N();
+      const int  ni = AixB.m;
+      std::fill(v.begin(), v.end(), 0.0);
+      for (int it = 0; it < spec.nts; ++it)
+        {
+          const double *pu = (it == 0) ? prev.data() : x.data() + (long)(it * ni - 1) * N;
+          const double *pv = (it == 0) ? prev_v.data() : v.data() + (long)(it * ni - 1) * N;
+          for (int i = 0; i < ni; ++i)
+            {
+              double *vi = v.data() + (long)(it * ni + i) * N;
+              for (int j = 0; j < ni; ++j)
+                if (AixB(i, j) != 0.0)
+                  for (long k = 0; k < N; ++k)
+                    vi[k] += AixB(i, j) * x[(long)(it * ni + j) * N + k];
+              if (spec.type == DG)
+                {
+                  if (AixG(i, 0) != 0.0)
+                    for (long k = 0; k < N; ++k)
+                      vi[k] += AixG(i, 0) * pu[k];
+                }
+              else
+                {
+                  if (AixG(i, 0) != 0.0)
+                    for (long k = 0; k < N; ++k)
+                      vi[k] += AixG(i, 0) * pv[k];
+                  if (AixZ(i, 0) != 0.0)
+                    for (long k = 0; k < N; ++k)
+                      vi[k] += AixZ(i, 0) * pu[k];
+                }
+            }
+        }
+      return rc;
+    }
+
+    int solve_slab(std::vector<double> &x, const std::vector<double> &prev, double time,
+                   const std::vector<double> *prev_v = nullptr)
     {
       const long          N = fine.N();
       std::vector<double> rhs(fine.size(), 0.0), tmp(N);
+      // wave: rhs_matrix_v.vmult_add(rhs, prev_v), mass part only
+      if (prev_v && !rhsV.all_zero())
+        {
+          spmv_free(fine.M, fine.Mv, fine.sp.constrained, prev_v->data(), tmp.data());
+          for (int j = 0; j < nt; ++j)
+            if (rhsV(j, 0) != 0.0)
+              for (long k = 0; k < N; ++k)
+                rhs[j * N + k] += rhsV(j, 0) * tmp[k];
+        }
       // rhs_matrix.vmult(rhs, prev_x) (operators.h:154-187)
       if (!rhsK.all_zero())
         {
@@ -2435,6 +2569,23 @@ extern "C"
     });
     return rc;
   }
+  // One wave slab (time_integrators.h:237-280): x (displacement) and v
+  // (velocity), nt blocks each, overwritten.
+  int oracle_solve_slab_wave(void *h, double *x, double *v, const double *prev,
+                             const double *prev_v, double time, int *its)
+  {
+    int rc = 0;
+    ORC_DISPATCH(h, {
+      const long          N = P.fine.N();
+      std::vector<double> xv(x, x + (long)P.nt * N), vv((long)P.nt * N), pv(prev, prev + N),
+        pvv(prev_v, prev_v + N);
+      rc = P.solve_slab_wave(xv, vv, pv, pvv, time);
+      std::copy(xv.begin(), xv.end(), x);
+      std::copy(vv.begin(), vv.end(), v);
+      *its = P.last_its;
+    });
+    return rc;
+  }
   // ErrorCalculator::evaluate_error of one slab (exact_solution.h:218-299):
   // out = {L2^2, Linf, H1semi^2}; x: nt blocks, prev: one block.
   int oracle_slab_error(void *h, const double *x, const double *prev, double time, double *out)
@@ -2454,15 +2605,31 @@ extern "C"
     ORC_DISPATCH(h, {
       const long          N  = P.fine.N();
       const int           nt = P.nt;
+      const bool          wave = P.spec.problem == 1;
       std::vector<double> x(size_t(N) * nt, 0.0), prev(N, 0.0);
-      // initial value = exact solution at t=0 (zero for the manufactured case)
+      std::vector<double> v(wave ? size_t(N) * nt : 0, 0.0), prev_v(wave ? N : 0, 0.0);
+      // initial values (tp_01.cc:566-568): u(0) = 0 for the manufactured
+      // solution; the wave velocity interpolates v(0)
+      if (wave)
+        {
+          auto v0 = P.interpolate_exact_v(0.0);
+          std::copy(v0.begin(), v0.end(), v.begin() + (long)(nt - 1) * N);
+        }
       double time = 0.0;
       int    slab = 0, total = 0;
       double l2 = 0, l8 = -1, h1 = 0;
       while (time < P.spec.end_time)
         {
           std::copy(x.begin() + (long)(nt - 1) * N, x.begin() + (long)nt * N, prev.begin());
-          int r = P.solve_slab(x, prev, time);
+          int r;
+          if (wave)
+            {
+              std::copy(v.begin() + (long)(nt - 1) * N, v.begin() + (long)nt * N,
+                        prev_v.begin());
+              r = P.solve_slab_wave(x, v, prev, prev_v, time);
+            }
+          else
+            r = P.solve_slab(x, prev, time);
           if (r != 0)
             rc = r;
           if (its_per_slab && slab < cap)

@@ -39,6 +39,10 @@ CASES = {
     # configs[4] shape (Q4, dG(3)), 2^3 cells, FP32 multigrid
     "3d_q4_dg3_fp32": dict(spec=dict(dim=3, refine=1, type=2, r=3, nts=1, p=4, r_min=2, vanka_invert_T=0,
                                      mg_float=1), perturb=0.1, kappa=True),
+    # configs[4] shape with the factored Vanka set up on the device
+    "3d_q4_dg3_fp32_factored": dict(spec=dict(dim=3, refine=1, type=2, r=3, nts=1, p=4, r_min=2,
+                                              mg_float=1, vanka_factored=1),
+                                    perturb=0.1, kappa=True),
     # FP32 multigrid with the factored Vanka (the benchmark configuration)
     "3d_q3_dg2_fp32_factored": dict(spec=dict(dim=3, refine=2, type=2, r=2, nts=1, p=3, r_min=0,
                                               mg_float=1, vanka_factored=1, vanka_invert_T=0),
@@ -263,6 +267,32 @@ def test_dense_coarse_operator(name, dense):
     assert rel_err(y1.cpu().numpy(), y2.cpu().numpy()) < (1e-4 if fp32 else 1e-10)
     ref = o.gmg_vmult(x)
     assert rel_err(nodes_to_blocks(y2.cpu().numpy(), nt), ref) < (1e-4 if fp32 else 1e-10)
+    h.close()
+
+
+@pytest.mark.parametrize("name", ["3d_q3_dg2_factored", "3d_q3_dg2_fp32_factored"])
+def test_vanka_device_setup_matches_host(name):
+    """The factored Vanka set up on the GPU (element and patch matrices,
+    batched generalized eigensolver, time blocks) applies the same operator
+    as the one set up on the host, on every level."""
+    spec, o, g = problems(name)
+    c = CASES[name]
+    dim = spec["dim"]
+    nc = [1 << spec["refine"]] * dim
+    X = box_vertices(dim, nc, perturb=c["perturb"], seed=7)
+    kap = log_uniform(int(np.prod(nc)), 1e-2, 1e2, seed=11)
+    h = st.Problem(dict(spec, vanka_device_setup=0), vertices=X, kappa=kap)
+    rng = np.random.default_rng(31)
+    dt = dtype_of(g)
+    for l in range(g.n_levels):
+        info = g.level_info(l)
+        x = torch.tensor(rng.standard_normal(info["n_nodes"] * info["nt"]), dtype=dt, device=DEV)
+        y1, y2 = torch.zeros_like(x), torch.zeros_like(x)
+        print(f"[case {name}] level {l} vanka device vs host setup", flush=True)
+        g.vanka(l, y1, x)
+        h.vanka(l, y2, x)
+        tol = 1e-5 if g.level_size == 4 else 1e-11
+        assert rel_err(y1.cpu().double().numpy(), y2.cpu().double().numpy()) < tol, l
     h.close()
 
 

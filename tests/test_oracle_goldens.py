@@ -227,3 +227,49 @@ def test_tp01_refinement4_samples():
         res = orc.Problem(spec).run_heat()
         assert res["total_its"] == total
         assert abs(res["Linf"] - linf) <= 2e-5 * linf
+
+
+# -------------------------------------------------------------- tp_01 wave
+TFW = [("tf05", orc.DG, 4), ("tf06", orc.CGP, 4), ("tf07", orc.DG, 1), ("tf08", orc.CGP, 1)]
+
+
+def tp01_wave_runs(max_ref):
+    """tp_01.cc:797-800 wave files tf05-tf08 (problemType wave, frequency 1:
+    u = sin(2 pi t) sin(2 pi x) sin(2 pi y), v(0) interpolated), after the 48
+    heat runs in the transcript."""
+    gold = load("tp_01")
+    out = []
+    k = 48
+    for name, type_, nts in TFW:
+        for j in range(3):
+            r = (1 if type_ == orc.DG else 2) + j
+            for ref in range(2, 6):
+                g = gold["runs"][k]
+                row = gold["tables"][k // 4]["rows"][ref - 2]
+                k += 1
+                if ref <= max_ref:
+                    out.append((dict(dim=2, refine=ref, type=type_, r=r, nts=nts, mg_float=1,
+                                     problem=1, r_min=0 if type_ == orc.DG else 1), g, row))
+    return out
+
+
+@pytest.mark.parametrize("max_ref", [3 if SLOW else 2])
+def test_tp01_wave_transcript(max_ref):
+    """The oracle's wave path (TimeIntegratorWave) reproduces the wave runs:
+    DoF, level and slab counts, the error norms to the printed digits, and
+    the average iterations to +-1 per slab with most runs exact."""
+    exact, n = 0, 0
+    for spec, g, row in tp01_wave_runs(max_ref):
+        P = orc.Problem(spec)
+        res = P.run_heat()
+        assert res["ndofs"] == g["dofs"] == row["s_dofs"]
+        assert P.n_levels - 1 == g["max_level"]
+        assert res["slabs"] == g["slabs"]
+        assert res["nblocks"] == row["t_dofs"]
+        for key in ("L2", "Linf", "H1"):
+            assert abs(res[key] - row[key]) <= 2e-5 * abs(row[key]), (spec, key, res[key], row[key])
+        avg = res["total_its"] / res["slabs"]
+        assert abs(avg - g["avg_its"]) <= 1.0, (spec, avg, g["avg_its"])
+        exact += abs(avg - g["avg_its"]) < 1e-12
+        n += 1
+    assert exact >= int(0.7 * n), (exact, n)
