@@ -350,6 +350,22 @@ extern "C"
     });
     return g != STFEM_OK ? g : rc;
   }
+  int stfem_solve_slab_wave(stfem_problem p, double *x, double *v, const double *prev_u,
+                             const double *prev_v, double time, int *its)
+  {
+    int rc = STFEM_OK;
+    int g  = guarded([&] {
+      auto &pb = P(p);
+      device_barrier(pb);
+      rc = pb.solve_slab_wave(x, v, prev_u, prev_v, time, its);
+      pb.context().sync();
+      if (rc == STFEM_ERR_NO_CONVERGENCE)
+        g_last_error = "FGMRES: iteration limit reached";
+      else if (rc == STFEM_ERR_NOT_FINITE)
+        g_last_error = "FGMRES: residual is not finite";
+    });
+    return g != STFEM_OK ? g : rc;
+  }
   int stfem_set_load_vector(stfem_problem p, const double *h)
   {
     return guarded([&] {

@@ -28,7 +28,45 @@ namespace stfem
       if (idx < total)
         sweep_item(idx, in, out, nt, d, n0, n1, m, cnt, col, w, K, add);
     }
+    __global__ void k_wave_velocity(long n_nodes, int nts, int ni, int dg,
+                                    const double *__restrict__ AixB,
+                                    const double *__restrict__ AixG,
+                                    const double *__restrict__ AixZ, const double *__restrict__ u,
+                                    const double *__restrict__ prev_u,
+                                    const double *__restrict__ prev_v, double *__restrict__ v)
+    {
+      const long node = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      if (node >= n_nodes)
+        return;
+      const int     nt = nts * ni;
+      const double *un = u + node * nt;
+      double       *vn = v + node * nt;
+      for (int it = 0; it < nts; ++it)
+        {
+          const double pu = it == 0 ? prev_u[node] : un[it * ni - 1];
+          const double pv = it == 0 ? prev_v[node] : vn[it * ni - 1];
+          for (int i = 0; i < ni; ++i)
+            {
+              double s = 0.0;
+              for (int j = 0; j < ni; ++j)
+                s += AixB[i * ni + j] * un[it * ni + j];
+              s += dg ? AixG[i] * pu : AixG[i] * pv + AixZ[i] * pu;
+              vn[it * ni + i] = s;
+            }
+        }
+    }
   } // namespace
+
+  void launch_wave_velocity(long n_nodes, int nts, int ni, bool dg, const double *AixB,
+                            const double *AixG, const double *AixZ, const double *u,
+                            const double *prev_u, const double *prev_v, double *v, hipStream_t s)
+  {
+    if (n_nodes == 0)
+      return;
+    k_wave_velocity<<<dim3((unsigned)((n_nodes + 255) / 256)), 256, 0, s>>>(
+      n_nodes, nts, ni, dg ? 1 : 0, AixB, AixG, AixZ, u, prev_u, prev_v, v);
+    STFEM_HIP(hipGetLastError());
+  }
 
   template <typename T>
   void launch_time_transfer(long n_nodes, const T *M, int ntd, int nts, const T *src, T *dst,

@@ -147,6 +147,18 @@ namespace stfem
                          const double *S, const double *D, int npts, const double *tpts,
                          double freq, double *cell_err, hipStream_t s);
 
+  // ------------------------------------------------------- wave velocity
+  // TimeIntegratorWave velocity recovery per node (node-major u, v with
+  // nt = nts * ni values per node): for every step it,
+  //   v_it = AixB u_it + AixG pu       (dG,  pu: previous step's last u)
+  //   v_it = AixB u_it + AixG pv + AixZ pu   (cG, pv: previous step's last v)
+  // with pu/pv = prev_u/prev_v for the first step.  AixB: ni x ni
+  // row-major, AixG, AixZ: ni values (device, FP64).
+  void launch_wave_velocity(long n_nodes, int nts, int ni, bool dg, const double *AixB,
+                            const double *AixG, const double *AixZ, const double *u,
+                            const double *prev_u, const double *prev_v, double *v,
+                            hipStream_t s);
+
   // ------------------------------------------------- dense coarse operator
   // v = e_j (length n)
   template <typename T>

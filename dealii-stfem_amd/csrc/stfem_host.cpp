@@ -1679,6 +1679,8 @@ namespace stfem
     geti("mg_float", ps.mg_float);
     geti("dirichlet", ps.dirichlet);
     geti("atomic_scatter", ps.atomic_scatter);
+    if (kv.count("problem"))
+      ps.problem = (kv["problem"] == "wave" || kv["problem"] == "1") ? 1 : 0;
     geti("vanka_device_setup", ps.vanka_device_setup);
     geti("smoothing_steps", ps.mg.smoothing_steps);
     getb("variable", ps.mg.variable);
@@ -1778,8 +1780,17 @@ namespace stfem
                                                    PreconditionVanka<T>::Form::explicit_inverse;
         for (int l = 0; l < nlev; ++l)
           {
-            const TimeWeights w = get_fe_time_weights(spec.type, rl[l], tl[l], nl[l]);
-            const MeshLevel  &m = mesh_[sref[l]];
+            TimeWeights w = get_fe_time_weights(spec.type, rl[l], tl[l], nl[l]);
+            if (spec.problem == 1)
+              {
+                // wave levels: the Schur-in-time transform of the level's
+                // multi-step weights taken as one interval (fe_time.h:383-412)
+                const WaveWeights ww =
+                  get_fe_time_weights_wave(spec.type, w.Alpha, w.Beta, w.Gamma, w.Zeta, 1);
+                w.Alpha = ww.lhs_uK;
+                w.Beta  = ww.lhs_uM;
+              }
+            const MeshLevel &m = mesh_[sref[l]];
             levels[l].A  = std::make_unique<SystemMatrix<T>>(ctx_, m, w.Alpha, w.Beta,
                                                              spec.atomic_scatter == 0);
             levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form,
@@ -1806,16 +1817,40 @@ namespace stfem
         gmg_ = std::make_unique<GMG<T>>(ctx_, std::move(levels), std::move(tr), spec.mg);
         plog("relaxation estimates and dense coarse operator");
         // FP64 fine system and rhs matrices (tp_01.cc:152-198, heat)
-        const MeshLevel  &F = mesh_.back();
-        const TimeWeights w = get_fe_time_weights(spec.type, spec.r, tau, spec.nts);
-        nt                  = w.Alpha.m;
-        fine_ = std::make_unique<SystemMatrix<double>>(ctx_, F, w.Alpha, w.Beta,
-                                                       spec.atomic_scatter == 0);
-        DMat zero(w.Gamma.m, 1);
-        const DMat rK = (spec.type == TimeStepType::CGP) ? w.Gamma : zero;
-        const DMat rM = (spec.type == TimeStepType::CGP) ? w.Zeta : w.Gamma;
-        rhs_ = std::make_unique<SystemMatrix<double>>(ctx_, F, rK, rM, spec.atomic_scatter == 0);
+        const MeshLevel  &F  = mesh_.back();
+        const TimeWeights w  = get_fe_time_weights(spec.type, spec.r, tau, spec.nts);
         const TimeWeights w1 = get_fe_time_weights(spec.type, spec.r, tau, 1);
+        const bool        tp = spec.atomic_scatter == 0;
+        nt                   = w.Alpha.m;
+        DMat zero(w.Gamma.m, 1);
+        if (spec.problem == 1)
+          {
+            // tp_01.cc:167-197: system (Alpha_lhs, Beta_lhs), rhs_matrix
+            // (rhs_uK, rhs_uM) on prev_u, rhs_matrix_v (0, rhs_vM) on prev_v
+            const WaveWeights ww =
+              get_fe_time_weights_wave(spec.type, w1.Alpha, w1.Beta, w1.Gamma, w1.Zeta, spec.nts);
+            fine_ = std::make_unique<SystemMatrix<double>>(ctx_, F, ww.lhs_uK, ww.lhs_uM, tp);
+            rhs_  = std::make_unique<SystemMatrix<double>>(ctx_, F, ww.rhs_uK, ww.rhs_uM, tp);
+            rhsv_ = std::make_unique<SystemMatrix<double>>(ctx_, F, DMat(ww.rhs_vM.m, 1),
+                                                           ww.rhs_vM, tp);
+            // TimeIntegratorWave velocity matrices (time_integrators.h:219-233)
+            const DMat Ai = w1.Alpha.inverse();
+            DMat       AixB = Ai * w1.Beta, AixG = Ai * w1.Gamma, AixZ = Ai * w1.Zeta;
+            if (spec.type == TimeStepType::DG)
+              AixG.scale(-1.0);
+            else
+              AixZ.scale(-1.0);
+            aixb_.upload(AixB.a.data(), AixB.a.size(), ctx_.stream);
+            aixg_.upload(AixG.a.data(), AixG.a.size(), ctx_.stream);
+            aixz_.upload(AixZ.a.data(), AixZ.a.size(), ctx_.stream);
+          }
+        else
+          {
+            fine_ = std::make_unique<SystemMatrix<double>>(ctx_, F, w.Alpha, w.Beta, tp);
+            const DMat rK = (spec.type == TimeStepType::CGP) ? w.Gamma : zero;
+            const DMat rM = (spec.type == TimeStepType::CGP) ? w.Zeta : w.Gamma;
+            rhs_ = std::make_unique<SystemMatrix<double>>(ctx_, F, rK, rM, tp);
+          }
         alpha1_ = w1.Alpha;
         gamma1_ = w1.Gamma;
         solver_ = std::make_unique<SolverFGMRES>(ctx_, fine_->m(), spec.solver);
@@ -1921,9 +1956,31 @@ namespace stfem
       // TimeIntegratorHeat::solve (time_integrators.h:157-180)
       int solve_slab(double *x, const double *prev, double time, int *its) override
       {
+        STFEM_REQUIRE(spec.problem == 0, "solve_slab: the wave problem needs solve_slab_wave");
+        return solve_displacement(x, prev, nullptr, time, its);
+      }
+      // TimeIntegratorWave::solve (time_integrators.h:237-280): the
+      // displacement slab with rhs_matrix_v * prev_v added to the right-hand
+      // side, then the velocities of every step from the slab's
+      // displacements and the previous step's last displacement/velocity
+      int solve_slab_wave(double *x, double *v, const double *prev_u, const double *prev_v,
+                          double time, int *its) override
+      {
+        STFEM_REQUIRE(spec.problem == 1, "solve_slab_wave: the problem is not the wave problem");
+        const int rc = solve_displacement(x, prev_u, prev_v, time, its);
+        launch_wave_velocity(fine_->n_nodes(), spec.nts, nt / spec.nts,
+                             spec.type == TimeStepType::DG, aixb_.get(), aixg_.get(),
+                             aixz_.get(), x, prev_u, prev_v, v, ctx_.stream);
+        return rc;
+      }
+      int solve_displacement(double *x, const double *prev, const double *prev_v, double time,
+                             int *its)
+      {
         const long N  = fine_->n_nodes();
         double    *rh = rhs_vec_.get();
         STFEM_HIP(hipMemsetAsync(rh, 0, size_t(N) * nt * sizeof(double), ctx_.stream));
+        if (prev_v)
+          rhsv_->vmult_add_nx1(rh, prev_v);
         rhs_->vmult_add_nx1(rh, prev);
         if (spec.source != 1 && load_.size())
           {
@@ -2017,20 +2074,35 @@ namespace stfem
       }
       int run_heat(double *stats, int *its_per_slab, int cap) override
       {
-        const long   N = fine_->n_nodes();
+        const long   N    = fine_->n_nodes();
+        const bool   wave = spec.problem == 1;
         DBuf<double> x{size_t(N) * nt}, prev{size_t(N)}, last{size_t(nt)};
+        DBuf<double> v{wave ? size_t(N) * nt : 0}, prev_v{wave ? size_t(N) : 0};
         launch_fill_zero<double>(x.get(), x.size(), ctx_.stream);
         std::vector<double> sel(nt, 0.0);
         sel[nt - 1] = 1.0; // picks the last block of a node-major vector
         last.upload(sel.data(), sel.size(), ctx_.stream);
+        // initial values (tp_01.cc:566-568): u(0) = 0 for the manufactured
+        // solution, the wave velocity interpolates v(0) = 2 pi f prod sin
+        if (wave)
+          {
+            const std::vector<double> v0 = interpolate_exact_v(0.0);
+            prev_v.upload(v0.data(), size_t(N), ctx_.stream);
+            ctx_.sync();
+          }
         double time = 0.0, l2 = 0.0, l8 = -1.0, h1 = 0.0;
         int    slab = 0, total = 0, rc = 0;
         while (time < spec.end_time)
           {
             launch_time_transfer<double>(N, last.get(), 1, nt, x.get(), prev.get(), false,
                                          ctx_.stream);
+            if (wave && slab > 0)
+              launch_time_transfer<double>(N, last.get(), 1, nt, v.get(), prev_v.get(), false,
+                                           ctx_.stream);
             int       its = 0;
-            const int r   = solve_slab(x.get(), prev.get(), time, &its);
+            const int r   = wave ? solve_slab_wave(x.get(), v.get(), prev.get(), prev_v.get(),
+                                                   time, &its)
+                                 : solve_slab(x.get(), prev.get(), time, &its);
             if (r != 0)
               rc = r;
             if (its_per_slab && slab < cap)
@@ -2069,6 +2141,49 @@ namespace stfem
 
     private:
       DBuf<double> err_tp_, err_cells_; // slab error: time table, per-cell errors
+      // VectorTools::interpolate of wave::ExactSolutionV (exact_solution.h:
+      // 143-163) at the fine nodes: support points are the GLL points mapped
+      // by each cell's Q1 map; constrained nodes stay 0
+      std::vector<double> interpolate_exact_v(double t) const
+      {
+        const MeshLevel    &m = mesh_.back();
+        const GridDesc     &g = m.g;
+        const int           dim = g.dim, n = sh_.n;
+        const double        PI = 3.14159265358979323846, f = spec.frequency;
+        const long          nv0 = g.nc[0] + 1, nv1 = g.nc[1] + 1;
+        std::vector<double> out(g.n_nodes, 0.0);
+        for (long c = 0; c < g.n_cells; ++c)
+          {
+            int cx, cy, cz;
+            cell_xyz(g, c, cx, cy, cz);
+            for (int c3 = 0; c3 < ((dim == 3) ? n : 1); ++c3)
+              for (int b = 0; b < n; ++b)
+                for (int a = 0; a < n; ++a)
+                  {
+                    const long gx = (long)cx * sh_.p + a, gy = (long)cy * sh_.p + b,
+                               gz = (long)cz * sh_.p + c3;
+                    if (node_constrained(g, gx, gy, gz))
+                      continue;
+                    const double xi[3] = {sh_.gll[a], sh_.gll[b], dim == 3 ? sh_.gll[c3] : 0.0};
+                    double       x[3] = {0, 0, 0};
+                    for (int vtx = 0; vtx < (1 << dim); ++vtx)
+                      {
+                        const long id = (cx + (vtx & 1)) +
+                                        nv0 * ((cy + ((vtx >> 1) & 1)) + nv1 * (cz + ((vtx >> 2) & 1)));
+                        double N0 = 1;
+                        for (int e = 0; e < dim; ++e)
+                          N0 *= ((vtx >> e) & 1) ? xi[e] : 1 - xi[e];
+                        for (int d = 0; d < dim; ++d)
+                          x[d] += N0 * m.X[id * dim + d];
+                      }
+                    double val = 2 * PI * f * std::cos(2 * PI * f * t);
+                    for (int d = 0; d < dim; ++d)
+                      val *= std::sin(2 * PI * f * x[d]);
+                    out[gx + g.nn[0] * (gy + g.nn[1] * gz)] = val;
+                  }
+          }
+        return out;
+      }
       // time profile g(t) of the source f(t,x) = g(t) h(x): the manufactured
       // heat source of exact_solution.h:53-75 splits this way; the other
       // sources are constant in time
@@ -2077,6 +2192,8 @@ namespace stfem
         if (spec.source != 0)
           return 1.0;
         const double PI = 3.14159265358979323846, f = spec.frequency;
+        if (spec.problem == 1) // wave::RHSFunction (exact_solution.h:166-180)
+          return std::pow(2.0, spec.dim) * (PI * f) * (PI * f) * std::sin(2 * PI * f * t);
         return spec.dim * 4 * PI * PI * f * f * std::sin(2 * PI * f * t) +
                2 * PI * f * std::cos(2 * PI * f * t);
       }
@@ -2158,10 +2275,11 @@ namespace stfem
       Shape1D                               sh_;
       std::vector<MeshLevel>                mesh_;
       std::unique_ptr<GMG<T>>               gmg_;
-      std::unique_ptr<SystemMatrix<double>> fine_, rhs_;
+      std::unique_ptr<SystemMatrix<double>> fine_, rhs_, rhsv_;
       std::unique_ptr<SolverFGMRES>         solver_;
       DMat                                  alpha1_, gamma1_;
       DBuf<double>                          rhs_vec_, load_, coefs_, ones_;
+      DBuf<double>                          aixb_, aixg_, aixz_; // wave velocity recovery
     };
   } // namespace
 

@@ -416,6 +416,7 @@ namespace stfem
     int          mg_float = 0;
     int          dirichlet = 63;
     int          atomic_scatter = 0; // 1: one-pass operator with atomic adds
+    int          problem = 0; // 0 heat, 1 wave (first order in time, tp_01.cc:167-197)
     int          vanka_device_setup = 1; // factored Vanka set up on the GPU
     GMGOptions   mg;
     SolverControlData solver;
@@ -447,6 +448,10 @@ namespace stfem
     virtual void gmg_vmult(double *dst, const double *src) = 0;
     virtual int  fgmres(double *x, const double *b, int *its, std::vector<double> *hist) = 0;
     virtual int  solve_slab(double *x, const double *prev, double time, int *its) = 0;
+    // TimeIntegratorWave::solve: displacement x and velocity v of one slab
+    // from the previous slab's last displacement and velocity
+    virtual int  solve_slab_wave(double *x, double *v, const double *prev_u,
+                                 const double *prev_v, double time, int *its) = 0;
     // ErrorCalculator::evaluate_error of one slab: out = {L2^2, Linf, H1semi^2}
     virtual void slab_error(const double *x, const double *prev, double time, double *out) = 0;
     // the heat time loop of tp_01.cc:630-683 from zero initial values:

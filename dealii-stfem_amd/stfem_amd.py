@@ -54,6 +54,7 @@ _SIGS = {
     "stfem_gmg_vmult": ([_vp, _vp, _vp], ctypes.c_int),
     "stfem_fgmres_solve": ([_vp, _vp, _vp, _ip, _dp, ctypes.c_int], ctypes.c_int),
     "stfem_solve_slab": ([_vp, _vp, _vp, ctypes.c_double, _ip], ctypes.c_int),
+    "stfem_solve_slab_wave": ([_vp, _vp, _vp, _vp, _vp, ctypes.c_double, _ip], ctypes.c_int),
     "stfem_slab_error": ([_vp, _vp, _vp, ctypes.c_double, _dp], ctypes.c_int),
     "stfem_run_heat": ([_vp, _dp, _ip, ctypes.c_int], ctypes.c_int),
     "stfem_set_load_vector": ([_vp, _dp], ctypes.c_int),
@@ -250,6 +251,13 @@ class Problem:
     def solve_slab(self, x, prev, time):
         its = ctypes.c_int()
         rc = load().stfem_solve_slab(self.h, _ptr(x), _ptr(prev), float(time), ctypes.byref(its))
+        return rc, its.value
+
+    def solve_slab_wave(self, x, v, prev_u, prev_v, time):
+        """One wave slab: displacement x and velocity v (nt values per node)."""
+        its = ctypes.c_int()
+        rc = load().stfem_solve_slab_wave(self.h, _ptr(x), _ptr(v), _ptr(prev_u), _ptr(prev_v),
+                                          float(time), ctypes.byref(its))
         return rc, its.value
 
     def slab_error(self, x, prev, time):

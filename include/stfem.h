@@ -84,7 +84,10 @@ int stfem_time_mg_sequence(int n_sp_lvl, int k_max, int k_min, int nts, int nts_
  * vanka_factored, exact_single_cell, use_graph, maxiter, abstol, reltol,
  * basis, extrapolate, end_time, frequency, source (0 manufactured heat
  * source, 1 none, 2 load vector set by stfem_set_load_vector, 3 seeded
- * Gaussian bumps with n_bumps, seed), dirichlet (face bit mask).
+ * Gaussian bumps with n_bumps, seed), dirichlet (face bit mask), problem
+ * (heat, or wave: the first-order-in-time wave system of tp_01.cc:167-197,
+ * solved with stfem_solve_slab_wave), atomic_scatter, vanka_device_setup,
+ * dense_coarse_dofs.
  * vertices: finest-level vertex coordinates ((nc+1)^dim points, dim each,
  * x fastest) or NULL for the uniform box; cell_coef: finest-level per-cell
  * coefficient or NULL for 1. */
@@ -124,6 +127,14 @@ int stfem_fgmres_solve(stfem_problem p, double *x, const double *b, int *its, do
  * (rhs_matrix, operators.h:154-187) + source, extrapolated start, FGMRES,
  * Dirichlet nodes zeroed.  x: nt values per node; prev: one per node. */
 int stfem_solve_slab(stfem_problem p, double *x, const double *prev, double time, int *its);
+/* TimeIntegratorWave::solve, time_integrators.h:237-280 (problem=wave, the
+ * first-order-in-time wave system of tp_01.cc:167-197): rhs from prev_u
+ * (rhs_matrix) and prev_v (rhs_matrix_v) + source, extrapolated start,
+ * FGMRES, Dirichlet nodes zeroed; then the velocities v of every block from
+ * x, prev_u and prev_v (AixB, AixG, AixZ of time_integrators.h:219-233).
+ * x, v: nt values per node; prev_u, prev_v: one per node. */
+int stfem_solve_slab_wave(stfem_problem p, double *x, double *v, const double *prev_u,
+                          const double *prev_v, double time, int *its);
 /* ErrorCalculator::evaluate_error (exact_solution.h:218-299) for one slab
  * against the manufactured heat solution (source=0): Gauss(r+1) points in
  * space and time; out[0..2] = {squared L2 error, L-infinity error, squared

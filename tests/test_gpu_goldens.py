@@ -11,14 +11,14 @@ tests use.
 import numpy as np
 import pytest
 
-from helpers import blocks_to_nodes
+from helpers import blocks_to_nodes, nodes_to_blocks, rel_err
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 import oracle as orc  # noqa: E402
 import stfem_amd as st  # noqa: E402
-from test_oracle_goldens import load, tp01_heat_runs, transfer01_runs  # noqa: E402
+from test_oracle_goldens import load, tp01_heat_runs, tp01_wave_runs, transfer01_runs  # noqa: E402
 
 DEV = "cuda:0"
 
@@ -78,6 +78,64 @@ def test_tp01_transcript_on_gpu():
             assert abs(res[key] - row[key]) <= 2e-5 * abs(row[key]), (spec, key, res[key], row[key])
         avg = res["total_its"] / res["slabs"]
         assert abs(avg - g["avg_its"]) <= 0.25, (spec, avg, g["avg_its"])
+        exact += abs(avg - g["avg_its"]) < 1e-12
+        n += 1
+        P.close()
+    assert exact >= int(0.7 * n), (exact, n)
+
+
+@pytest.mark.parametrize("spec", [dict(dim=2, refine=3, type=2, r=1, nts=1),
+                                  dict(dim=2, refine=3, type=2, r=2, nts=4),
+                                  dict(dim=2, refine=3, type=1, r=2, nts=1, r_min=1),
+                                  dict(dim=2, refine=3, type=1, r=3, nts=4, r_min=1),
+                                  dict(dim=3, refine=2, type=2, r=1, nts=2)])
+def test_wave_slabs_match_oracle(spec):
+    """TimeIntegratorWave::solve on the GPU against the oracle's, slab by
+    slab: displacements and velocities of every block and the FGMRES
+    iteration counts (the right-hand side from prev_v enters from the
+    second slab on, where the velocities are non-zero)."""
+    spec = dict(spec, problem=1, mg_float=1)
+    o, g = orc.Problem(spec), st.Problem(spec)
+    n, nt = g.n_nodes, g.nt
+    xo, vo = np.zeros(n * nt), np.zeros(n * nt)
+    xg = torch.zeros(n * nt, dtype=torch.float64, device=DEV)
+    vg = torch.zeros(n * nt, dtype=torch.float64, device=DEV)
+    pu = torch.zeros(n, dtype=torch.float64, device=DEV)
+    pv = torch.zeros(n, dtype=torch.float64, device=DEV)
+    time = 0.0
+    for k in range(3):
+        pu_o, pv_o = xo[(nt - 1) * n:].copy(), vo[(nt - 1) * n:].copy()
+        pu.copy_(torch.tensor(pu_o, device=DEV))
+        pv.copy_(torch.tensor(pv_o, device=DEV))
+        rc_o, xo, vo, its_o = orc.solve_slab_wave(o, xo, pu_o, pv_o, time)
+        print(f"[wave slab {k}] {spec}", flush=True)
+        rc_g, its_g = g.solve_slab_wave(xg, vg, pu, pv, time)
+        assert rc_o == 0 and rc_g == 0
+        assert abs(its_g - its_o) <= 1, (k, its_g, its_o)
+        assert rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo) < 1e-7, k
+        assert rel_err(nodes_to_blocks(vg.cpu().numpy(), nt), vo) < 1e-7, k
+        time += spec["nts"] * g.tau
+    g.close()
+
+
+def test_tp01_wave_transcript_on_gpu():
+    """The wave runs tf05-tf08 of tests/tp_01.output (2D, refinements 2-3)
+    through the GPU driver: counts, error norms to the printed digits,
+    average iterations within one per slab, most of them exact."""
+    exact, n = 0, 0
+    for spec, g, row in tp01_wave_runs(3):
+        print(f"[tp_01 wave] {spec}", flush=True)
+        P = st.Problem(spec)
+        res = P.run_heat()
+        assert res["rc"] == 0, (spec, res)
+        assert res["ndofs"] == g["dofs"] == row["s_dofs"]
+        assert P.n_levels - 1 == g["max_level"]
+        assert res["slabs"] == g["slabs"]
+        assert res["nblocks"] == row["t_dofs"]
+        for key in ("L2", "Linf", "H1"):
+            assert abs(res[key] - row[key]) <= 2e-5 * abs(row[key]), (spec, key, res[key], row[key])
+        avg = res["total_its"] / res["slabs"]
+        assert abs(avg - g["avg_its"]) <= 1.0, (spec, avg, g["avg_its"])
         exact += abs(avg - g["avg_its"]) < 1e-12
         n += 1
         P.close()
