@@ -1,0 +1,145 @@
+// Cell-major buffers of the two-pass (atomic-free) scatter (gfx950): the
+// per-cell results of a cell kernel are stored in a fixed layout, and one
+// thread per node sums the contributions of the cells sharing it in a fixed
+// order.  Used by the matrix-free operator and by the Vanka smoother.
+#pragma once
+
+#include "kernels.h"
+
+namespace stfem
+{
+  namespace
+  {
+    // Position of the cell-local value (cell (cx,cy,cz), local node
+    // (a,b,c), output component j) in the cell-major result buffer: for each
+    // row of cells along x, local (c,b) and component, the values of all
+    // cells of the row lie contiguous in x ((cx, a) with a fastest), so that
+    // both the per-cell stores and the per-node gathers are coalesced.  In
+    // 2D the local line index is b (y) and c is unused.
+    template <int DIM, int N>
+    __host__ __device__ __forceinline__ long cell_slot(int nc0, int nc1, int nto, int cx, int cy,
+                                                       int cz, int a, int b, int c, int j)
+    {
+      const long L = long(nc0) * N;
+      if constexpr (DIM == 3)
+        {
+          const long row = long(cz) * nc1 + cy;
+          return (((row * N + c) * N + b) * nto + j) * L + long(cx) * N + a;
+        }
+      else
+        return ((long(cy) * N + b) * nto + j) * L + long(cx) * N + a;
+    }
+    template <int DIM, int N>
+    __device__ __forceinline__ long cell_slot(const GridDesc &g, int nto, int cx, int cy, int cz,
+                                              int a, int b, int c, int j)
+    {
+      return cell_slot<DIM, N>(g.nc[0], g.nc[1], nto, cx, cy, cz, a, b, c, j);
+    }
+
+    // Second pass of the two-pass forms: node `node` of the level sums its
+    // (one to 2^dim) cell-local results from the cell-major buffer, in a
+    // fixed order (cells by increasing x, then y, then z), and stores
+    //   mode 0: dst = sum,  mode 1: dst += sum,  mode 2: dst = rhs - sum.
+    // SKIP_DIR (operator): Dirichlet nodes get zero (modes 0, 2) or stay
+    // unchanged (mode 1); otherwise (Vanka, whose patches cover them) they
+    // are summed like every other node.
+    // NT > 0: compile-time number of components (all loads issued before
+    // the sums); NT == 0: nto at run time.
+    template <int DIM, int N, int NT, typename T, bool SKIP_DIR = true>
+    __device__ __forceinline__ void gather_node(const GridDesc &g, int nto,
+                                                const T *__restrict__ cells, T *__restrict__ dst,
+                                                const T *__restrict__ rhs, int mode, long node)
+    {
+      constexpr int p   = N - 1;
+      const long    nn0 = g.nn[0], nn1 = g.nn[1];
+      const long    gx  = node % nn0, r = node / nn0;
+      const long    gy  = (DIM == 3) ? r % nn1 : r;
+      const long    gz  = (DIM == 3) ? r / nn1 : 0;
+      const int     d   = g.dirichlet;
+      bool bnd = (gx == 0 && (d & 1)) || (gx == nn0 - 1 && (d & 2)) || (gy == 0 && (d & 4)) ||
+                 (gy == nn1 - 1 && (d & 8));
+      if (DIM == 3)
+        bnd = bnd || (gz == 0 && (d & 16)) || (gz == g.nn[2] - 1 && (d & 32));
+      if (!SKIP_DIR)
+        bnd = false;
+      const int nt  = NT > 0 ? NT : nto;
+      T *const  out = dst + node * nt;
+      if (bnd)
+        {
+          if (mode != 1)
+            for (int j = 0; j < nt; ++j)
+              out[j] = T(0);
+          return;
+        }
+      // the cells sharing the node and its local index in each, per direction
+      int cs[3][2] = {}, ls[3][2] = {}, nsh[3] = {1, 1, 1};
+      const long gc[3] = {gx, gy, gz};
+#pragma unroll
+      for (int e = 0; e < DIM; ++e)
+        {
+          int c = int(gc[e] / p), l = int(gc[e] - long(c) * p);
+          if (c == g.nc[e])
+            c = g.nc[e] - 1, l = p;
+          // lower neighbour first: fixed summation order
+          if (l == 0 && c > 0)
+            {
+              cs[e][0] = c - 1;
+              ls[e][0] = p;
+              cs[e][1] = c;
+              ls[e][1] = 0;
+              nsh[e]   = 2;
+            }
+          else
+            {
+              cs[e][0] = c;
+              ls[e][0] = l;
+            }
+        }
+      constexpr int MAXS = 1 << DIM;
+      long          slot[MAXS];
+      int           ns = 0;
+#pragma unroll
+      for (int kz = 0; kz < (DIM == 3 ? 2 : 1); ++kz)
+#pragma unroll
+        for (int ky = 0; ky < 2; ++ky)
+#pragma unroll
+          for (int kx = 0; kx < 2; ++kx)
+            {
+              const bool use = kx < nsh[0] && ky < nsh[1] && (DIM == 2 || kz < nsh[2]);
+              if (use)
+                slot[ns] = (DIM == 3) ? cell_slot<DIM, N>(g, nt, cs[0][kx], cs[1][ky], cs[2][kz],
+                                                          ls[0][kx], ls[1][ky], ls[2][kz], 0)
+                                      : cell_slot<DIM, N>(g, nt, cs[0][kx], cs[1][ky], 0,
+                                                          ls[0][kx], ls[1][ky], 0, 0);
+              ns += use ? 1 : 0;
+            }
+      const long L = long(g.nc[0]) * N; // component stride of cell_slot
+      if constexpr (NT > 0)
+        {
+          T v[MAXS][NT];
+#pragma unroll
+          for (int k = 0; k < MAXS; ++k)
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+              v[k][j] = k < ns ? cells[slot[k] + j * L] : T(0);
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            {
+              T s = v[0][j];
+#pragma unroll
+              for (int k = 1; k < MAXS; ++k)
+                s += v[k][j];
+              out[j] = mode == 0 ? s : mode == 1 ? out[j] + s : rhs[node * NT + j] - s;
+            }
+        }
+      else
+        for (int j = 0; j < nt; ++j)
+          {
+            T s = T(0);
+            for (int k = 0; k < ns; ++k)
+              s += cells[slot[k] + j * L];
+            out[j] = mode == 0 ? s : mode == 1 ? out[j] + s : rhs[node * nt + j] - s;
+          }
+    }
+  } // namespace
+} // namespace stfem

@@ -20,7 +20,7 @@
 // stay in step.
 #pragma once
 
-#include "kernels.h"
+#include "dev_cells.h"
 
 namespace stfem
 {
@@ -137,9 +137,10 @@ namespace stfem
 
     // Factored apply for runtime nt and nloc (fallback shapes), one cell per
     // workgroup.  LDS: x, y (nt*nloc each) and, when STAGE, S (nloc*nloc).
+    // S rows have the stride ld in global memory.
     template <typename T, bool STAGE>
     __device__ __forceinline__ void vanka_factored_cell(const PatchMap &M, int nt, int nloc,
-                                                        const T *__restrict__ S,
+                                                        int ld, const T *__restrict__ S,
                                                         const T *__restrict__ W,
                                                         const T *__restrict__ rhs,
                                                         const T *__restrict__ ax,
@@ -151,13 +152,17 @@ namespace stfem
       T *const ss = ys + nt * nloc;
       int      cx, cy, cz;
       M.cell_xyz(cell, cx, cy, cz);
-      const T *Sg = S + (size_t)cell * nloc * nloc;
+      const T *Sg = S + (size_t)cell * nloc * ld;
       if constexpr (STAGE)
         for (int l = tid; l < nloc * nloc; l += nthr)
-          ss[l] = Sg[l];
+          {
+            const int a = l / nloc, k = l - a * nloc;
+            ss[l]       = Sg[a * ld + k];
+          }
       gather(M, cx, cy, cz, nt, nloc, rhs, ax, xs, tid, nthr);
       __syncthreads();
-      const T *Sc = STAGE ? ss : Sg;
+      const T  *Sc  = STAGE ? ss : Sg;
+      const int sst = STAGE ? nloc : ld;
       const T *Wc = W + (size_t)cell * nloc * nt * nt;
       // y_{i,k} = sum_a S[a][k] x_{i,a}  then  z_{:,k} = W_k y_{:,k}
       for (int k = tid; k < nloc; k += nthr)
@@ -167,7 +172,7 @@ namespace stfem
             yk[i] = T(0);
           for (int a = 0; a < nloc; ++a)
             {
-              const T s = Sc[a * nloc + k];
+              const T s = Sc[a * sst + k];
               for (int i = 0; i < nt; ++i)
                 yk[i] += s * xs[i * nloc + a];
             }
@@ -187,7 +192,7 @@ namespace stfem
           T o[16];
           for (int i = 0; i < nt; ++i)
             o[i] = T(0);
-          const T *Sa = Sc + a * nloc;
+          const T *Sa = Sc + a * sst;
           for (int k = 0; k < nloc; ++k)
             {
               const T s = Sa[k];
@@ -201,158 +206,466 @@ namespace stfem
         }
     }
 
-    // Factored apply with compile-time patch size NL and time dofs NT: 256
-    // threads per cell in P = 256/G groups of G = 64*ceil(NL/64) lanes.  Each
-    // group handles a slice of the contraction index; partial sums meet in
-    // LDS.  S is staged into LDS with a padded row stride (NL+1) so that both
-    // the column reads of S^T x and the row reads of S z are conflict-free.
-    // Four barriers.
+    // Factored apply with compile-time patch size NL and time dofs NT: one
+    // cell per 256-thread workgroup, in P = 256/G groups of G = 64*ceil(NL/64)
+    // lanes; each group handles a slice of the contraction index and the
+    // partial sums meet in LDS.  S (row a = contraction index of S^T x) is
+    // stored with rows padded to 16 bytes and copied to LDS with 16-byte
+    // loads, all of a thread's loads issued before its first LDS store; the
+    // LDS row stride is odd (NL, or NL+1 for even NL) so that the strided
+    // row reads of S z are conflict-free.  W_k is prefetched into registers
+    // with S.  The
+    // local vectors are node-major ([a][i]) so a node's NT values are one
+    // broadcast read.  Four barriers.
+    // TWO_PASS: the corrections omega/valence * B_c^{-1} r_c go to the
+    // cell-major buffer `cells` (cell_slot layout, NT components) for the
+    // node gather; otherwise they are added to u with atomics.
+    template <int NL>
+    struct V2Shape
+    {
+      // patch sizes N^dim with N <= MAX_N determine dim and N
+      static constexpr int dim = (NL == 8 || NL == 27 || NL == 64 || NL == 125 || NL == 216) ? 3 : 2;
+      static constexpr int n   = dim == 3 ? (NL == 8 ? 2 : NL == 27 ? 3 : NL == 64 ? 4 : NL == 125 ? 5 : 6)
+                                          : (NL == 4 ? 2 : NL == 9 ? 3 : NL == 16 ? 4 : NL == 25 ? 5 : 6);
+      static_assert((dim == 3 ? n * n * n : n * n) == NL, "unsupported patch size");
+    };
     template <int NL>
     constexpr int v2_groups()
     {
       return 256 / (((NL + 63) / 64) * 64);
     }
+    template <int NL>
+    constexpr int v2_ld()
+    {
+      return (NL & 1) ? NL : NL + 1;
+    }
+    // elements of the LDS region of S, rounded to 16 bytes
+    template <typename T, int NL>
+    constexpr int v2_sreg()
+    {
+      constexpr int VW = 16 / int(sizeof(T));
+      return (NL * v2_ld<NL>() + VW - 1) / VW * VW;
+    }
     template <typename T, int NL, int NT>
     constexpr size_t v2_lds()
     {
-      return sizeof(T) * (size_t(NL) * (NL + 1) + 2 * NT * NL + v2_groups<NL>() * NT * NL);
+      return sizeof(T) * (size_t(v2_sreg<T, NL>()) + 2 * NT * NL + v2_groups<NL>() * NT * NL);
     }
 
-    template <typename T, int NL, int NT>
+    template <typename T, int NL, int NT, bool TWO_PASS>
     __device__ __forceinline__ void vanka_v2_cell(const PatchMap &M, const T *__restrict__ S,
                                                   const T *__restrict__ W,
                                                   const T *__restrict__ rhs,
                                                   const T *__restrict__ ax, T *__restrict__ u,
-                                                  T omega, long cell, bool valid, int tid,
-                                                  T *smem)
+                                                  T *__restrict__ cells, T omega, long cell,
+                                                  int tid, T *smem)
     {
       constexpr int G  = ((NL + 63) / 64) * 64;
       constexpr int P  = 256 / G;
-      constexpr int LD = NL + 1;
+      constexpr int LD = v2_ld<NL>();
       constexpr int CH = (NL + P - 1) / P; // contraction slice per group
-      T *const ss  = smem;                 // NL * LD
-      T *const xs  = ss + NL * LD;         // NT * NL
-      T *const ys  = xs + NT * NL;         // NT * NL
-      T *const red = ys + NT * NL;         // P * NT * NL
+      constexpr int VW = 16 / int(sizeof(T));
+      typedef T vec_t __attribute__((ext_vector_type(VW)));
+      T *const xs  = smem + v2_sreg<T, NL>(); // NL x NT
+      T *const ys  = xs + NT * NL;            // NL x NT
+      T *const red = ys + NT * NL;            // P x NT x NL
 
-      int cx = 0, cy = 0, cz = 0;
-      if (valid)
+      int cx, cy, cz;
+      M.cell_xyz(cell, cx, cy, cz);
+      // W_k of this thread's k
+      T wk[NT * NT];
+      if (tid < NL)
         {
-          M.cell_xyz(cell, cx, cy, cz);
-          const T *Sg = S + (size_t)cell * NL * NL;
-          constexpr int VW = 16 / int(sizeof(T));
-          if constexpr ((NL * NL) % VW == 0)
-            {
-              // 16-byte loads: every cell's block starts 16-byte aligned
-              typedef T vec_t __attribute__((ext_vector_type(VW)));
-              const vec_t *Sv = reinterpret_cast<const vec_t *>(Sg);
-              for (int l4 = tid; l4 < NL * NL / VW; l4 += 256)
-                {
-                  const vec_t v = Sv[l4];
+          const T *Wk = W + ((size_t)cell * NL + tid) * NT * NT;
+#pragma unroll
+          for (int e = 0; e < NT * NT; ++e)
+            wk[e] = Wk[e];
+        }
+      // S into LDS: 16-byte loads of the row-padded block (row stride LDG),
+      // all of a thread's loads issued before its first LDS store
+      {
+        constexpr int LDG = (NL + VW - 1) / VW * VW;
+        constexpr int NV  = NL * LDG / VW;
+        constexpr int PER = (NV + 255) / 256;
+        constexpr int RB  = PER < 16 ? PER : 16;
+        const vec_t  *src = reinterpret_cast<const vec_t *>(S + (size_t)cell * NL * LDG);
+        for (int r0 = 0; r0 < PER; r0 += RB)
+          {
+            vec_t v[RB];
+#pragma unroll
+            for (int m = 0; m < RB; ++m)
+              {
+                const int i = (r0 + m) * 256 + tid;
+                if (r0 + m < PER && i < NV)
+                  v[m] = src[i];
+              }
+#pragma unroll
+            for (int m = 0; m < RB; ++m)
+              {
+                const int i = (r0 + m) * 256 + tid;
+                if (r0 + m < PER && i < NV)
 #pragma unroll
                   for (int e = 0; e < VW; ++e)
                     {
-                      const int l = l4 * VW + e, a = l / NL, k = l - a * NL;
-                      ss[a * LD + k] = v[e];
+                      const int l = i * VW + e, a = l / LDG, k = l - a * LDG;
+                      if (k < NL)
+                        smem[a * LD + k] = v[m][e];
                     }
-                }
-            }
-          else
-            for (int l = tid; l < NL * NL; l += 256)
-              {
-                const int a = l / NL, k = l - a * NL;
-                ss[a * LD + k] = Sg[l];
               }
-          for (int l = tid; l < NT * NL; l += 256)
-            {
-              const int  i = l / NL, j = l - i * NL;
-              float      iv;
-              const long g = M.node(cx, cy, cz, j, iv);
-              T          v = rhs[g * NT + i];
-              if (ax)
-                v -= ax[g * NT + i];
-              xs[l] = v;
-            }
+          }
+      }
+      const T *ss = smem;
+      // local residual, node-major
+      for (int l = tid; l < NT * NL; l += 256)
+        {
+          const int  a = l / NT, i = l - a * NT;
+          float      iv;
+          const long gn = M.node(cx, cy, cz, a, iv);
+          T          v  = rhs[gn * NT + i];
+          if (ax)
+            v -= ax[gn * NT + i];
+          xs[l] = v;
         }
       __syncthreads();
       const int part = tid / G, lane = tid - part * G;
       const int a0 = part * CH, a1 = (a0 + CH < NL) ? a0 + CH : NL;
-      // partial y_{i,k} over a in [a0, a1)
-      if (valid && lane < NL)
+      // partial y_{k,i} = sum_{a in slice} S[a][k] x_{a,i}
+      if (lane < NL)
         {
           T y[NT];
 #pragma unroll
           for (int i = 0; i < NT; ++i)
             y[i] = T(0);
+#pragma unroll 4
           for (int a = a0; a < a1; ++a)
             {
               const T s = ss[a * LD + lane];
 #pragma unroll
               for (int i = 0; i < NT; ++i)
-                y[i] += s * xs[i * NL + a];
+                y[i] += s * xs[a * NT + i];
             }
 #pragma unroll
           for (int i = 0; i < NT; ++i)
             red[(part * NT + i) * NL + lane] = y[i];
         }
       __syncthreads();
-      // z_{:,k} = W_k sum_parts y_{:,k}
-      if (valid)
-        for (int k = tid; k < NL; k += 256)
-          {
-            T y[NT];
+      // z_{k,:} = W_k sum_parts y_{k,:}
+      if (tid < NL)
+        {
+          const int k = tid;
+          T         y[NT];
 #pragma unroll
-            for (int i = 0; i < NT; ++i)
-              {
-                T s = T(0);
+          for (int i = 0; i < NT; ++i)
+            {
+              T s = T(0);
 #pragma unroll
-                for (int q = 0; q < P; ++q)
-                  s += red[(q * NT + i) * NL + k];
-                y[i] = s;
-              }
-            const T *Wk = W + ((size_t)cell * NL + k) * NT * NT;
+              for (int q = 0; q < P; ++q)
+                s += red[(q * NT + i) * NL + k];
+              y[i] = s;
+            }
 #pragma unroll
-            for (int i = 0; i < NT; ++i)
-              {
-                T z = T(0);
+          for (int i = 0; i < NT; ++i)
+            {
+              T z = T(0);
 #pragma unroll
-                for (int j = 0; j < NT; ++j)
-                  z += Wk[i * NT + j] * y[j];
-                ys[i * NL + k] = z;
-              }
-          }
+              for (int j = 0; j < NT; ++j)
+                z += wk[i * NT + j] * y[j];
+              ys[k * NT + i] = z;
+            }
+        }
       __syncthreads();
-      // partial out_{i,a} over k in the group's slice
-      if (valid && lane < NL)
+      // partial out_{a,i} = sum_{k in slice} S[a][k] z_{k,i}
+      if (lane < NL)
         {
           T o[NT];
 #pragma unroll
           for (int i = 0; i < NT; ++i)
             o[i] = T(0);
+#pragma unroll 4
           for (int k = a0; k < a1; ++k)
             {
               const T s = ss[lane * LD + k];
 #pragma unroll
               for (int i = 0; i < NT; ++i)
-                o[i] += s * ys[i * NL + k];
+                o[i] += s * ys[k * NT + i];
             }
 #pragma unroll
           for (int i = 0; i < NT; ++i)
             red[(part * NT + i) * NL + lane] = o[i];
         }
       __syncthreads();
-      if (valid)
-        for (int l = tid; l < NT * NL; l += 256)
-          {
-            const int i = l / NL, a = l - i * NL;
-            T         s = T(0);
+      using Sh = V2Shape<NL>;
+      constexpr int n = Sh::n;
+      for (int l = tid; l < NT * NL; l += 256)
+        {
+          const int i = l / NL, a = l - i * NL;
+          T         s = T(0);
 #pragma unroll
-            for (int q = 0; q < P; ++q)
-              s += red[(q * NT + i) * NL + a];
-            float      iv;
-            const long g = M.node(cx, cy, cz, a, iv);
-            atomicAdd(&u[g * NT + i], omega * T(iv) * s);
+          for (int q = 0; q < P; ++q)
+            s += red[(q * NT + i) * NL + a];
+          float      iv;
+          const long gn  = M.node(cx, cy, cz, a, iv);
+          const T    val = omega * T(iv) * s;
+          if constexpr (TWO_PASS)
+            {
+              const int la = a % n, lb = (a / n) % n, lc = (Sh::dim == 3) ? a / (n * n) : 0;
+              cells[cell_slot<Sh::dim, n>(M.nc0, M.nc1, NT, cx, cy, cz, la, lb, lc, i)] = val;
+            }
+          else
+            atomicAdd(&u[gn * NT + i], val);
+        }
+    }
+
+    // Register-tiled factored apply (FP32, NL <= 128), 256 threads per
+    // workgroup.  The threads form a 16 x 16 grid (ta, tk); thread (ta, tk)
+    // keeps S[a][k] for rows a = ta + 16m (m < R) and the 16-byte column
+    // groups g = tk + 16h (h < H) in registers, loaded with 16-byte loads
+    // from the row-padded layout (16 lanes read 256 contiguous bytes of a
+    // row); S takes no LDS, so several cells share a CU.  !PIPE: one cell
+    // per workgroup.  PIPE: a persistent workgroup walks a contiguous range
+    // of cells and loads the next cell's S tile, residual values and W_k
+    // into a second register set while the current cell is processed (twice
+    // the registers, one workgroup per CU).  Partial sums over the 16 tiles of a column
+    // (of a row) meet in LDS laid out [tile][i][index], which makes the
+    // reduction reads consecutive across lanes.  Four barriers per cell.
+    // The corrections go to the cell-major buffer (two-pass form).
+    template <typename T, int NL, int NT>
+    struct V3
+    {
+      static constexpr int    VW  = 16 / int(sizeof(T));
+      static constexpr int    LDG = (NL + VW - 1) / VW * VW;              // global row stride
+      static constexpr int    R   = (NL + 15) / 16;                        // rows per thread
+      static constexpr int    H   = (NL + 16 * VW - 1) / (16 * VW);        // column groups per thread
+      static constexpr int    KP  = 16 * VW * H;                           // padded column count
+      static constexpr int    AP  = 16 * R;                                // padded row count
+      static constexpr int    NX  = (NT * NL + 255) / 256;                 // residual values per thread
+      static constexpr int    RS  = NT * AP + 1; // odd tile stride of the row partials
+      static constexpr size_t lds = sizeof(T) * (size_t(NT * NL) + size_t(NT * KP) +
+                                                 (size_t(16) * NT * KP > size_t(16) * RS ?
+                                                    size_t(16) * NT * KP : size_t(16) * RS));
+    };
+
+    template <typename T, int NL, int NT, bool PIPE>
+    __device__ __forceinline__ void vanka_v3_range(const PatchMap &M, long n_cells,
+                                                   const T *__restrict__ S,
+                                                   const T *__restrict__ W,
+                                                   const T *__restrict__ rhs,
+                                                   const T *__restrict__ ax, T *__restrict__ cells,
+                                                   T omega, int tid, T *smem)
+    {
+      using C = V3<T, NL, NT>;
+      constexpr int VW = C::VW, LDG = C::LDG, R = C::R, H = C::H, KP = C::KP, AP = C::AP,
+                    NX = C::NX, RS = C::RS;
+      static_assert(NL <= 128, "register tiles hold at most 8 rows");
+      typedef T vec_t __attribute__((ext_vector_type(VW)));
+      using Sh = V2Shape<NL>;
+      constexpr int n = Sh::n;
+      T *const xs  = smem;         // [a][i]
+      T *const ys  = xs + NT * NL; // [i][k], row stride KP
+      T *const red = ys + NT * KP; // [16][NT][KP], or [16][RS] for the row partials
+      const int ta = tid >> 4, tk = tid & 15;
+
+      struct Buf
+      {
+        vec_t s[R][H];
+        T     x[NX];
+        T     w[NT * NT];
+      };
+      auto load = [&](Buf &b, long cell) {
+        const T *Sg = S + (size_t)cell * NL * LDG;
+#pragma unroll
+        for (int m = 0; m < R; ++m)
+#pragma unroll
+          for (int h = 0; h < H; ++h)
+            {
+              const int a = ta + 16 * m, g = tk + 16 * h;
+              if (a < NL && VW * g < LDG)
+                b.s[m][h] = *reinterpret_cast<const vec_t *>(Sg + a * LDG + VW * g);
+              else
+                b.s[m][h] = vec_t(T(0));
+            }
+        int cx, cy, cz;
+        M.cell_xyz(cell, cx, cy, cz);
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+          {
+            const int l = tid + 256 * j;
+            T         v = T(0);
+            if (l < NT * NL)
+              {
+                const int  a = l / NT, i = l - a * NT;
+                float      iv;
+                const long gn = M.node(cx, cy, cz, a, iv);
+                v             = rhs[gn * NT + i];
+                if (ax)
+                  v -= ax[gn * NT + i];
+              }
+            b.x[j] = v;
           }
+        if (tid < NL)
+          {
+            const T *Wk = W + ((size_t)cell * NL + tid) * NT * NT;
+#pragma unroll
+            for (int e = 0; e < NT * NT; ++e)
+              b.w[e] = Wk[e];
+          }
+      };
+      auto process = [&](const Buf &b, long cell) {
+#pragma unroll
+        for (int j = 0; j < NX; ++j)
+          {
+            const int l = tid + 256 * j;
+            if (l < NT * NL)
+              xs[l] = b.x[j];
+          }
+        __syncthreads();
+        // partial y[k][i] over this thread's rows
+        {
+          T y[H][VW][NT];
+#pragma unroll
+          for (int h = 0; h < H; ++h)
+#pragma unroll
+            for (int e = 0; e < VW; ++e)
+#pragma unroll
+              for (int i = 0; i < NT; ++i)
+                y[h][e][i] = T(0);
+#pragma unroll
+          for (int m = 0; m < R; ++m)
+            {
+              const int a  = ta + 16 * m;
+              const int ac = a < NL ? a : NL - 1; // rows past NL hold zeros
+              T         xv[NT];
+#pragma unroll
+              for (int i = 0; i < NT; ++i)
+                xv[i] = xs[ac * NT + i];
+#pragma unroll
+              for (int h = 0; h < H; ++h)
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+#pragma unroll
+                  for (int i = 0; i < NT; ++i)
+                    y[h][e][i] += b.s[m][h][e] * xv[i];
+            }
+#pragma unroll
+          for (int h = 0; h < H; ++h)
+#pragma unroll
+            for (int i = 0; i < NT; ++i)
+              {
+                vec_t v;
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                  v[e] = y[h][e][i];
+                *reinterpret_cast<vec_t *>(red + (ta * NT + i) * KP + VW * (tk + 16 * h)) = v;
+              }
+        }
+        __syncthreads();
+        // z_k = W_k sum_ta y_k
+        if (tid < NL)
+          {
+            const int k = tid;
+            T         y[NT];
+#pragma unroll
+            for (int i = 0; i < NT; ++i)
+              {
+                T acc = T(0);
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                  acc += red[(q * NT + i) * KP + k];
+                y[i] = acc;
+              }
+#pragma unroll
+            for (int i = 0; i < NT; ++i)
+              {
+                T z = T(0);
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+                  z += b.w[i * NT + j] * y[j];
+                ys[i * KP + k] = z;
+              }
+          }
+        else if (tid < KP)
+#pragma unroll
+          for (int i = 0; i < NT; ++i)
+            ys[i * KP + tid] = T(0); // padding columns
+        __syncthreads();
+        // partial out[a][i] over this thread's columns
+        {
+          T o[R][NT];
+#pragma unroll
+          for (int m = 0; m < R; ++m)
+#pragma unroll
+            for (int i = 0; i < NT; ++i)
+              o[m][i] = T(0);
+#pragma unroll
+          for (int h = 0; h < H; ++h)
+            {
+              vec_t zv[NT]; // z of this thread's VW columns of group h
+#pragma unroll
+              for (int i = 0; i < NT; ++i)
+                zv[i] = *reinterpret_cast<const vec_t *>(ys + i * KP + VW * (tk + 16 * h));
+#pragma unroll
+              for (int e = 0; e < VW; ++e)
+#pragma unroll
+                for (int m = 0; m < R; ++m)
+#pragma unroll
+                  for (int i = 0; i < NT; ++i)
+                    o[m][i] += b.s[m][h][e] * zv[i][e];
+            }
+#pragma unroll
+          for (int m = 0; m < R; ++m)
+#pragma unroll
+            for (int i = 0; i < NT; ++i)
+              red[tk * RS + i * AP + ta + 16 * m] = o[m][i];
+        }
+        __syncthreads();
+        if (tid < NL)
+          {
+            int cx, cy, cz;
+            M.cell_xyz(cell, cx, cy, cz);
+            const int a = tid;
+            float     iv;
+            (void)M.node(cx, cy, cz, a, iv);
+            const T   sc = omega * T(iv);
+            const int la = a % n, lb = (a / n) % n, lc = (Sh::dim == 3) ? a / (n * n) : 0;
+#pragma unroll
+            for (int i = 0; i < NT; ++i)
+              {
+                T acc = T(0);
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                  acc += red[q * RS + i * AP + a];
+                cells[cell_slot<Sh::dim, n>(M.nc0, M.nc1, NT, cx, cy, cz, la, lb, lc, i)] =
+                  sc * acc;
+              }
+          }
+      };
+
+      if constexpr (!PIPE)
+        {
+          // one cell per workgroup
+          Buf A;
+          load(A, (long)blockIdx.x);
+          process(A, (long)blockIdx.x);
+        }
+      else
+        {
+          const long per = (n_cells + gridDim.x - 1) / gridDim.x;
+          const long c0  = (long)blockIdx.x * per;
+          const long c1  = (c0 + per < n_cells) ? c0 + per : n_cells;
+          if (c0 >= c1)
+            return;
+          Buf A, B;
+          load(A, c0);
+          for (long c = c0; c < c1; c += 2)
+            {
+              load(B, (c + 1 < c1) ? c + 1 : c1 - 1);
+              process(A, c);
+              if (c + 1 >= c1)
+                break;
+              load(A, (c + 2 < c1) ? c + 2 : c1 - 1);
+              process(B, c + 1);
+            }
+        }
     }
   } // namespace
 } // namespace stfem

@@ -29,14 +29,47 @@ namespace stfem
                                               (int)threadIdx.x, lds0, lds1);
     }
 
-    template <int DIM, int N, int NT, typename T>
+    template <int DIM, int N, int NT, typename T, bool SKIP_DIR>
     __global__ void __launch_bounds__(256)
       k_gather_nodes(const GridDesc g, int nto, const T *__restrict__ cells, T *__restrict__ dst,
                      const T *__restrict__ rhs, int mode)
     {
       const long node = (long)blockIdx.x * 256 + threadIdx.x;
       if (node < g.n_nodes)
-        gather_node<DIM, N, NT, T>(g, nto, cells, dst, rhs, mode, node);
+        gather_node<DIM, N, NT, T, SKIP_DIR>(g, nto, cells, dst, rhs, mode, node);
+    }
+
+    template <int DIM, int N, typename T, bool SKIP_DIR>
+    void gather_dn(const GridDesc &g, int nto, const T *cells, T *dst, const T *rhs, int mode,
+                   hipStream_t s)
+    {
+      const dim3 grid{(unsigned)((g.n_nodes + 255) / 256), 1, 1};
+      switch (nto)
+        {
+          case 1: k_gather_nodes<DIM, N, 1, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, 1, cells, dst, rhs, mode); break;
+          case 2: k_gather_nodes<DIM, N, 2, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, 2, cells, dst, rhs, mode); break;
+          case 3: k_gather_nodes<DIM, N, 3, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, 3, cells, dst, rhs, mode); break;
+          case 4: k_gather_nodes<DIM, N, 4, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, 4, cells, dst, rhs, mode); break;
+          default:
+            k_gather_nodes<DIM, N, 0, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, nto, cells, dst, rhs, mode);
+            break;
+        }
+      STFEM_HIP(hipGetLastError());
+    }
+    template <typename T, bool SKIP_DIR>
+    void gather_any(const GridDesc &g, int nto, const T *cells, T *dst, const T *rhs, int mode,
+                    hipStream_t s)
+    {
+      const int N = g.p + 1;
+#define STFEM_GATHER_CASE(D, NN)                                                                   \
+  if (g.dim == D && N == NN)                                                                       \
+    return gather_dn<D, NN, T, SKIP_DIR>(g, nto, cells, dst, rhs, mode, s);
+      STFEM_GATHER_CASE(2, 2) STFEM_GATHER_CASE(2, 3) STFEM_GATHER_CASE(2, 4)
+      STFEM_GATHER_CASE(2, 5) STFEM_GATHER_CASE(2, 6) STFEM_GATHER_CASE(3, 2)
+      STFEM_GATHER_CASE(3, 3) STFEM_GATHER_CASE(3, 4) STFEM_GATHER_CASE(3, 5)
+      STFEM_GATHER_CASE(3, 6)
+#undef STFEM_GATHER_CASE
+      throw Error(STATUS_ERR_UNSUPPORTED, "gather: unsupported dim/degree");
     }
 
     template <int DIM, int N, typename T, bool TO_CELLS>
@@ -96,18 +129,7 @@ namespace stfem
           return dispatch_jb<DIM, N, T, false>(a, src, dst, s);
         }
       dispatch_jb<DIM, N, T, true>(a, src, cells, s);
-      const dim3 grid{(unsigned)((a.g.n_nodes + 255) / 256), 1, 1};
-      switch (a.nt_out)
-        {
-          case 1: k_gather_nodes<DIM, N, 1, T><<<grid, 256, 0, s>>>(a.g, 1, cells, dst, rhs, mode); break;
-          case 2: k_gather_nodes<DIM, N, 2, T><<<grid, 256, 0, s>>>(a.g, 2, cells, dst, rhs, mode); break;
-          case 3: k_gather_nodes<DIM, N, 3, T><<<grid, 256, 0, s>>>(a.g, 3, cells, dst, rhs, mode); break;
-          case 4: k_gather_nodes<DIM, N, 4, T><<<grid, 256, 0, s>>>(a.g, 4, cells, dst, rhs, mode); break;
-          default:
-            k_gather_nodes<DIM, N, 0, T><<<grid, 256, 0, s>>>(a.g, a.nt_out, cells, dst, rhs, mode);
-            break;
-        }
-      STFEM_HIP(hipGetLastError());
+      gather_dn<DIM, N, T, true>(a.g, a.nt_out, cells, dst, rhs, mode, s);
     }
   } // namespace
 
@@ -141,6 +163,24 @@ namespace stfem
         }
     throw Error(STATUS_ERR_UNSUPPORTED, "st_apply: unsupported dim/degree");
   }
+
+  template <typename T>
+  void launch_gather_cells(const GridDesc &g, int nto, const T *cells, T *dst, const T *rhs,
+                           int mode, bool skip_dirichlet, hipStream_t s)
+  {
+    STFEM_REQUIRE(nto >= 1 && mode >= 0 && mode <= 2, "gather: bad arguments");
+    STFEM_REQUIRE(mode != 2 || rhs, "gather: mode 2 needs rhs");
+    if (g.n_nodes == 0)
+      return;
+    if (skip_dirichlet)
+      gather_any<T, true>(g, nto, cells, dst, rhs, mode, s);
+    else
+      gather_any<T, false>(g, nto, cells, dst, rhs, mode, s);
+  }
+  template void launch_gather_cells<double>(const GridDesc &, int, const double *, double *,
+                                            const double *, int, bool, hipStream_t);
+  template void launch_gather_cells<float>(const GridDesc &, int, const float *, float *,
+                                           const float *, int, bool, hipStream_t);
 
   template void launch_st_apply<double>(const StOpArgs<double> &, const double *, double *,
                                         double *, int, const double *, hipStream_t);

@@ -227,7 +227,7 @@ namespace stfem
     // chunk; Z: column-major eigenvectors, lam: eigenvalues; one thread per
     // (cell, k).
     template <typename T>
-    __global__ void k_pack_factors(long ncell, int nl, int nt, const double *__restrict__ Z,
+    __global__ void k_pack_factors(long ncell, int nl, int ld, int nt, const double *__restrict__ Z,
                                    const double *__restrict__ lam, const double *__restrict__ Alpha,
                                    const double *__restrict__ Beta, T *__restrict__ S,
                                    T *__restrict__ W, int *__restrict__ singular)
@@ -238,9 +238,13 @@ namespace stfem
       const long c = id / nl;
       const int  k = int(id - c * nl);
       const double *Zc = Z + size_t(c) * nl * nl;
-      T            *Sc = S + size_t(c) * nl * nl;
+      T            *Sc = S + size_t(c) * nl * ld;
       for (int i = 0; i < nl; ++i)
-        Sc[size_t(i) * nl + k] = T(Zc[size_t(k) * nl + i]);
+        Sc[size_t(i) * ld + k] = T(Zc[size_t(k) * nl + i]);
+      if (k == 0) // row padding
+        for (int i = 0; i < nl; ++i)
+          for (int kk = nl; kk < ld; ++kk)
+            Sc[size_t(i) * ld + kk] = T(0);
       double       A[8][8], I[8][8];
       const double l = lam[size_t(c) * nl + k];
       for (int i = 0; i < nt; ++i)
@@ -380,9 +384,8 @@ namespace stfem
     const long n = ncell * nl;
     if (n == 0)
       return;
-    k_pack_factors<T><<<dim3((unsigned)((n + 127) / 128)), 128, 0, s>>>(ncell, nl, nt, Z, lam,
-                                                                       Alpha, Beta, S, W,
-                                                                       singular);
+    k_pack_factors<T><<<dim3((unsigned)((n + 127) / 128)), 128, 0, s>>>(
+      ncell, nl, vanka_s_ld(nl, sizeof(T)), nt, Z, lam, Alpha, Beta, S, W, singular);
     STFEM_HIP(hipGetLastError());
   }
   template void launch_pack_factors<double>(long, int, int, const double *, const double *,

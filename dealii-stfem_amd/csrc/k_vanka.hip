@@ -13,6 +13,7 @@
 // dev_vanka.h.
 #include "dev_vanka.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace stfem
@@ -33,25 +34,37 @@ namespace stfem
 
     template <typename T, bool STAGE>
     __global__ void __launch_bounds__(256)
-      k_vanka_factored(const PatchMap M, int nt, int nloc, const T *__restrict__ S,
+      k_vanka_factored(const PatchMap M, int nt, int nloc, int ld, const T *__restrict__ S,
                        const T *__restrict__ W, const T *__restrict__ rhs,
                        const T *__restrict__ ax, T *__restrict__ u, T omega)
     {
       extern __shared__ unsigned char smem_raw[];
-      vanka_factored_cell<T, STAGE>(M, nt, nloc, S, W, rhs, ax, u, omega, (long)blockIdx.x,
+      vanka_factored_cell<T, STAGE>(M, nt, nloc, ld, S, W, rhs, ax, u, omega, (long)blockIdx.x,
                                     (int)threadIdx.x, (int)blockDim.x,
                                     reinterpret_cast<T *>(smem_raw));
     }
 
-    template <typename T, int NL, int NT>
+    template <typename T, int NL, int NT, bool TWO_PASS>
     __global__ void __launch_bounds__(256)
       k_vanka_factored_v2(const PatchMap M, const T *__restrict__ S, const T *__restrict__ W,
                           const T *__restrict__ rhs, const T *__restrict__ ax,
-                          T *__restrict__ u, T omega)
+                          T *__restrict__ u, T *__restrict__ cells, T omega)
     {
-      extern __shared__ unsigned char smem_raw[];
-      vanka_v2_cell<T, NL, NT>(M, S, W, rhs, ax, u, omega, (long)blockIdx.x, true,
-                               (int)threadIdx.x, reinterpret_cast<T *>(smem_raw));
+      extern __shared__ __align__(16) unsigned char smem_raw[];
+      vanka_v2_cell<T, NL, NT, TWO_PASS>(M, S, W, rhs, ax, u, cells, omega, (long)blockIdx.x,
+                                         (int)threadIdx.x, reinterpret_cast<T *>(smem_raw));
+    }
+
+    // !PIPE: at most 128 VGPRs, so that four workgroups share a CU
+    template <typename T, int NL, int NT, bool PIPE>
+    __global__ void __launch_bounds__(256, PIPE ? 1 : 4)
+      k_vanka_factored_v3(const PatchMap M, long n_cells, const T *__restrict__ S,
+                          const T *__restrict__ W, const T *__restrict__ rhs,
+                          const T *__restrict__ ax, T *__restrict__ cells, T omega)
+    {
+      extern __shared__ __align__(16) unsigned char smem_raw[];
+      vanka_v3_range<T, NL, NT, PIPE>(M, n_cells, S, W, rhs, ax, cells, omega, (int)threadIdx.x,
+                                      reinterpret_cast<T *>(smem_raw));
     }
 
     size_t max_lds_per_block()
@@ -86,10 +99,12 @@ namespace stfem
 
   template <typename T>
   void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *W, const T *rhs,
-                             const T *ax, T *u, T omega, hipStream_t s)
+                             const T *ax, T *u, T omega, T *cells, int mode, int variant,
+                             hipStream_t s)
   {
     if (g.n_cells == 0)
       return;
+    STFEM_REQUIRE(mode == 0 || mode == 1, "vanka factored: mode must be 0 or 1");
     STFEM_REQUIRE(nt <= 16, "vanka factored: at most 16 time dofs per patch");
     const int    nloc = patch_size(g);
     const int    thr  = nloc >= 256 ? 256 : ((nloc + 63) / 64) * 64;
@@ -98,18 +113,68 @@ namespace stfem
     const PatchMap M  = make_map(g);
     // compile-time patch shapes first
     bool done = false;
+    auto try_v3 = [&](auto nl_c, auto nt_c) {
+      constexpr int NL = decltype(nl_c)::value, NT = decltype(nt_c)::value;
+      if constexpr (NL <= 128 && sizeof(T) == 4 && V3<T, NL, NT>::lds <= 64 * 1024)
+        {
+          if (done || nloc != NL || nt != NT || !cells)
+            return;
+          const bool use = variant == 3 || variant == 4 || (variant == 0 && NL > 64);
+          if (!use)
+            return;
+          constexpr size_t lds = V3<T, NL, NT>::lds;
+          if (variant == 4)
+            {
+              static int resident = 0;
+              if (!resident)
+                {
+                  int dev = 0, cus = 0, per = 0;
+                  STFEM_HIP(hipGetDevice(&dev));
+                  STFEM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+                  STFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per, (const void *)k_vanka_factored_v3<T, NL, NT, true>, 256, lds));
+                  resident = std::max(1, cus * std::max(1, per));
+                }
+              const long grid = std::min<long>(g.n_cells, resident);
+              k_vanka_factored_v3<T, NL, NT, true><<<dim3((unsigned)grid), dim3(256), lds, s>>>(
+                M, g.n_cells, S, W, rhs, ax, cells, omega);
+            }
+          else
+            k_vanka_factored_v3<T, NL, NT, false><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
+              M, g.n_cells, S, W, rhs, ax, cells, omega);
+          STFEM_HIP(hipGetLastError());
+          launch_gather_cells<T>(g, NT, cells, u, nullptr, mode, false, s);
+          done = true;
+        }
+    };
     auto try_v2 = [&](auto nl_c, auto nt_c) {
       constexpr int NL = decltype(nl_c)::value, NT = decltype(nt_c)::value;
+      try_v3(nl_c, nt_c);
       if (done || nloc != NL || nt != NT)
         return;
       const size_t lds = v2_lds<T, NL, NT>();
       if (lds > max_lds_per_block())
         return;
-      if (lds > 64 * 1024)
-        STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v2<T, NL, NT>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-      k_vanka_factored_v2<T, NL, NT><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
-        M, S, W, rhs, ax, u, omega);
+      if (cells)
+        {
+          if (lds > 64 * 1024)
+            STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v2<T, NL, NT, true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+          k_vanka_factored_v2<T, NL, NT, true><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
+            M, S, W, rhs, ax, u, cells, omega);
+          STFEM_HIP(hipGetLastError());
+          launch_gather_cells<T>(g, NT, cells, u, nullptr, mode, false, s);
+        }
+      else
+        {
+          if (mode == 0)
+            launch_fill_zero<T>(u, size_t(g.n_nodes) * NT, s);
+          if (lds > 64 * 1024)
+            STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v2<T, NL, NT, false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+          k_vanka_factored_v2<T, NL, NT, false><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
+            M, S, W, rhs, ax, u, cells, omega);
+        }
       done = true;
     };
     auto for_nt = [&](auto nl_c) {
@@ -120,6 +185,8 @@ namespace stfem
       try_v2(nl_c, std::integral_constant<int, 6>{});
       try_v2(nl_c, std::integral_constant<int, 8>{});
     };
+    for_nt(std::integral_constant<int, 4>{});
+    for_nt(std::integral_constant<int, 8>{});
     for_nt(std::integral_constant<int, 9>{});
     for_nt(std::integral_constant<int, 16>{});
     for_nt(std::integral_constant<int, 25>{});
@@ -132,17 +199,19 @@ namespace stfem
         STFEM_HIP(hipGetLastError());
         return;
       }
+    if (mode == 0)
+      launch_fill_zero<T>(u, size_t(g.n_nodes) * nt, s);
     if (full <= max_lds_per_block())
       {
         if (full > 64 * 1024)
           STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored<T, true>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(full)));
         k_vanka_factored<T, true><<<dim3((unsigned)g.n_cells), dim3((unsigned)thr), full, s>>>(
-          M, nt, nloc, S, W, rhs, ax, u, omega);
+          M, nt, nloc, vanka_s_ld(nloc, sizeof(T)), S, W, rhs, ax, u, omega);
       }
     else
       k_vanka_factored<T, false><<<dim3((unsigned)g.n_cells), dim3((unsigned)thr), base, s>>>(
-        M, nt, nloc, S, W, rhs, ax, u, omega);
+        M, nt, nloc, vanka_s_ld(nloc, sizeof(T)), S, W, rhs, ax, u, omega);
     STFEM_HIP(hipGetLastError());
   }
 
@@ -153,8 +222,8 @@ namespace stfem
                                              const float *, float *, float, hipStream_t);
   template void launch_vanka_factored<double>(const GridDesc &, int, const double *,
                                               const double *, const double *, const double *,
-                                              double *, double, hipStream_t);
+                                              double *, double, double *, int, int, hipStream_t);
   template void launch_vanka_factored<float>(const GridDesc &, int, const float *, const float *,
-                                             const float *, const float *, float *, float,
-                                             hipStream_t);
+                                             const float *, const float *, float *, float, float *,
+                                             int, int, hipStream_t);
 } // namespace stfem

@@ -58,10 +58,31 @@ namespace stfem
   void launch_vanka_explicit(const GridDesc &g, int nt, const T *binv, const T *rhs, const T *ax,
                              T *u, T omega, hipStream_t s);
   // Factored form: Binv_c = (I x S_c) blockdiag_k (W_{c,k}) (I x S_c^T),
-  // S_c (n_c x n_c, row-major), W_{c,k} (nt x nt row-major) per mode k.
+  // S_c (n_c x n_c, row-major with the row stride vanka_s_ld(n_c, sizeof(T)),
+  // padding zero), W_{c,k} (nt x nt row-major) per mode k.
+  // cells (n_cells * n_c * nt values) non-null: two-pass form for the
+  // compile-time patch shapes, mode 0 u = correction, 1 u += correction;
+  // otherwise (and for the other shapes) u += correction with atomics
+  // (mode 0 then zero-fills u first).  variant: 0 automatic, 2 the
+  // LDS-staged kernel, 3 the register-tiled kernel (FP32, patches of at
+  // most 128 nodes), 4 its software-pipelined persistent form; shapes a
+  // variant does not cover fall back to the others.
+  // row stride of the factored Vanka's S blocks: rows padded to 16 bytes
+  inline int vanka_s_ld(int nloc, size_t tsize)
+  {
+    const int vw = int(16 / tsize);
+    return (nloc + vw - 1) / vw * vw;
+  }
   template <typename T>
   void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *W, const T *rhs,
-                             const T *ax, T *u, T omega, hipStream_t s);
+                             const T *ax, T *u, T omega, T *cells, int mode, int variant,
+                             hipStream_t s);
+  // Node gather of a cell-major buffer (cell_slot layout, nto components):
+  // mode 0 dst = sum, 1 dst += sum, 2 dst = rhs - sum; skip_dirichlet:
+  // Dirichlet nodes are left out (zero for modes 0, 2, unchanged for 1).
+  template <typename T>
+  void launch_gather_cells(const GridDesc &g, int nto, const T *cells, T *dst, const T *rhs,
+                           int mode, bool skip_dirichlet, hipStream_t s);
   // p[0..n) = 0 with a kernel (safe to record into a HIP graph)
   template <typename T>
   void launch_fill_zero(T *p, size_t n, hipStream_t s);
@@ -128,8 +149,8 @@ namespace stfem
   // [e0, e0 + ne), which must contain every neighbour of the patches.
   void launch_patch_matrices(const GridDesc &g, long p0, long np, long e0, const double *Me,
                              const double *Ke, double *Mp, double *Kp, hipStream_t s);
-  // S_c (row-major, eigenvector k = column k) from the column-major
-  // eigenvectors Z, and W_{c,k} = (lam_k Alpha + Beta)^{-1} (nt <= 8); sets
+  // S_c (row-major with the row stride vanka_s_ld, eigenvector k = column k)
+  // from the column-major eigenvectors Z, and W_{c,k} = (lam_k Alpha + Beta)^{-1} (nt <= 8); sets
   // *singular when a time block cannot be inverted.
   template <typename T>
   void launch_pack_factors(long ncell, int nl, int nt, const double *Z, const double *lam,

@@ -102,8 +102,9 @@ namespace stfem
   std::vector<MeshLevel> build_mesh_levels(int dim, const int subdiv[3], int refine, int p,
                                            const double *fine_vertices, const double *lo,
                                            const double *hi, const double *fine_kappa,
-                                           int dirichlet_mask)
+                                           int dirichlet_mask, int coarse_coef)
   {
+    STFEM_REQUIRE(coarse_coef >= 0 && coarse_coef <= 3, "mesh: coarse_coef must be in [0, 3]");
     STFEM_REQUIRE(dim == 2 || dim == 3, "mesh: dim must be 2 or 3");
     STFEM_REQUIRE(p >= 1 && p < MAX_N, "mesh: space degree must be in [1, 5]");
     STFEM_REQUIRE(refine >= 0 && refine < 12, "mesh: refinement out of range");
@@ -168,10 +169,30 @@ namespace stfem
           }
         else
           {
-            // value of the finest cell containing each quadrature point
             m.coef_mode     = 2;
             const long nqc  = (dim == 3) ? nq * nq * nq : nq * nq;
             m.coef.resize(size_t(m.g.n_cells * nqc));
+            // per direction: the fine sub-cells met by each point's share
+            // of [0,1] and their overlap weights
+            std::vector<std::vector<std::pair<int, double>>> share(nq);
+            for (int q = 0; q < nq; ++q)
+              {
+                if (coarse_coef == 0)
+                  {
+                    int sub = int(std::floor(sh.qx[q] * fac));
+                    share[q].push_back({std::min(std::max(sub, 0), fac - 1), 1.0});
+                    continue;
+                  }
+                const double lo_q = q == 0 ? 0.0 : 0.5 * (sh.qx[q - 1] + sh.qx[q]);
+                const double hi_q = q == nq - 1 ? 1.0 : 0.5 * (sh.qx[q] + sh.qx[q + 1]);
+                for (int sub = 0; sub < fac; ++sub)
+                  {
+                    const double ov = std::min(hi_q, double(sub + 1) / fac) -
+                                      std::max(lo_q, double(sub) / fac);
+                    if (ov > 0)
+                      share[q].push_back({sub, ov});
+                  }
+              }
             for (long c = 0; c < m.g.n_cells; ++c)
               {
                 const int cx = int(c % m.g.nc[0]);
@@ -179,16 +200,26 @@ namespace stfem
                 const int cz = (dim == 3) ? int(c / ((long)m.g.nc[0] * m.g.nc[1])) : 0;
                 for (long q = 0; q < nqc; ++q)
                   {
-                    const int qi[3] = {int(q % nq), int((q / nq) % nq), int(q / (nq * nq))};
-                    long      f[3] = {cx, cy, cz};
-                    for (int d = 0; d < dim; ++d)
-                      {
-                        int sub = int(std::floor(sh.qx[qi[d]] * fac));
-                        sub     = std::min(std::max(sub, 0), fac - 1);
-                        f[d]    = f[d] * fac + sub;
-                      }
-                    const long fid = f[0] + (long)fnc[0] * (f[1] + (long)fnc[1] * f[2]);
-                    m.coef[c * nqc + q] = fine_kappa[fid];
+                    const int qi[3] = {int(q % nq), int((q / nq) % nq),
+                                       dim == 3 ? int(q / (nq * nq)) : 0};
+                    const auto &sz = dim == 3 ? share[qi[2]] : share[0];
+                    double      acc = 0.0, wsum = 0.0;
+                    for (size_t kz = 0; kz < (dim == 3 ? sz.size() : 1); ++kz)
+                      for (const auto &ey : share[qi[1]])
+                        for (const auto &ex : share[qi[0]])
+                          {
+                            const long f0 = long(cx) * fac + ex.first;
+                            const long f1 = long(cy) * fac + ey.first;
+                            const long f2 = dim == 3 ? long(cz) * fac + sz[kz].first : 0;
+                            const double w  = ex.second * ey.second * (dim == 3 ? sz[kz].second : 1.0);
+                            const double kv = fine_kappa[f0 + (long)fnc[0] * (f1 + (long)fnc[1] * f2)];
+                            acc += w * (coarse_coef == 1 ? std::log(kv) :
+                                        coarse_coef == 2 ? 1.0 / kv : kv);
+                            wsum += w;
+                          }
+                    acc /= wsum;
+                    m.coef[c * nqc + q] = coarse_coef <= 1 ? (coarse_coef == 0 ? acc : std::exp(acc)) :
+                                          coarse_coef == 2 ? 1.0 / acc : acc;
                   }
               }
           }
@@ -747,14 +778,17 @@ namespace stfem
     else
       {
         STFEM_REQUIRE(nt <= 16, "Vanka factored form supports at most 16 time dofs");
-        S_.alloc(size_t(nc) * bs);
+        const int    ld  = vanka_s_ld(nl, sizeof(T)); // row-padded S blocks
+        const size_t bsp = size_t(nl) * ld;
+        S_.alloc(size_t(nc) * bsp);
         W_.alloc(size_t(nc) * nl * nt * nt);
+        cells_.alloc(size_t(nc) * nl * nt);
         const long     ch = std::max<long>(1, std::min<long>(nc, 4096));
         std::vector<T> sS, sW;
         for (long c0 = 0; c0 < nc; c0 += ch)
           {
             const long c1 = std::min(nc, c0 + ch);
-            sS.resize(size_t(c1 - c0) * bs);
+            sS.assign(size_t(c1 - c0) * bsp, T(0));
             sW.resize(size_t(c1 - c0) * nl * nt * nt);
             bool bad = false;
 #pragma omp parallel
@@ -818,7 +852,7 @@ namespace stfem
                       }
                   sym_eigen(C, nl, lam); // C now holds the eigenvectors Q (columns)
                   // S = L^{-T} Q
-                  T *Sd = sS.data() + size_t(c - c0) * bs;
+                  T *Sd = sS.data() + size_t(c - c0) * bsp;
                   for (int col = 0; col < nl; ++col)
                     for (int i = nl - 1; i >= 0; --i)
                       {
@@ -830,7 +864,7 @@ namespace stfem
                       }
                   for (int i = 0; i < nl; ++i)
                     for (int col = 0; col < nl; ++col)
-                      Sd[size_t(i) * nl + col] = T(Y[size_t(i) * nl + col]);
+                      Sd[size_t(i) * ld + col] = T(Y[size_t(i) * nl + col]);
                   // W_k = (lambda_k Alpha + Beta)^{-1}
                   T *Wd = sW.data() + size_t(c - c0) * nl * nt * nt;
                   std::vector<double> Bk(size_t(nt) * nt), wk;
@@ -848,7 +882,7 @@ namespace stfem
             }
             if (bad)
               throw Error(STATUS_ERR_INTERNAL, "Vanka factored: patch not positive definite");
-            STFEM_HIP(hipMemcpy(S_.get() + size_t(c0) * bs, sS.data(), sS.size() * sizeof(T),
+            STFEM_HIP(hipMemcpy(S_.get() + size_t(c0) * bsp, sS.data(), sS.size() * sizeof(T),
                                 hipMemcpyHostToDevice));
             STFEM_HIP(hipMemcpy(W_.get() + size_t(c0) * nl * nt * nt, sW.data(),
                                 sW.size() * sizeof(T), hipMemcpyHostToDevice));
@@ -872,8 +906,10 @@ namespace stfem
     const long      nc = g.n_cells;
     const size_t    bs = size_t(nl) * nl;
     hipStream_t     s  = ctx_.stream;
-    S_.alloc(size_t(nc) * bs);
+    const size_t bsp = size_t(nl) * vanka_s_ld(nl, sizeof(T)); // row-padded S block
+    S_.alloc(size_t(nc) * bsp);
     W_.alloc(size_t(nc) * nl * nt * nt);
+    cells_.alloc(size_t(nc) * nl * nt);
     DBuf<double> X, coef, dA, dB;
     X.upload(m.X.data(), m.X.size(), s);
     if (m.coef_mode != 0)
@@ -917,7 +953,7 @@ namespace stfem
           throw Error(STATUS_ERR_INTERNAL,
                       std::string("rocSOLVER sygvj: ") + rocblas_status_to_string(st));
         launch_pack_factors<T>(np, nl, nt, Kp.get(), lam.get(), dA.get(), dB.get(),
-                               S_.get() + size_t(p0) * bs, W_.get() + size_t(p0) * nl * nt * nt,
+                               S_.get() + size_t(p0) * bsp, W_.get() + size_t(p0) * nl * nt * nt,
                                flag.get(), s);
         STFEM_HIP(hipMemcpyAsync(hinfo.data(), info.get(), size_t(np) * sizeof(int),
                                  hipMemcpyDeviceToHost, s));
@@ -936,6 +972,9 @@ namespace stfem
   template <typename T>
   void PreconditionVanka<T>::vmult(T *dst, const T *src) const
   {
+    if (form_ == Form::factored)
+      return launch_vanka_factored<T>(grid_, nt_, S_.get(), W_.get(), src, nullptr, dst, T(1),
+                                      cells_.get(), 0, variant_, ctx_.stream);
     launch_fill_zero<T>(dst, size_t(grid_.n_nodes) * nt_, ctx_.stream);
     relax(dst, src, nullptr, T(1));
   }
@@ -945,7 +984,8 @@ namespace stfem
     if (form_ == Form::explicit_inverse)
       launch_vanka_explicit<T>(grid_, nt_, binv_.get(), rhs, ax, u, omega, ctx_.stream);
     else
-      launch_vanka_factored<T>(grid_, nt_, S_.get(), W_.get(), rhs, ax, u, omega, ctx_.stream);
+      launch_vanka_factored<T>(grid_, nt_, S_.get(), W_.get(), rhs, ax, u, omega, cells_.get(), 1,
+                               variant_, ctx_.stream);
   }
   template <typename T>
   double PreconditionVanka<T>::sweep_bytes() const
@@ -1679,9 +1719,11 @@ namespace stfem
     geti("mg_float", ps.mg_float);
     geti("dirichlet", ps.dirichlet);
     geti("atomic_scatter", ps.atomic_scatter);
+    geti("coarse_coef", ps.coarse_coef);
     if (kv.count("problem"))
       ps.problem = (kv["problem"] == "wave" || kv["problem"] == "1") ? 1 : 0;
     geti("vanka_device_setup", ps.vanka_device_setup);
+    geti("vanka_kernel", ps.vanka_kernel);
     geti("smoothing_steps", ps.mg.smoothing_steps);
     getb("variable", ps.mg.variable);
     getb("estimate_relaxation", ps.mg.estimate_relaxation);
@@ -1736,7 +1778,7 @@ namespace stfem
         spec = s;
         PhaseLog plog;
         mesh_ = build_mesh_levels(spec.dim, spec.subdiv, spec.refine, spec.p, vertices, spec.lo,
-                                  spec.hi, kappa, spec.dirichlet);
+                                  spec.hi, kappa, spec.dirichlet, spec.coarse_coef);
         plog("mesh levels");
         // time step (tp_01.cc:87, 124-127) unless given
         if (spec.tau > 0)
@@ -1795,6 +1837,7 @@ namespace stfem
                                                              spec.atomic_scatter == 0);
             levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form,
                                                                   spec.vanka_device_setup != 0);
+            levels[l].V->set_variant(spec.vanka_kernel);
             levels[l].r  = rl[l];
             levels[l].nts = nl[l];
             levels[l].type_to_here = l > 0 ? seq[l - 1] : TimeMGType::none;

@@ -125,11 +125,16 @@ namespace stfem
   // level s has subdiv * 2^s cells per direction and takes the finest
   // level's vertex positions (coarse vertices are a subset).  A per-cell
   // coefficient of the finest level is evaluated on coarser levels at the
-  // quadrature points (value of the finest cell containing the point).
+  // quadrature points: coarse_coef 0 takes the value of the finest cell
+  // containing the point (a coefficient function sampled at the points, as
+  // MatrixFreeOperator::evaluate_coefficient does); 1, 2, 3 take the
+  // geometric, harmonic or arithmetic mean over the finest cells that meet
+  // the point's share of the coarse cell (per direction the interval between
+  // the midpoints of neighbouring Gauss points), weighted by overlap.
   std::vector<MeshLevel> build_mesh_levels(int dim, const int subdiv[3], int refine, int p,
                                            const double *fine_vertices, const double *lo,
                                            const double *hi, const double *fine_kappa,
-                                           int dirichlet_mask);
+                                           int dirichlet_mask, int coarse_coef = 0);
 
   // Element mass / stiffness matrices of all cells (lexicographic local
   // numbering, n_c x n_c each), and their assembly into the principal
@@ -203,6 +208,8 @@ namespace stfem
     void   relax(T *u, const T *rhs, const T *ax, T omega) const;
     long   n_cells() const { return grid_.n_cells; }
     Form   form() const { return form_; }
+    // factored sweep kernel (0 automatic, 2 LDS-staged, 3 register-tiled)
+    void   set_variant(int v) { variant_ = v; }
     double sweep_bytes() const;
     size_t storage_bytes() const { return (binv_.size() + S_.size() + W_.size()) * sizeof(T); }
 
@@ -214,6 +221,8 @@ namespace stfem
     int      nt_;
     Form     form_;
     DBuf<T>  binv_, S_, W_;
+    mutable DBuf<T> cells_; // cell-major corrections of the two-pass sweep
+    int             variant_ = 0;
   };
 
   // ------------------------------------------------------ transfers
@@ -417,7 +426,9 @@ namespace stfem
     int          dirichlet = 63;
     int          atomic_scatter = 0; // 1: one-pass operator with atomic adds
     int          problem = 0; // 0 heat, 1 wave (first order in time, tp_01.cc:167-197)
+    int          coarse_coef = 0; // coarse-level coefficient: 0 sampled, 1 geometric, 2 harmonic, 3 arithmetic mean
     int          vanka_device_setup = 1; // factored Vanka set up on the GPU
+    int          vanka_kernel = 0; // factored sweep kernel: 0 automatic, 2 LDS-staged, 3 register-tiled
     GMGOptions   mg;
     SolverControlData solver;
     int          extrapolate = 1;
