@@ -55,9 +55,9 @@ namespace stfem
                                          (int)threadIdx.x, reinterpret_cast<T *>(smem_raw));
     }
 
-    // !PIPE: at most 128 VGPRs, so that four workgroups share a CU
+    // !PIPE: register budget for four (FP32) or two (FP64) workgroups per CU
     template <typename T, int NL, int NT, bool PIPE>
-    __global__ void __launch_bounds__(256, PIPE ? 1 : 4)
+    __global__ void __launch_bounds__(256, PIPE ? 1 : (sizeof(T) == 4 ? 4 : 2))
       k_vanka_factored_v3(const PatchMap M, long n_cells, const T *__restrict__ S,
                           const T *__restrict__ W, const T *__restrict__ rhs,
                           const T *__restrict__ ax, T *__restrict__ cells, T omega)
@@ -115,14 +115,20 @@ namespace stfem
     bool done = false;
     auto try_v3 = [&](auto nl_c, auto nt_c) {
       constexpr int NL = decltype(nl_c)::value, NT = decltype(nt_c)::value;
-      if constexpr (NL <= 128 && sizeof(T) == 4 && V3<T, NL, NT>::lds <= 64 * 1024)
+      if constexpr (NL <= 128 && V3<T, NL, NT>::lds <= 80 * 1024)
         {
           if (done || nloc != NL || nt != NT || !cells)
             return;
-          const bool use = variant == 3 || variant == 4 || (variant == 0 && NL > 64);
+          const bool use = variant == 3 || (variant == 4 && sizeof(T) == 4) ||
+                           (variant == 0 && NL > 64);
           if (!use)
             return;
           constexpr size_t lds = V3<T, NL, NT>::lds;
+          if (lds > max_lds_per_block())
+            return;
+          if (lds > 64 * 1024)
+            STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v3<T, NL, NT, false>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
           if (variant == 4)
             {
               static int resident = 0;

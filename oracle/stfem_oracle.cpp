@@ -706,6 +706,7 @@ namespace orc
   {
     int                 dim = 2;
     int                 nc[3] = {1, 1, 1};
+    int                 dirichlet = 63; // faces with zero Dirichlet: bit 2d low, 2d+1 high side
     std::vector<double> X;
     long n_cells() const { return (long)nc[0] * nc[1] * (dim == 3 ? nc[2] : 1); }
     long nvx(int d) const { return nc[d] + 1; }
@@ -763,9 +764,11 @@ namespace orc
         for (long j = 0; j < N[1]; ++j)
           for (long i = 0; i < N[0]; ++i)
             {
-              bool b = (i == 0 || i == N[0] - 1 || j == 0 || j == N[1] - 1);
+              const int m = g.dirichlet;
+              bool b = (i == 0 && (m & 1)) || (i == N[0] - 1 && (m & 2)) ||
+                       (j == 0 && (m & 4)) || (j == N[1] - 1 && (m & 8));
               if (dim == 3)
-                b = b || k == 0 || k == N[2] - 1;
+                b = b || (k == 0 && (m & 16)) || (k == N[2] - 1 && (m & 32));
               constrained[i + N[0] * (j + N[1] * k)] = b;
             }
     }
@@ -867,18 +870,29 @@ namespace orc
   }
 
   // Coefficient: per finest-level cell value (or constant 1 when absent).
-  // On a coarser level the coefficient at a quadrature point is the value of
-  // the finest cell containing that point in reference coordinates.
+  // On a coarser level the coefficient at a quadrature point is, for mode 0,
+  // the value of the finest cell containing that point in reference
+  // coordinates (a coefficient function sampled at the point, as
+  // MatrixFreeOperator::evaluate_coefficient does, operators.h:445-457);
+  // modes 1, 2, 3 average over the finest cells that meet the point's share
+  // of the coarse cell (per direction the interval between the midpoints of
+  // neighbouring Gauss points), weighted by overlap: geometric, harmonic,
+  // arithmetic mean.  The averaged modes are an extension of this build for
+  // rough coefficients; the reference has only the sampled form.
   struct Coefficient
   {
     int                 dim = 2;
     int                 fine_nc[3] = {1, 1, 1};
     std::vector<double> kappa; // empty: constant 1
+    int                 mode = 0;
+    std::vector<double> gq; // Gauss points of the quadrature the levels use
     double at(int lev_shift, int cx, int cy, int cz, const double *xi) const
     {
       if (kappa.empty())
         return 1.0;
       const int s = 1 << lev_shift;
+      if (mode != 0 && s > 1)
+        return averaged(s, cx, cy, cz, xi);
       int       f[3] = {cx, cy, cz};
       for (int d = 0; d < dim; ++d)
         {
@@ -888,6 +902,45 @@ namespace orc
         }
       long id = f[0] + (long)fine_nc[0] * (f[1] + (long)fine_nc[1] * (dim == 3 ? f[2] : 0));
       return kappa[id];
+    }
+    double averaged(int s, int cx, int cy, int cz, const double *xi) const
+    {
+      const int nq = (int)gq.size();
+      std::vector<std::pair<int, double>> share[3];
+      for (int d = 0; d < 3; ++d)
+        {
+          if (d >= dim)
+            {
+              share[d].push_back({0, 1.0});
+              continue;
+            }
+          int q = 0;
+          for (int k = 1; k < nq; ++k)
+            if (std::abs(gq[k] - xi[d]) < std::abs(gq[q] - xi[d]))
+              q = k;
+          const double lo = q == 0 ? 0.0 : 0.5 * (gq[q - 1] + gq[q]);
+          const double hi = q == nq - 1 ? 1.0 : 0.5 * (gq[q] + gq[q + 1]);
+          for (int sub = 0; sub < s; ++sub)
+            {
+              const double ov = std::min(hi, double(sub + 1) / s) - std::max(lo, double(sub) / s);
+              if (ov > 0)
+                share[d].push_back({sub, ov});
+            }
+        }
+      double acc = 0.0, wsum = 0.0;
+      for (const auto &ez : share[2])
+        for (const auto &ey : share[1])
+          for (const auto &ex : share[0])
+            {
+              const long   f0 = long(cx) * s + ex.first, f1 = long(cy) * s + ey.first;
+              const long   f2 = dim == 3 ? long(cz) * s + ez.first : 0;
+              const double w  = ex.second * ey.second * (dim == 3 ? ez.second : 1.0);
+              const double kv = kappa[f0 + (long)fine_nc[0] * (f1 + (long)fine_nc[1] * f2)];
+              acc += w * (mode == 1 ? std::log(kv) : mode == 2 ? 1.0 / kv : kv);
+              wsum += w;
+            }
+      acc /= wsum;
+      return mode == 1 ? std::exp(acc) : mode == 2 ? 1.0 / acc : acc;
     }
   };
 
@@ -1390,6 +1443,8 @@ namespace orc
     double              frequency = 1.0;
     int                 rhs = 0; // 0: manufactured solution's source, 1: zero
     int                 problem = 0; // 0: heat, 1: wave (tp_01.cc:167-197, time_integrators.h:183-280)
+    int                 coarse_coef = 0; // coarse-level coefficient: 0 sampled, 1/2/3 geometric/harmonic/arithmetic mean
+    int                 dirichlet = 63;  // zero-Dirichlet faces (bit 2d: low side, 2d+1: high side)
     int                 fgmres_variant = 0; // 0: deal.II 9.5 projected solve at j>0
     int                 dealii_order = 0;   // power-iteration start vector in deal.II dof order
     double              omega_scale = 1.0;  // diagnostic multiplier on estimated omegas
@@ -1457,6 +1512,8 @@ namespace orc
     geti("rhs", sp.rhs);
     if (kv.count("problem"))
       sp.problem = (kv["problem"] == "wave" || kv["problem"] == "1") ? 1 : 0;
+    geti("coarse_coef", sp.coarse_coef);
+    geti("dirichlet", sp.dirichlet);
     geti("fgmres_variant", sp.fgmres_variant);
     geti("dealii_order", sp.dealii_order);
     getd("omega_scale", sp.omega_scale);
@@ -1505,6 +1562,7 @@ namespace orc
       // the finest vertices.
       Grid g;
       g.dim         = spec.dim;
+      g.dirichlet   = spec.dirichlet;
       const int fac = 1 << (spec.refine - s);
       for (int d = 0; d < 3; ++d)
         g.nc[d] = (d < spec.dim) ? fine_grid.nc[d] / fac : 1;
@@ -1554,7 +1612,8 @@ namespace orc
       spec = s;
       sh.init(spec.p, spec.p + 1);
       // fine grid
-      fine_grid.dim = spec.dim;
+      fine_grid.dim       = spec.dim;
+      fine_grid.dirichlet = spec.dirichlet;
       for (int d = 0; d < 3; ++d)
         fine_grid.nc[d] = (d < spec.dim) ? spec.subdiv[d] << spec.refine : 1;
       const long nvx = fine_grid.nc[0] + 1, nvy = fine_grid.nc[1] + 1,
@@ -1576,6 +1635,8 @@ namespace orc
         coef.fine_nc[d] = fine_grid.nc[d];
       if (kappa)
         coef.kappa.assign(kappa, kappa + fine_grid.n_cells());
+      coef.mode = spec.coarse_coef;
+      coef.gq   = sh.gq;
       // tau rule (tp_01.cc:87, 124-127)
       if (spec.tau <= 0)
         {

@@ -50,6 +50,11 @@ CASES = {
     # configs[2] time discretisation cG(2) with Q2 (heat operator), 4^3 cells
     "3d_q2_cg2": dict(spec=dict(dim=3, refine=2, type=1, r=2, nts=2, p=2, r_min=1), perturb=0.0,
                       kappa=False),
+    # arithmetic-mean coarse coefficients, Q4 dG(3), FP32 factored Vanka
+    "3d_q4_dg3_cc3": dict(spec=dict(dim=3, refine=2, type=2, r=3, nts=1, p=4, r_min=2,
+                                    mg_float=1, vanka_factored=1, coarse_coef=3,
+                                    smoothing_steps=2),
+                          perturb=0.1, kappa=True),
 }
 
 
