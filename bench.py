@@ -42,7 +42,10 @@ def box_vertices(dim, nc, perturb, seed):
 
 
 def workload(refine):
-    """Spec, vertices and coefficient of the 3D heat workload on 2^refine cells per side."""
+    """Spec, vertices and coefficient of the 3D heat workload on 2^refine cells per side
+    (Q3 space, dG(2) time, vertices perturbed by up to 0.2h, cellwise log-uniform
+    coefficient in [1e-2, 1e2], seeded Gaussian bumps).  Two pre- and two
+    post-smoothing steps with the factored Vanka smoother."""
     import numpy as np
     dim = 3
     nc = [1 << refine] * dim
@@ -51,7 +54,7 @@ def workload(refine):
     kappa = np.exp(rng.uniform(np.log(1e-2), np.log(1e2), int(np.prod(nc))))
     spec = dict(dim=3, refine=refine, p=3, type=2, r=2, nts=1, r_min=0, tau=0.125,
                 end_time=1.0, source=3, n_bumps=4, seed=7, reltol=1e-8, abstol=1e-300,
-                maxiter=200, basis=30, vanka_factored=1)
+                maxiter=200, basis=30, vanka_factored=1, smoothing_steps=2)
     return spec, X, kappa
 
 
@@ -59,7 +62,11 @@ def workload_large():
     """Spec, vertices and coefficient of the large 3D heat workload: 64^3
     cells perturbed by up to 0.1h, Q4 space, dG(3) time (4 time dofs per
     slab, 67.9M space-time dofs per slab), T = 1 in 4 slabs, cellwise
-    log-uniform coefficient in [1e-3, 1e3], seeded Gaussian bumps."""
+    log-uniform coefficient in [1e-3, 1e3], seeded Gaussian bumps.  Two pre-
+    and two post-smoothing steps; the coarse levels take the arithmetic mean
+    of the finest-level coefficient over each quadrature point's share of
+    the cell (with the point-sampled coefficient the multigrid does not
+    converge for this contrast)."""
     import numpy as np
     dim, refine = 3, 6
     nc = [1 << refine] * dim
@@ -68,7 +75,7 @@ def workload_large():
     kappa = np.exp(rng.uniform(np.log(1e-3), np.log(1e3), int(np.prod(nc))))
     spec = dict(dim=3, refine=refine, p=4, type=2, r=3, nts=1, tau=0.25, end_time=1.0,
                 source=3, n_bumps=4, seed=7, reltol=1e-8, abstol=1e-300, maxiter=200,
-                basis=30, vanka_factored=1)
+                basis=30, vanka_factored=1, smoothing_steps=2, coarse_coef=3)
     return spec, X, kappa
 
 

@@ -120,7 +120,7 @@ namespace stfem
           if (done || nloc != NL || nt != NT || !cells)
             return;
           const bool use = variant == 3 || (variant == 4 && sizeof(T) == 4) ||
-                           (variant == 0 && NL > 64);
+                           (variant == 0 && NL >= 64);
           if (!use)
             return;
           constexpr size_t lds = V3<T, NL, NT>::lds;
