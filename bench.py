@@ -159,7 +159,9 @@ def pmc_traffic(kernel, name, refine, timeout_s=150):
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="stfem_pmc_", dir="/tmp")
-        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+        # counters only for the measured kernel and the calibration copy
+        cmd = [exe, "--kernel-include-regex", f"copyBuffer|{name}", "--pmc", counter,
+               "--output-format", "csv", "-d", d, "-o", "run", "--",
                sys.executable, os.path.abspath(__file__), "--kernel-only", kernel,
                "--refine", str(refine)]
         log(f"counter pass {counter}: {' '.join(cmd)}")
@@ -270,7 +272,7 @@ def main():
     # sweeps (pre/post), three level vmults and one FP64 vmult
     if 2 * vk_ms >= 3 * vm32_ms + vm64_ms:
         dom = dict(kernel="vanka_factored_fp32", ms=vk_ms, bytes=vk_bytes, only="vanka",
-                   symbol=r"k_vanka_factored")
+                   symbol=r"k_vanka_factored_v")
     else:
         dom = dict(kernel="st_vmult_fp64", ms=vm64_ms, bytes=vm64_bytes, only="vmult64",
                    symbol=r"k_st_apply<3, \d, \d, double")
