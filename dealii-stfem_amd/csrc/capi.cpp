@@ -366,6 +366,43 @@ extern "C"
     });
     return g != STFEM_OK ? g : rc;
   }
+  int stfem_forward_trajectory(stfem_problem p, int n_slabs, double *traj, int *its)
+  {
+    return guarded([&] {
+      STFEM_REQUIRE(traj, "null trajectory");
+      auto &pb = P(p);
+      device_barrier(pb);
+      pb.forward_trajectory(n_slabs, traj, its);
+    });
+  }
+  int stfem_misfit(stfem_problem p, const double *obs, int n_slabs, double *misfit, int *its)
+  {
+    return guarded([&] {
+      STFEM_REQUIRE(obs && misfit, "null argument");
+      auto &pb = P(p);
+      device_barrier(pb);
+      *misfit = pb.misfit(obs, n_slabs, its);
+    });
+  }
+  int stfem_adjoint_gradient(stfem_problem p, const double *obs, int n_slabs, double *grad,
+                             double *misfit, int *its)
+  {
+    return guarded([&] {
+      STFEM_REQUIRE(obs && grad && misfit, "null argument");
+      auto &pb = P(p);
+      device_barrier(pb);
+      pb.adjoint_gradient(obs, n_slabs, grad, misfit, its);
+    });
+  }
+  int stfem_set_cell_coefficient(stfem_problem p, const double *kappa)
+  {
+    return guarded([&] {
+      STFEM_REQUIRE(kappa, "null coefficient");
+      auto &pb = P(p);
+      device_barrier(pb);
+      pb.set_cell_coefficient(kappa);
+    });
+  }
   int stfem_set_load_vector(stfem_problem p, const double *h)
   {
     return guarded([&] {

@@ -157,6 +157,14 @@ namespace stfem
                            const double *Alpha, const double *Beta, T *S, T *W, int *singular,
                            hipStream_t s);
 
+  // ------------------------------------------------------- adjoint
+  // grad[c] -= kappa[c] * sum_j lam_j^T K_c w_j with K_c the unit-coefficient
+  // cell stiffness (Gauss(p+1) points qx/qw on [0,1], S/D the FE_Q(p) basis
+  // values/derivatives there, [q*(p+1) + a]); lam, w node-major, nt values.
+  void launch_cell_grad(const GridDesc &g, const double *vertices, int nt, const double *lam,
+                        const double *w, const double *kappa, const double *qx, const double *qw,
+                        const double *S, const double *D, double *grad, hipStream_t s);
+
   // ------------------------------------------------------- slab error
   // Per-cell {L2^2, Linf, H1semi^2} errors of a slab against the
   // manufactured heat solution: nq Gauss points (qx, qw on [0,1]) with the

@@ -504,6 +504,14 @@ namespace stfem
     apply(dst, src, dA_, dB_, 0);
   }
   template <typename T>
+  void SystemMatrix<T>::set_cell_coefficient(const double *kappa)
+  {
+    std::vector<T> c(kappa, kappa + grid_.n_cells);
+    coef_.upload(c.data(), c.size(), ctx_.stream);
+    coef_mode_ = 1;
+    ctx_.sync();
+  }
+  template <typename T>
   void SystemMatrix<T>::Tvmult(T *dst, const T *src) const
   {
     STFEM_REQUIRE(nt_in_ == nt_out_, "Tvmult needs square time matrices");
@@ -1796,68 +1804,7 @@ namespace stfem
             STFEM_REQUIRE(nsteps >= 1, "spec: end time shorter than the coarse step");
             tau = spec.end_time * std::pow(2.0, -(spec.refine + 1)) / nsteps;
           }
-        // level sequence (tp_01.cc:205-219)
-        const int n_sp = spec.mg_space ? spec.refine + 1 : 1;
-        const int rmin = spec.space_time_mg ? spec.r_min : spec.r;
-        const int tmin = spec.space_time_mg ? std::max(spec.nts_min, 1) : spec.nts;
-        const auto seq = get_time_mg_sequence(n_sp, spec.r, rmin, spec.nts, tmin,
-                                              spec.time_before_space != 0);
-        const int nlev = int(seq.size()) + 1;
-        std::vector<int>    sref(nlev), rl(nlev), nl(nlev);
-        std::vector<double> tl(nlev);
-        sref[nlev - 1] = spec.refine;
-        rl[nlev - 1]   = spec.r;
-        nl[nlev - 1]   = spec.nts;
-        tl[nlev - 1]   = tau;
-        for (int l = nlev - 2; l >= 0; --l)
-          {
-            const TimeMGType t = seq[l];
-            sref[l] = sref[l + 1] - ((t == TimeMGType::none && spec.mg_space) ? 1 : 0);
-            rl[l]   = rl[l + 1] - (t == TimeMGType::k ? 1 : 0);
-            nl[l]   = nl[l + 1] / (t == TimeMGType::tau ? 2 : 1);
-            tl[l]   = tl[l + 1] * (t == TimeMGType::tau ? 2.0 : 1.0);
-          }
-        std::vector<GMGLevel<T>> levels(nlev);
-        const auto form = spec.mg.vanka_factored ? PreconditionVanka<T>::Form::factored :
-                                                   PreconditionVanka<T>::Form::explicit_inverse;
-        for (int l = 0; l < nlev; ++l)
-          {
-            TimeWeights w = get_fe_time_weights(spec.type, rl[l], tl[l], nl[l]);
-            if (spec.problem == 1)
-              {
-                // wave levels: the Schur-in-time transform of the level's
-                // multi-step weights taken as one interval (fe_time.h:383-412)
-                const WaveWeights ww =
-                  get_fe_time_weights_wave(spec.type, w.Alpha, w.Beta, w.Gamma, w.Zeta, 1);
-                w.Alpha = ww.lhs_uK;
-                w.Beta  = ww.lhs_uM;
-              }
-            const MeshLevel &m = mesh_[sref[l]];
-            levels[l].A  = std::make_unique<SystemMatrix<T>>(ctx_, m, w.Alpha, w.Beta,
-                                                             spec.atomic_scatter == 0);
-            levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form,
-                                                                  spec.vanka_device_setup != 0);
-            levels[l].V->set_variant(spec.vanka_kernel);
-            levels[l].r  = rl[l];
-            levels[l].nts = nl[l];
-            levels[l].type_to_here = l > 0 ? seq[l - 1] : TimeMGType::none;
-            ctx_.sync();
-            plog("level operator and Vanka", l);
-          }
-        std::vector<TwoLevelTransfer<T>> tr(nlev);
-        for (int l = 1; l < nlev; ++l)
-          {
-            const TimeMGType t = seq[l - 1];
-            if (t == TimeMGType::none)
-              tr[l].space = std::make_unique<MGTwoLevelBlockTransfer<T>>(
-                ctx_, mesh_[sref[l]].g, mesh_[sref[l - 1]].g, sh_, levels[l].A->n_blocks());
-            else
-              tr[l].time = std::make_unique<MGTwoLevelTransferST<T>>(
-                ctx_, spec.type, rl[l], nl[l], t, mesh_[sref[l]].g.n_nodes,
-                spec.mg.overwrite_time_transfer);
-          }
-        plog("transfers");
-        gmg_ = std::make_unique<GMG<T>>(ctx_, std::move(levels), std::move(tr), spec.mg);
+        gmg_ = build_gmg(false, plog);
         plog("relaxation estimates and dense coarse operator");
         // FP64 fine system and rhs matrices (tp_01.cc:152-198, heat)
         const MeshLevel  &F  = mesh_.back();
@@ -1939,6 +1886,83 @@ namespace stfem
             });
           }
         ctx_.sync();
+      }
+
+      // The space-time multigrid of tp_01.cc:200-359 / GMG::reinit
+      // (gmg.h:708-790): level sequence, level operators, Vanka smoothers,
+      // transfers.  transpose: the same hierarchy for the transposed system
+      // (time matrices transposed on every level), the preconditioner of the
+      // adjoint solves.
+      std::unique_ptr<GMG<T>> build_gmg(bool transpose, PhaseLog &plog)
+      {
+        // level sequence (tp_01.cc:205-219)
+        const int n_sp = spec.mg_space ? spec.refine + 1 : 1;
+        const int rmin = spec.space_time_mg ? spec.r_min : spec.r;
+        const int tmin = spec.space_time_mg ? std::max(spec.nts_min, 1) : spec.nts;
+        const auto seq = get_time_mg_sequence(n_sp, spec.r, rmin, spec.nts, tmin,
+                                              spec.time_before_space != 0);
+        const int nlev = int(seq.size()) + 1;
+        std::vector<int>    sref(nlev), rl(nlev), nl(nlev);
+        std::vector<double> tl(nlev);
+        sref[nlev - 1] = spec.refine;
+        rl[nlev - 1]   = spec.r;
+        nl[nlev - 1]   = spec.nts;
+        tl[nlev - 1]   = tau;
+        for (int l = nlev - 2; l >= 0; --l)
+          {
+            const TimeMGType t = seq[l];
+            sref[l] = sref[l + 1] - ((t == TimeMGType::none && spec.mg_space) ? 1 : 0);
+            rl[l]   = rl[l + 1] - (t == TimeMGType::k ? 1 : 0);
+            nl[l]   = nl[l + 1] / (t == TimeMGType::tau ? 2 : 1);
+            tl[l]   = tl[l + 1] * (t == TimeMGType::tau ? 2.0 : 1.0);
+          }
+        std::vector<GMGLevel<T>> levels(nlev);
+        const auto form = spec.mg.vanka_factored ? PreconditionVanka<T>::Form::factored :
+                                                   PreconditionVanka<T>::Form::explicit_inverse;
+        for (int l = 0; l < nlev; ++l)
+          {
+            TimeWeights w = get_fe_time_weights(spec.type, rl[l], tl[l], nl[l]);
+            if (transpose)
+              {
+                // adjoint levels: (Alpha^T x K + Beta^T x M), K and M symmetric
+                w.Alpha = w.Alpha.transposed();
+                w.Beta  = w.Beta.transposed();
+              }
+            if (spec.problem == 1)
+              {
+                // wave levels: the Schur-in-time transform of the level's
+                // multi-step weights taken as one interval (fe_time.h:383-412)
+                const WaveWeights ww =
+                  get_fe_time_weights_wave(spec.type, w.Alpha, w.Beta, w.Gamma, w.Zeta, 1);
+                w.Alpha = ww.lhs_uK;
+                w.Beta  = ww.lhs_uM;
+              }
+            const MeshLevel &m = mesh_[sref[l]];
+            levels[l].A  = std::make_unique<SystemMatrix<T>>(ctx_, m, w.Alpha, w.Beta,
+                                                             spec.atomic_scatter == 0);
+            levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form,
+                                                                  spec.vanka_device_setup != 0);
+            levels[l].V->set_variant(spec.vanka_kernel);
+            levels[l].r  = rl[l];
+            levels[l].nts = nl[l];
+            levels[l].type_to_here = l > 0 ? seq[l - 1] : TimeMGType::none;
+            ctx_.sync();
+            plog("level operator and Vanka", l);
+          }
+        std::vector<TwoLevelTransfer<T>> tr(nlev);
+        for (int l = 1; l < nlev; ++l)
+          {
+            const TimeMGType t = seq[l - 1];
+            if (t == TimeMGType::none)
+              tr[l].space = std::make_unique<MGTwoLevelBlockTransfer<T>>(
+                ctx_, mesh_[sref[l]].g, mesh_[sref[l - 1]].g, sh_, levels[l].A->n_blocks());
+            else
+              tr[l].time = std::make_unique<MGTwoLevelTransferST<T>>(
+                ctx_, spec.type, rl[l], nl[l], t, mesh_[sref[l]].g.n_nodes,
+                spec.mg.overwrite_time_transfer);
+          }
+        plog("transfers");
+        return std::make_unique<GMG<T>>(ctx_, std::move(levels), std::move(tr), spec.mg);
       }
 
       int n_levels() const override { return gmg_->n_levels(); }
@@ -2168,6 +2192,182 @@ namespace stfem
         stats[6] = nt;
         return rc;
       }
+      // ------------------------------------------------------------ adjoint
+      // Discrete adjoint of the heat slab sequence A x_n = R E x_{n-1} + f_n
+      // (TimeIntegratorHeat, time_integrators.h:157-180), x_{-1} = 0, with
+      // the misfit J = 1/2 sum_n e_n^T (Alpha (x) M) e_n, e_n = x_n - d_n
+      // (for dG, Alpha is the time mass matrix, so J is half the squared
+      // L2(space-time) norm of the mismatch).  Backward in n:
+      //   A^T lambda_n = (Alpha (x) M) e_n + E^T R^T lambda_{n+1}
+      //   dJ/dtheta_c = -sum_n lambda_n^T (dA/dtheta_c) x_n
+      // (R = Gamma (x) M carries no coefficient for dG).  Only the slab start
+      // states are kept from the forward sweep; each slab is solved again
+      // before its adjoint solve.  The adjoint solves use FGMRES with the
+      // multigrid of the transposed system.
+      void ensure_adjoint()
+      {
+        STFEM_REQUIRE(spec.problem == 0, "adjoint: heat problem only");
+        STFEM_REQUIRE(spec.type == TimeStepType::DG, "adjoint: dG time discretisation only");
+        if (gmg_adj_)
+          return;
+        const MeshLevel &F = mesh_.back();
+        const bool       tp = spec.atomic_scatter == 0;
+        const DMat      &A = fine_->alpha();
+        fineT_  = std::make_unique<SystemMatrix<double>>(ctx_, F, A.transposed(),
+                                                         fine_->beta().transposed(), tp);
+        misfit_ = std::make_unique<SystemMatrix<double>>(ctx_, F, DMat(A.m, A.n), A, tp);
+        rhsT_   = std::make_unique<SystemMatrix<double>>(ctx_, F, rhs_->alpha().transposed(),
+                                                         rhs_->beta().transposed(), tp);
+        dalpha_.upload(A.a.data(), A.a.size(), ctx_.stream);
+        std::vector<double> k(size_t(F.g.n_cells), F.coef_mode == 0 ? F.coef_value : 0.0);
+        if (F.coef_mode == 1)
+          k.assign(F.coef.begin(), F.coef.end());
+        kappa_.upload(k.data(), k.size(), ctx_.stream);
+        std::vector<double> sel(nt, 0.0);
+        sel[nt - 1] = 1.0;
+        last_.upload(sel.data(), sel.size(), ctx_.stream); // picks / places the last block
+        ctx_.sync();
+        PhaseLog plog;
+        gmg_adj_ = build_gmg(true, plog);
+        plog("adjoint multigrid");
+      }
+      // 1/2 e^T (Alpha x M) e with e = x - d (e, qe: scratch)
+      double slab_misfit(const double *x, const double *d, double *e, double *qe)
+      {
+        const long m = fine_->m();
+        STFEM_HIP(hipMemcpyAsync(e, x, size_t(m) * sizeof(double), hipMemcpyDeviceToDevice,
+                                 ctx_.stream));
+        launch_axpby<double>(m, -1.0, d, 1.0, e, ctx_.stream);
+        misfit_->vmult(qe, e);
+        launch_dot_partials<double>(m, e, qe, ctx_.partial.get(), ctx_.stream);
+        std::vector<double> h(static_cast<size_t>(reduce_slots()));
+        STFEM_HIP(hipMemcpyAsync(h.data(), ctx_.partial.get(), h.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx_.stream));
+        ctx_.sync();
+        return 0.5 * sum_partials(h.data());
+      }
+      void forward_trajectory(int n_slabs, double *traj, int *its) override
+      {
+        STFEM_REQUIRE(n_slabs >= 1, "trajectory: at least one slab");
+        const long   N = fine_->n_nodes(), m = fine_->m();
+        DBuf<double> x{size_t(m)}, prev{size_t(N)};
+        std::vector<double> sel(nt, 0.0);
+        sel[nt - 1] = 1.0;
+        DBuf<double> last;
+        last.upload(sel.data(), sel.size(), ctx_.stream);
+        launch_fill_zero<double>(x.get(), x.size(), ctx_.stream);
+        for (int n = 0; n < n_slabs; ++n)
+          {
+            launch_time_transfer<double>(N, last.get(), 1, nt, x.get(), prev.get(), false,
+                                         ctx_.stream);
+            int       k = 0;
+            const int r = solve_slab(x.get(), prev.get(), n * spec.nts * tau, &k);
+            if (its)
+              its[n] = k;
+            if (r != 0)
+              throw Error(r, "forward trajectory: slab solve failed");
+            STFEM_HIP(hipMemcpyAsync(traj + size_t(n) * m, x.get(), size_t(m) * sizeof(double),
+                                     hipMemcpyDeviceToDevice, ctx_.stream));
+          }
+        ctx_.sync();
+      }
+      double misfit(const double *obs, int n_slabs, int *its) override
+      {
+        ensure_adjoint();
+        const long   N = fine_->n_nodes(), m = fine_->m();
+        DBuf<double> x{size_t(m)}, prev{size_t(N)}, e{size_t(m)}, qe{size_t(m)};
+        launch_fill_zero<double>(x.get(), x.size(), ctx_.stream);
+        double J = 0.0;
+        for (int n = 0; n < n_slabs; ++n)
+          {
+            launch_time_transfer<double>(N, last_.get(), 1, nt, x.get(), prev.get(), false,
+                                         ctx_.stream);
+            int       k = 0;
+            const int r = solve_slab(x.get(), prev.get(), n * spec.nts * tau, &k);
+            if (its)
+              its[n] = k;
+            if (r != 0)
+              throw Error(r, "misfit: slab solve failed");
+            J += slab_misfit(x.get(), obs + size_t(n) * m, e.get(), qe.get());
+          }
+        return J;
+      }
+      void adjoint_gradient(const double *obs, int n_slabs, double *grad, double *J,
+                            int *its) override
+      {
+        STFEM_REQUIRE(n_slabs >= 1, "adjoint: at least one slab");
+        ensure_adjoint();
+        const GridDesc &g = fine_->grid();
+        const long      N = g.n_nodes, m = fine_->m();
+        DBuf<double> ck{size_t(n_slabs) * N}, x{size_t(m)}, e{size_t(m)}, qe{size_t(m)},
+          lam{size_t(m)}, w{size_t(m)}, lr{size_t(N)};
+        // forward sweep: slab start states only
+        launch_fill_zero<double>(x.get(), x.size(), ctx_.stream);
+        double Jv = 0.0;
+        for (int n = 0; n < n_slabs; ++n)
+          {
+            double *prev = ck.get() + size_t(n) * N;
+            launch_time_transfer<double>(N, last_.get(), 1, nt, x.get(), prev, false,
+                                         ctx_.stream);
+            int       k = 0;
+            const int r = solve_slab(x.get(), prev, n * spec.nts * tau, &k);
+            if (its)
+              its[2 * n] = k;
+            if (r != 0)
+              throw Error(r, "adjoint: forward slab solve failed");
+            Jv += slab_misfit(x.get(), obs + size_t(n) * m, e.get(), qe.get());
+          }
+        // backward sweep
+        launch_fill_zero<double>(grad, size_t(g.n_cells), ctx_.stream);
+        for (int n = n_slabs - 1; n >= 0; --n)
+          {
+            const double *prev = ck.get() + size_t(n) * N;
+            int           k    = 0;
+            int           r    = solve_slab(x.get(), prev, n * spec.nts * tau, &k);
+            if (r != 0)
+              throw Error(r, "adjoint: slab recomputation failed");
+            // adjoint right-hand side: (Alpha x M) e_n + E^T R^T lambda_{n+1}
+            (void)slab_misfit(x.get(), obs + size_t(n) * m, e.get(), qe.get());
+            if (n + 1 < n_slabs)
+              launch_time_transfer<double>(N, last_.get(), nt, 1, lr.get(), qe.get(), true,
+                                           ctx_.stream);
+            launch_fill_zero<double>(lam.get(), lam.size(), ctx_.stream);
+            r = solver_->solve(*fineT_, *gmg_adj_, lam.get(), qe.get());
+            if (its)
+              its[2 * n + 1] = solver_->last_step();
+            if (r != 0)
+              throw Error(r, "adjoint: adjoint slab solve failed");
+            launch_zero_dirichlet<double>(g, nt, lam.get(), ctx_.stream);
+            // g_c -= kappa_c sum_j lambda_j^T K_c (Alpha x)_j
+            launch_time_transfer<double>(N, dalpha_.get(), nt, nt, x.get(), w.get(), false,
+                                         ctx_.stream);
+            launch_cell_grad(g, fine_->vertices(), nt, lam.get(), w.get(), kappa_.get(),
+                             sh_.qx.data(), sh_.qw.data(), sh_.S.data(), sh_.Sd.data(), grad,
+                             ctx_.stream);
+            // R^T lambda_n for the previous slab's last block
+            rhsT_->vmult(lr.get(), lam.get());
+          }
+        ctx_.sync();
+        *J = Jv;
+      }
+      void set_cell_coefficient(const double *kappa) override
+      {
+        MeshLevel &F = mesh_.back();
+        F.coef_mode  = 1;
+        F.coef.assign(kappa, kappa + F.g.n_cells);
+        fine_->set_cell_coefficient(kappa);
+        rhs_->set_cell_coefficient(kappa);
+        if (rhsv_)
+          rhsv_->set_cell_coefficient(kappa);
+        if (gmg_adj_)
+          {
+            fineT_->set_cell_coefficient(kappa);
+            rhsT_->set_cell_coefficient(kappa);
+            kappa_.upload(kappa, size_t(F.g.n_cells), ctx_.stream);
+            ctx_.sync();
+          }
+      }
+
       void set_load_vector(const double *h) override
       {
         load_.upload(h, size_t(fine_->n_nodes()), ctx_.stream);
@@ -2319,6 +2519,9 @@ namespace stfem
       std::vector<MeshLevel>                mesh_;
       std::unique_ptr<GMG<T>>               gmg_;
       std::unique_ptr<SystemMatrix<double>> fine_, rhs_, rhsv_;
+      std::unique_ptr<SystemMatrix<double>> fineT_, misfit_, rhsT_; // adjoint
+      std::unique_ptr<GMG<T>>               gmg_adj_;
+      DBuf<double>                          dalpha_, kappa_, last_;
       std::unique_ptr<SolverFGMRES>         solver_;
       DMat                                  alpha1_, gamma1_;
       DBuf<double>                          rhs_vec_, load_, coefs_, ones_;

@@ -161,6 +161,8 @@ namespace stfem
     void vmult_add_nx1(T *dst, const T *src) const;
     // dst = rhs - (Alpha x K + Beta x M) src (the residual of the V-cycle)
     void residual(T *dst, const T *rhs, const T *src) const;
+    // replaces the coefficient by one value per cell (host array, n_cells)
+    void set_cell_coefficient(const double *kappa);
     long        n_nodes() const { return grid_.n_nodes; }
     int         n_blocks() const { return nt_out_; }
     const double *vertices() const { return verts_.get(); }
@@ -469,6 +471,19 @@ namespace stfem
     // stats = {slabs, total iterations, L2, Linf, H1semi, n_nodes, nt}
     virtual int  run_heat(double *stats, int *its_per_slab, int cap) = 0;
     virtual void set_load_vector(const double *h_host) = 0;
+    // ---- adjoint of the heat slab sequence (dG time), misfit
+    //   J = 1/2 sum_n (x_n - d_n)^T (Alpha (x) M) (x_n - d_n)
+    // over n_slabs slabs from zero initial values; obs and traj hold the
+    // node-major slab vectors one after the other (device).
+    virtual void forward_trajectory(int n_slabs, double *traj, int *its) = 0;
+    virtual double misfit(const double *obs, int n_slabs, int *its) = 0;
+    // dJ/d(log kappa_c) into grad (device, n_cells), misfit into *J; its:
+    // per slab the forward and the adjoint FGMRES iterations (2*n_slabs)
+    virtual void adjoint_gradient(const double *obs, int n_slabs, double *grad, double *J,
+                                  int *its) = 0;
+    // new per-cell coefficient of the fine system (host, n_cells); the
+    // multigrid preconditioners keep the coefficient they were built with
+    virtual void set_cell_coefficient(const double *kappa) = 0;
     virtual long n_nodes() const = 0;
     virtual Context &context() = 0;
     virtual double fine_vmult_bytes() const = 0;

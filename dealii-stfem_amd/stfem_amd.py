@@ -58,6 +58,10 @@ _SIGS = {
     "stfem_slab_error": ([_vp, _vp, _vp, ctypes.c_double, _dp], ctypes.c_int),
     "stfem_run_heat": ([_vp, _dp, _ip, ctypes.c_int], ctypes.c_int),
     "stfem_set_load_vector": ([_vp, _dp], ctypes.c_int),
+    "stfem_forward_trajectory": ([_vp, ctypes.c_int, _vp, _ip], ctypes.c_int),
+    "stfem_misfit": ([_vp, _vp, ctypes.c_int, _dp, _ip], ctypes.c_int),
+    "stfem_adjoint_gradient": ([_vp, _vp, ctypes.c_int, _vp, _dp, _ip], ctypes.c_int),
+    "stfem_set_cell_coefficient": ([_vp, _dp], ctypes.c_int),
     "stfem_bytes_model": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
     "stfem_blocks_to_nodes": ([ctypes.c_long, ctypes.c_int, _vp, _vp], ctypes.c_int),
     "stfem_nodes_to_blocks": ([ctypes.c_long, ctypes.c_int, _vp, _vp], ctypes.c_int),
@@ -277,6 +281,35 @@ class Problem:
         return dict(rc=rc, slabs=n, total_its=int(stats[1]), L2=stats[2], Linf=stats[3],
                     H1=stats[4], ndofs=int(stats[5]), nblocks=int(stats[6]),
                     its=[per[i] for i in range(min(n, max_slabs))])
+
+    # ---- adjoint of the heat slab sequence (dG)
+    def forward_trajectory(self, n_slabs, traj):
+        """Forward slab solutions from zero initial values into the device
+        tensor traj (n_slabs * n_nodes * nt); returns iterations per slab."""
+        its = np.zeros(n_slabs, np.int32)
+        _check(load().stfem_forward_trajectory(self.h, int(n_slabs), _ptr(traj),
+                                               its.ctypes.data_as(_ip)))
+        return its
+
+    def misfit(self, obs, n_slabs):
+        J = ctypes.c_double()
+        its = np.zeros(n_slabs, np.int32)
+        _check(load().stfem_misfit(self.h, _ptr(obs), int(n_slabs), ctypes.byref(J),
+                                   its.ctypes.data_as(_ip)))
+        return J.value, its
+
+    def adjoint_gradient(self, obs, n_slabs, grad):
+        """dJ/dlog(kappa) into the device tensor grad (n_cells); returns
+        (J, per-slab forward and adjoint iterations)."""
+        J = ctypes.c_double()
+        its = np.zeros(2 * n_slabs, np.int32)
+        _check(load().stfem_adjoint_gradient(self.h, _ptr(obs), int(n_slabs), _ptr(grad),
+                                             ctypes.byref(J), its.ctypes.data_as(_ip)))
+        return J.value, its
+
+    def set_cell_coefficient(self, kappa):
+        k = np.ascontiguousarray(kappa, np.float64)
+        _check(load().stfem_set_cell_coefficient(self.h, _p(k)))
 
     def set_load_vector(self, h):
         h = np.ascontiguousarray(h, np.float64)

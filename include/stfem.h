@@ -151,6 +151,34 @@ int stfem_run_heat(stfem_problem p, double *stats, int *its_per_slab, int cap);
 /* host array h (one value per fine node): source f(t,x) = h(x) (int phi_i)
  * with unit time profile (source=2) */
 int stfem_set_load_vector(stfem_problem p, const double *h);
+
+/* ---- Adjoint of the heat slab sequence (dG time discretisation).  No
+ * counterpart in the reference, which has only the forward path; built on
+ * its slab system (TimeIntegratorHeat::solve, time_integrators.h:157-180)
+ * and SystemMatrix::Tvmult (operators.h:124-152).  The sequence starts from
+ * zero initial values; slab n covers [n*nts*tau, (n+1)*nts*tau).  Slab
+ * vectors are node-major (nt values per node), one slab after the other.
+ * Misfit J = 1/2 sum_n (x_n - d_n)^T (Alpha (x) M) (x_n - d_n): for dG,
+ * Alpha is the time mass matrix, so J is half the squared L2(space-time)
+ * norm of the mismatch. */
+/* forward solutions of n_slabs slabs into traj (device, n_slabs*n_nodes*nt);
+ * its: FGMRES iterations per slab (host, n_slabs) or NULL */
+int stfem_forward_trajectory(stfem_problem p, int n_slabs, double *traj, int *its);
+/* J for the observations obs (device, n_slabs*n_nodes*nt) */
+int stfem_misfit(stfem_problem p, const double *obs, int n_slabs, double *misfit, int *its);
+/* dJ/d(log kappa_c) for every fine cell into grad (device, n_cells) and J
+ * into *misfit: forward sweep keeping only the slab start states, then
+ * backward: each slab is solved again, then its adjoint
+ *   A^T lambda_n = (Alpha (x) M) e_n + E^T R^T lambda_{n+1}
+ * with FGMRES and the multigrid of the transposed system, and
+ *   grad_c -= kappa_c sum_j lambda_{n,j}^T K_c (Alpha x_n)_j.
+ * its: per slab the forward and the adjoint iterations (host, 2*n_slabs) or
+ * NULL. */
+int stfem_adjoint_gradient(stfem_problem p, const double *obs, int n_slabs, double *grad,
+                           double *misfit, int *its);
+/* new per-cell coefficient of the fine system (host, n_cells); the
+ * multigrid preconditioners keep the coefficient they were built with */
+int stfem_set_cell_coefficient(stfem_problem p, const double *kappa);
 /* algorithmic HBM bytes of one application: what 0 fine vmult, 1 Vanka
  * sweep on `level` */
 int stfem_bytes_model(stfem_problem p, int what, int level, double *bytes);

@@ -289,3 +289,44 @@ def solve_slab(problem, x, prev, time):
     rc = _solve_slab_sig().oracle_solve_slab(problem.h, _p(x), _p(prev), float(time),
                                              ctypes.byref(its))
     return rc, x, its.value
+
+
+def forward_trajectory(problem, n_slabs):
+    """Forward slab solutions from zero initial values (block-major per slab,
+    slabs one after the other); FP64 problems only."""
+    info = problem.level_info(problem.n_levels - 1)
+    m = info["n_nodes"] * info["nt"]
+    out = np.zeros(n_slabs * m)
+    L = load()
+    L.oracle_forward_trajectory.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    rc = L.oracle_forward_trajectory(problem.h, int(n_slabs), _p(out))
+    if rc != 0:
+        raise RuntimeError(f"oracle_forward_trajectory: {rc}")
+    return out
+
+
+def misfit(problem, obs, n_slabs):
+    L = load()
+    L.oracle_misfit.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
+    obs = np.ascontiguousarray(obs, np.float64)
+    J = np.zeros(1)
+    rc = L.oracle_misfit(problem.h, _p(obs), int(n_slabs), _p(J))
+    if rc != 0:
+        raise RuntimeError(f"oracle_misfit: {rc}")
+    return float(J[0])
+
+
+def adjoint_gradient(problem, problem_t, obs, n_slabs):
+    """dJ/dlog(kappa) per fine cell and J: problem_t is the same problem built
+    with transpose_time=1 (its FGMRES solves the adjoint slab systems)."""
+    L = load()
+    L.oracle_adjoint_gradient.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp, ctypes.c_int,
+                                          _dp, _dp]
+    obs = np.ascontiguousarray(obs, np.float64)
+    info = problem.level_info(problem.n_levels - 1)
+    grad = np.zeros(info["n_cells"])
+    J = np.zeros(1)
+    rc = L.oracle_adjoint_gradient(problem.h, problem_t.h, _p(obs), int(n_slabs), _p(grad), _p(J))
+    if rc != 0:
+        raise RuntimeError(f"oracle_adjoint_gradient: {rc}")
+    return grad, float(J[0])

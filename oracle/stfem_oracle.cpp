@@ -1445,6 +1445,7 @@ namespace orc
     int                 problem = 0; // 0: heat, 1: wave (tp_01.cc:167-197, time_integrators.h:183-280)
     int                 coarse_coef = 0; // coarse-level coefficient: 0 sampled, 1/2/3 geometric/harmonic/arithmetic mean
     int                 dirichlet = 63;  // zero-Dirichlet faces (bit 2d: low side, 2d+1: high side)
+    int                 transpose_time = 0; // every level's time matrices transposed (adjoint system)
     int                 fgmres_variant = 0; // 0: deal.II 9.5 projected solve at j>0
     int                 dealii_order = 0;   // power-iteration start vector in deal.II dof order
     double              omega_scale = 1.0;  // diagnostic multiplier on estimated omegas
@@ -1514,6 +1515,7 @@ namespace orc
       sp.problem = (kv["problem"] == "wave" || kv["problem"] == "1") ? 1 : 0;
     geti("coarse_coef", sp.coarse_coef);
     geti("dirichlet", sp.dirichlet);
+    geti("transpose_time", sp.transpose_time);
     geti("fgmres_variant", sp.fgmres_variant);
     geti("dealii_order", sp.dealii_order);
     getd("omega_scale", sp.omega_scale);
@@ -1598,8 +1600,8 @@ namespace orc
           w[0]    = ww[0];
           w[1]    = ww[1];
         }
-      L.Alpha   = w[0];
-      L.Beta    = w[1];
+      L.Alpha   = spec.transpose_time ? transpose(w[0]) : w[0];
+      L.Beta    = spec.transpose_time ? transpose(w[1]) : w[1];
       L.nt      = w[0].m;
       L.r       = r;
       L.nts     = nts;
@@ -1748,8 +1750,8 @@ namespace orc
           rhsK = w[2], rhsM = w[3];
         else
           rhsK = zero, rhsM = w[2];
-        F.Alpha  = w[0];
-        F.Beta   = w[1];
+        F.Alpha  = spec.transpose_time ? transpose(w[0]) : w[0];
+        F.Beta   = spec.transpose_time ? transpose(w[1]) : w[1];
         F.nt     = w[0].m;
         F.At     = F.Alpha.a;
         F.Bt     = F.Beta.a;
@@ -2656,6 +2658,160 @@ extern "C"
       std::vector<double> xv(x, x + (long)P.nt * N), pv(prev, prev + N);
       P.slab_error(time, xv, pv, out);
     });
+    return 0;
+  }
+  // ---- Discrete adjoint of the heat slab sequence (FP64, dG), the check of
+  // the GPU adjoint.  Slab n: A x_n = R E x_{n-1} + f_n from x_{-1} = 0
+  // (TimeIntegratorHeat::solve, time_integrators.h:157-180); misfit
+  // J = 1/2 sum_n e_n^T (Alpha (x) M) e_n, e_n = x_n - d_n; adjoint
+  // A^T lambda_n = (Alpha (x) M) e_n + E^T R^T lambda_{n+1} solved with the
+  // problem hT (spec transpose_time=1: FGMRES with the multigrid of the
+  // transposed system); dJ/dlog(kappa_c) = -kappa_c sum_n sum_j
+  // lambda_{n,j}^T K_c (Alpha x_n)_j with the unit-coefficient element
+  // stiffness K_c.  Vectors block-major (block j = time dof j).
+  static double misfit_slab(Problem<double> &P, const double *x, const double *d,
+                            std::vector<double> &qe)
+  {
+    const long N = P.fine.N();
+    const int  nt = P.nt;
+    std::vector<double> e(size_t(N) * nt), tmp(N);
+    for (size_t k = 0; k < e.size(); ++k)
+      e[k] = x[k] - d[k];
+    qe.assign(size_t(N) * nt, 0.0);
+    const Mat &A = P.fine.Alpha;
+    for (int i = 0; i < nt; ++i)
+      {
+        spmv_free(P.fine.M, P.fine.Mv, P.fine.sp.constrained, e.data() + size_t(i) * N, tmp.data());
+        for (int j = 0; j < nt; ++j)
+          if (A(j, i) != 0.0)
+            for (long k = 0; k < N; ++k)
+              qe[size_t(j) * N + k] += A(j, i) * tmp[k];
+      }
+    double J = 0.0;
+    for (size_t k = 0; k < e.size(); ++k)
+      J += e[k] * qe[k];
+    return 0.5 * J;
+  }
+  int oracle_forward_trajectory(void *h, int S, double *traj)
+  {
+    auto *ap = (AnyProblem *)h;
+    if (ap->fp32)
+      return -1;
+    Problem<double> &P = *ap->pd;
+    const long       N = P.fine.N();
+    const int        nt = P.nt;
+    std::vector<double> x(size_t(N) * nt, 0.0), prev(N);
+    for (int n = 0; n < S; ++n)
+      {
+        std::copy(x.begin() + size_t(nt - 1) * N, x.begin() + size_t(nt) * N, prev.begin());
+        if (P.solve_slab(x, prev, n * P.spec.nts * P.tau) != 0)
+          return -3;
+        std::copy(x.begin(), x.end(), traj + size_t(n) * N * nt);
+      }
+    return 0;
+  }
+  int oracle_adjoint_gradient(void *h, void *hT, const double *obs, int S, double *grad,
+                              double *misfit)
+  {
+    auto *ap = (AnyProblem *)h, *at = (AnyProblem *)hT;
+    if (ap->fp32 || at->fp32)
+      return -1;
+    Problem<double> &P = *ap->pd, &PT = *at->pd;
+    if (P.spec.type != DG || P.spec.problem != 0 || !PT.spec.transpose_time)
+      return -2;
+    const long N = P.fine.N();
+    const int  nt = P.nt;
+    const Space &sp = P.fine.sp;
+    std::vector<std::vector<double>> ck(S, std::vector<double>(N));
+    std::vector<double> x(size_t(N) * nt, 0.0), qe, lam(size_t(N) * nt), lr(N, 0.0), tmp(N);
+    double J = 0.0;
+    for (int n = 0; n < S; ++n)
+      {
+        std::copy(x.begin() + size_t(nt - 1) * N, x.begin() + size_t(nt) * N, ck[n].begin());
+        if (P.solve_slab(x, ck[n], n * P.spec.nts * P.tau) != 0)
+          return -3;
+        J += misfit_slab(P, x.data(), obs + size_t(n) * N * nt, qe);
+      }
+    const long nc = sp.g.n_cells();
+    std::fill(grad, grad + nc, 0.0);
+    Coefficient unit; // constant 1
+    unit.dim = sp.dim;
+    std::vector<long>   idx;
+    std::vector<double> Me, Ke;
+    const Mat          &A = P.fine.Alpha;
+    for (int n = S - 1; n >= 0; --n)
+      {
+        if (P.solve_slab(x, ck[n], n * P.spec.nts * P.tau) != 0)
+          return -3;
+        misfit_slab(P, x.data(), obs + size_t(n) * N * nt, qe);
+        if (n + 1 < S)
+          for (long k = 0; k < N; ++k)
+            qe[size_t(nt - 1) * N + k] += lr[k];
+        std::fill(lam.begin(), lam.end(), 0.0);
+        if (PT.fgmres(lam.data(), qe.data()) != 0)
+          return -4;
+        for (int j = 0; j < nt; ++j)
+          for (long k = 0; k < N; ++k)
+            if (sp.constrained[k])
+              lam[size_t(j) * N + k] = 0.0;
+        // w_j = sum_i Alpha(j,i) x_i
+        std::vector<double> w(size_t(N) * nt, 0.0);
+        for (int j = 0; j < nt; ++j)
+          for (int i = 0; i < nt; ++i)
+            if (A(j, i) != 0.0)
+              for (long k = 0; k < N; ++k)
+                w[size_t(j) * N + k] += A(j, i) * x[size_t(i) * N + k];
+        for (long c = 0; c < nc; ++c)
+          {
+            element_matrices(sp, P.sh, unit, 0, c, Me, Ke);
+            sp.cell_dofs(c, idx);
+            const int nl = (int)idx.size();
+            double    s  = 0.0;
+            for (int j = 0; j < nt; ++j)
+              for (int a = 0; a < nl; ++a)
+                {
+                  if (sp.constrained[idx[a]])
+                    continue;
+                  double r = 0.0;
+                  for (int b = 0; b < nl; ++b)
+                    if (!sp.constrained[idx[b]])
+                      r += Ke[size_t(a) * nl + b] * w[size_t(j) * N + idx[b]];
+                  s += lam[size_t(j) * N + idx[a]] * r;
+                }
+            const double kap = P.coef.kappa.empty() ? 1.0 : P.coef.kappa[c];
+            grad[c] -= kap * s;
+          }
+        // R^T lambda_n = sum_j Gamma_j M lambda_{n,j} (dG: R = Gamma (x) M)
+        std::fill(lr.begin(), lr.end(), 0.0);
+        for (int j = 0; j < nt; ++j)
+          if (P.rhsM(j, 0) != 0.0)
+            {
+              spmv_free(P.fine.M, P.fine.Mv, sp.constrained, lam.data() + size_t(j) * N, tmp.data());
+              for (long k = 0; k < N; ++k)
+                lr[k] += P.rhsM(j, 0) * tmp[k];
+            }
+      }
+    *misfit = J;
+    return 0;
+  }
+  int oracle_misfit(void *h, const double *obs, int S, double *misfit)
+  {
+    auto *ap = (AnyProblem *)h;
+    if (ap->fp32)
+      return -1;
+    Problem<double> &P = *ap->pd;
+    const long       N = P.fine.N();
+    const int        nt = P.nt;
+    std::vector<double> x(size_t(N) * nt, 0.0), prev(N), qe;
+    double J = 0.0;
+    for (int n = 0; n < S; ++n)
+      {
+        std::copy(x.begin() + size_t(nt - 1) * N, x.begin() + size_t(nt) * N, prev.begin());
+        if (P.solve_slab(x, prev, n * P.spec.nts * P.tau) != 0)
+          return -3;
+        J += misfit_slab(P, x.data(), obs + size_t(n) * N * nt, qe);
+      }
+    *misfit = J;
     return 0;
   }
   // Full heat run (tp_01.cc:630-683): stats = [slabs, total_its, L2, Linf,
