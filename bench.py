@@ -79,6 +79,106 @@ def workload_large():
     return spec, X, kappa
 
 
+def workload_wave(refine=5):
+    """3D acoustic wave u_tt - div(c^2 grad u) = 0 as the first-order-in-time
+    system of tp_01.cc:167-197: unit cube, 2^refine cells per side, Q2 space,
+    cG(2) time, tau = 1/8 (8 slabs to T = 1), c^2 = 1 for z < 0.5 and 4 above
+    (per cell), Gaussian pulse exp(-|x - x0|^2 / (2 * 0.05^2)) at a seeded
+    centre x0 in [0.3, 0.7]^3 as initial displacement, zero initial velocity,
+    no source; two pre- and two post-smoothing steps."""
+    import numpy as np
+    dim = 3
+    nc = [1 << refine] * dim
+    X = box_vertices(dim, nc, 0.0, seed=0)
+    zc = (np.arange(nc[2]) + 0.5) / nc[2]
+    kappa = np.repeat(np.where(zc < 0.5, 1.0, 4.0), nc[0] * nc[1])
+    centre = np.random.default_rng(2028).uniform(0.3, 0.7, dim)
+    spec = dict(dim=3, refine=refine, p=2, type=1, r=2, nts=1, r_min=1, tau=0.125,
+                end_time=1.0, problem=1, source=1, reltol=1e-8, abstol=1e-300, maxiter=200,
+                basis=30, vanka_factored=1, smoothing_steps=2)
+    return spec, X, kappa, centre
+
+
+def wave_initial(g, centre, width=0.05):
+    """Nodal interpolant of the Gaussian pulse (zero on the Dirichlet boundary)."""
+    import numpy as np
+    xyz = g.node_coordinates()
+    u0 = np.exp(-np.sum((xyz - centre) ** 2, axis=1) / (2 * width * width))
+    on_bnd = np.any((xyz < 1e-12) | (xyz > 1 - 1e-12), axis=1)
+    u0[on_bnd] = 0.0
+    return u0
+
+
+def run_wave(spec, X, kappa, centre, steps, warmup=1):
+    """steps wave slabs from the pulse after `warmup` untimed solves of the
+    first slab; returns (setup s, ms, its, converged, final u, v, dofs)."""
+    import torch
+    import stfem_amd as st
+    t0 = time.perf_counter()
+    g = st.Problem(spec, vertices=X, kappa=kappa)
+    setup_s = time.perf_counter() - t0
+    n, nt = g.n_nodes, g.nt
+    dev = "cuda:0"
+    u = torch.zeros(n * nt, dtype=torch.float64, device=dev)
+    v = torch.zeros_like(u)
+    u0 = torch.tensor(wave_initial(g, centre), dtype=torch.float64, device=dev)
+    pu, pv = u0.clone(), torch.zeros(n, dtype=torch.float64, device=dev)
+    for _ in range(warmup):
+        g.solve_slab_wave(u, v, pu, pv, 0.0)
+    pu.copy_(u0)
+    pv.zero_()
+    its, ok = 0, True
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        rc, it = g.solve_slab_wave(u, v, pu, pv, k * g.tau)
+        its += it
+        ok = ok and rc == 0
+        pu.copy_(u.view(n, nt)[:, nt - 1])
+        pv.copy_(v.view(n, nt)[:, nt - 1])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    log(f"wave mg_float={spec.get('mg_float')}: setup {setup_s:.1f} s, {steps} slabs "
+        f"{ms:.1f} ms, {its} FGMRES iterations{'' if ok else ' (not converged)'}")
+    out = (setup_s, ms, its, ok, u.clone(), v.clone(), n * nt)
+    g.close()
+    return out
+
+
+def main_wave(a):
+    """The 3D wave workload: FP32 multigrid against FP64, solution difference."""
+    import torch
+    spec, X, kappa, centre = workload_wave(a.refine)
+    for kv in a.set:
+        k, v = kv.split("=", 1)
+        spec[k] = int(v) if v.lstrip("-").isdigit() else float(v)
+    s32, ms32, its32, ok32, u32, v32, nst = run_wave(dict(spec, mg_float=1), X, kappa, centre,
+                                                    a.steps, a.warmup)
+    s64, ms64, its64, ok64, u64, v64, _ = run_wave(dict(spec, mg_float=0), X, kappa, centre,
+                                                  a.steps, a.warmup)
+    du = (torch.linalg.norm(u32 - u64) / torch.linalg.norm(u64)).item()
+    dv = (torch.linalg.norm(v32 - v64) / torch.linalg.norm(v64)).item()
+    out = {
+        "metric": "space-time DoF/s per solve to 1e-8 (FP32-MG vs FP64), 3D wave",
+        "value": a.steps * nst / (ms32 * 1e-3),
+        "unit": "space-time DoF/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms32 / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (FGMRES) + f32 (multigrid)", "data": "synthetic",
+        "config": {"workload": (f"3D wave, {1 << a.refine}^3 cells, Q2 space, cG(2) time, "
+                                f"c^2 = 1 | 4 split at z = 0.5, Gaussian pulse width 0.05, "
+                                f"v0 = 0, tol 1e-8"),
+                   "space_time_dofs_per_slab": nst},
+        "fp64_dofs_per_s": a.steps * nst / (ms64 * 1e-3),
+        "speedup_fp32mg_vs_fp64": ms64 / ms32,
+        "fgmres_its_fp32mg": its32, "fgmres_its_fp64": its64,
+        "converged": {"fp32mg": ok32, "fp64": ok64},
+        "setup_s": {"fp32mg": s32, "fp64": s64},
+        "fp32mg_vs_fp64_rel_l2": {"u": du, "v": dv},
+    }
+    print(json.dumps(out))
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -216,8 +316,10 @@ def main():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--refine", type=int, default=5)
-    ap.add_argument("--workload", choices=["default", "large"], default="default",
-                    help="large: 64^3 cells, Q4 space, dG(3) time (67.9M dofs per slab)")
+    ap.add_argument("--workload", choices=["default", "large", "wave"], default="default",
+                    help="large: 64^3 cells, Q4 space, dG(3) time (67.9M dofs per slab); "
+                         "wave: 3D wave, Q2/cG(2), two-layer c^2, Gaussian pulse (--refine 5: "
+                         "32^3 cells)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-pmc", action="store_true",
@@ -235,6 +337,8 @@ def main():
 
     import torch
     torch.cuda.synchronize()
+    if a.workload == "wave":
+        return main_wave(a)
     spec, X, kappa = workload(a.refine) if a.workload == "default" else workload_large()
     if a.dense_coarse_dofs is not None:
         spec["dense_coarse_dofs"] = a.dense_coarse_dofs

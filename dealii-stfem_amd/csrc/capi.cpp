@@ -366,6 +366,15 @@ extern "C"
     });
     return g != STFEM_OK ? g : rc;
   }
+  int stfem_node_coordinates(stfem_problem p, double *xyz)
+  {
+    return guarded([&] {
+      STFEM_REQUIRE(xyz, "null output");
+      std::vector<double> v;
+      P(p).node_coordinates_host(v);
+      std::copy(v.begin(), v.end(), xyz);
+    });
+  }
   int stfem_forward_trajectory(stfem_problem p, int n_slabs, double *traj, int *its)
   {
     return guarded([&] {

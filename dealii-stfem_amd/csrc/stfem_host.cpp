@@ -2389,12 +2389,37 @@ namespace stfem
       // by each cell's Q1 map; constrained nodes stay 0
       std::vector<double> interpolate_exact_v(double t) const
       {
-        const MeshLevel    &m = mesh_.back();
-        const GridDesc     &g = m.g;
-        const int           dim = g.dim, n = sh_.n;
+        const GridDesc     &g = mesh_.back().g;
+        const int           dim = g.dim;
         const double        PI = 3.14159265358979323846, f = spec.frequency;
-        const long          nv0 = g.nc[0] + 1, nv1 = g.nc[1] + 1;
+        std::vector<double> xyz;
+        node_coordinates_host(xyz);
         std::vector<double> out(g.n_nodes, 0.0);
+        for (long k = 0; k < g.n_nodes; ++k)
+          {
+            const long gx = k % g.nn[0], gy = (k / g.nn[0]) % g.nn[1],
+                       gz = dim == 3 ? k / (long(g.nn[0]) * g.nn[1]) : 0;
+            if (node_constrained(g, gx, gy, gz))
+              continue;
+            double val = 2 * PI * f * std::cos(2 * PI * f * t);
+            for (int d = 0; d < dim; ++d)
+              val *= std::sin(2 * PI * f * xyz[k * dim + d]);
+            out[k] = val;
+          }
+        return out;
+      }
+
+    public:
+      // support points of the fine FE_Q(p) nodes: Gauss-Lobatto points mapped
+      // by each cell's Q1 map (DoFTools::map_dofs_to_support_points); a node
+      // shared by several cells takes the value of the last cell visited
+      void node_coordinates_host(std::vector<double> &xyz) const override
+      {
+        const MeshLevel &m = mesh_.back();
+        const GridDesc  &g = m.g;
+        const int        dim = g.dim, n = sh_.n;
+        const long       nv0 = g.nc[0] + 1, nv1 = g.nc[1] + 1;
+        xyz.assign(size_t(g.n_nodes) * dim, 0.0);
         for (long c = 0; c < g.n_cells; ++c)
           {
             int cx, cy, cz;
@@ -2405,8 +2430,6 @@ namespace stfem
                   {
                     const long gx = (long)cx * sh_.p + a, gy = (long)cy * sh_.p + b,
                                gz = (long)cz * sh_.p + c3;
-                    if (node_constrained(g, gx, gy, gz))
-                      continue;
                     const double xi[3] = {sh_.gll[a], sh_.gll[b], dim == 3 ? sh_.gll[c3] : 0.0};
                     double       x[3] = {0, 0, 0};
                     for (int vtx = 0; vtx < (1 << dim); ++vtx)
@@ -2419,14 +2442,14 @@ namespace stfem
                         for (int d = 0; d < dim; ++d)
                           x[d] += N0 * m.X[id * dim + d];
                       }
-                    double val = 2 * PI * f * std::cos(2 * PI * f * t);
+                    const long k = gx + g.nn[0] * (gy + g.nn[1] * gz);
                     for (int d = 0; d < dim; ++d)
-                      val *= std::sin(2 * PI * f * x[d]);
-                    out[gx + g.nn[0] * (gy + g.nn[1] * gz)] = val;
+                      xyz[k * dim + d] = x[d];
                   }
           }
-        return out;
       }
+
+    private:
       // time profile g(t) of the source f(t,x) = g(t) h(x): the manufactured
       // heat source of exact_solution.h:53-75 splits this way; the other
       // sources are constant in time

@@ -471,6 +471,8 @@ namespace stfem
     // stats = {slabs, total iterations, L2, Linf, H1semi, n_nodes, nt}
     virtual int  run_heat(double *stats, int *its_per_slab, int cap) = 0;
     virtual void set_load_vector(const double *h_host) = 0;
+    // fine-level node coordinates (n_nodes x dim, host)
+    virtual void node_coordinates_host(std::vector<double> &xyz) const = 0;
     // ---- adjoint of the heat slab sequence (dG time), misfit
     //   J = 1/2 sum_n (x_n - d_n)^T (Alpha (x) M) (x_n - d_n)
     // over n_slabs slabs from zero initial values; obs and traj hold the

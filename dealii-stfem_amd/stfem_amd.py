@@ -62,6 +62,7 @@ _SIGS = {
     "stfem_misfit": ([_vp, _vp, ctypes.c_int, _dp, _ip], ctypes.c_int),
     "stfem_adjoint_gradient": ([_vp, _vp, ctypes.c_int, _vp, _dp, _ip], ctypes.c_int),
     "stfem_set_cell_coefficient": ([_vp, _dp], ctypes.c_int),
+    "stfem_node_coordinates": ([_vp, _dp], ctypes.c_int),
     "stfem_bytes_model": ([_vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
     "stfem_blocks_to_nodes": ([ctypes.c_long, ctypes.c_int, _vp, _vp], ctypes.c_int),
     "stfem_nodes_to_blocks": ([ctypes.c_long, ctypes.c_int, _vp, _vp], ctypes.c_int),
@@ -199,6 +200,7 @@ class Problem:
                                            device, ctypes.byref(h)))
         self.h = h
         self.device = device
+        self.dim = int(spec.get("dim", 2))
         info = np.zeros(6, np.int64)
         _check(load().stfem_problem_info(self.h, info.ctypes.data_as(_lp)))
         self.n_levels, self.nt, self.n_nodes, self.n_cells, self.level_size, self.nts = \
@@ -281,6 +283,12 @@ class Problem:
         return dict(rc=rc, slabs=n, total_its=int(stats[1]), L2=stats[2], Linf=stats[3],
                     H1=stats[4], ndofs=int(stats[5]), nblocks=int(stats[6]),
                     its=[per[i] for i in range(min(n, max_slabs))])
+
+    def node_coordinates(self):
+        """Support points of the fine nodes, (n_nodes, dim) host array."""
+        xyz = np.zeros(self.n_nodes * self.dim)
+        _check(load().stfem_node_coordinates(self.h, _p(xyz)))
+        return xyz.reshape(self.n_nodes, self.dim)
 
     # ---- adjoint of the heat slab sequence (dG)
     def forward_trajectory(self, n_slabs, traj):

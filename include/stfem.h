@@ -148,6 +148,11 @@ int stfem_slab_error(stfem_problem p, const double *x, const double *prev, doubl
  * dofs per slab}; its_per_slab (optional, cap entries) gets the iterations
  * of every slab.  Returns the first non-OK solver status. */
 int stfem_run_heat(stfem_problem p, double *stats, int *its_per_slab, int cap);
+/* support points of the fine nodes (host, n_nodes x dim, x fastest node
+ * numbering): the Gauss-Lobatto points of FE_Q(p) mapped by each cell's Q1
+ * map, as DoFTools::map_dofs_to_support_points gives them (tp_01.cc:415-438
+ * evaluates solutions there) */
+int stfem_node_coordinates(stfem_problem p, double *xyz);
 /* host array h (one value per fine node): source f(t,x) = h(x) (int phi_i)
  * with unit time profile (source=2) */
 int stfem_set_load_vector(stfem_problem p, const double *h);

@@ -140,3 +140,36 @@ def test_tp01_wave_transcript_on_gpu():
         n += 1
         P.close()
     assert exact >= int(0.7 * n), (exact, n)
+
+
+def test_wave_layered_pulse_matches_oracle():
+    """The 3D wave workload's setting at 4^3 cells (Q2, cG(2), c^2 = 1 | 4
+    split at z = 0.5, Gaussian pulse initial displacement, zero velocity, no
+    source): three slabs against the oracle's TimeIntegratorWave::solve,
+    displacements and velocities of every block."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), ".."))
+    import bench
+    spec, X, kappa, centre = bench.workload_wave(2)
+    spec = dict(spec, rhs=1, mg_float=1)
+    o, g = orc.Problem(spec, coords=X, kappa=kappa), st.Problem(spec, vertices=X, kappa=kappa)
+    n, nt = g.n_nodes, g.nt
+    u0 = bench.wave_initial(g, centre)
+    assert np.abs(u0).max() > 0.1
+    xo, vo = np.zeros(n * nt), np.zeros(n * nt)
+    xg = torch.zeros(n * nt, dtype=torch.float64, device=DEV)
+    vg = torch.zeros(n * nt, dtype=torch.float64, device=DEV)
+    pu_o, pv_o = u0.copy(), np.zeros(n)
+    for k in range(3):
+        pu = torch.tensor(pu_o, dtype=torch.float64, device=DEV)
+        pv = torch.tensor(pv_o, dtype=torch.float64, device=DEV)
+        rc_o, xo, vo, its_o = orc.solve_slab_wave(o, xo, pu_o, pv_o, k * g.tau)
+        print(f"[wave pulse slab {k}]", flush=True)
+        rc_g, its_g = g.solve_slab_wave(xg, vg, pu, pv, k * g.tau)
+        assert rc_o == 0 and rc_g == 0
+        assert abs(its_g - its_o) <= 1, (k, its_g, its_o)
+        assert rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo) < 1e-7, k
+        assert rel_err(nodes_to_blocks(vg.cpu().numpy(), nt), vo) < 1e-7, k
+        pu_o, pv_o = xo[(nt - 1) * n:].copy(), vo[(nt - 1) * n:].copy()
+    g.close()
