@@ -78,11 +78,13 @@ namespace stfem
     // (cell_slot layout, every local node including Dirichlet ones);
     // otherwise they are added to the node vector dst with atomics,
     // skipping Dirichlet nodes.
+    // vlds: (1 << DIM) * DIM values per cell slot, the cell's vertex
+    // coordinates (read by every thread of the cell in the geometry phase).
     template <int DIM, int N, int JB, typename T, bool TO_CELLS>
     __device__ __forceinline__ void st_apply_cells(const StOpArgs<T> &A, const T *__restrict__ src,
                                                    T *__restrict__ dst, int j0, int jn, long blk,
                                                    int tid, T *__restrict__ lds0,
-                                                   T *__restrict__ lds1)
+                                                   T *__restrict__ lds1, T *__restrict__ vlds)
     {
       using Ly = Layout<DIM, N>;
       using Tb = Tab<T>;
@@ -146,6 +148,24 @@ namespace stfem
           }
       }
       const int nti = A.nt_in, nto = A.nt_out;
+      // --- the cell's vertex coordinates into LDS (visible after the first
+      //     barrier of the evaluation loop)
+      constexpr int NVX = (1 << DIM) * DIM;
+      T *const      Vl  = vlds + lc * NVX;
+      if (lc < CPB)
+        for (int e = t; e < NVX; e += TPC) // TPC may be smaller than NVX
+          {
+            const int v = e / DIM, d = e - v * DIM;
+            T         x = T((v >> d) & 1);
+            if (valid)
+              {
+                const long nv0 = nc0 + 1, nv1 = nc1 + 1;
+                const long vx = cx + (v & 1), vy = cy + ((v >> 1) & 1);
+                const long vz = (DIM == 3) ? cz + ((v >> 2) & 1) : 0;
+                x = T(A.vertices[(vx + nv0 * (vy + nv1 * vz)) * DIM + d]);
+              }
+            Vl[e] = x;
+          }
 
       // --- accumulators at the quadrature points of the owned line
       T accV[JB][N];
@@ -288,29 +308,8 @@ namespace stfem
       // --- geometry at the quadrature points: JxW and the metric
       //     G = kappa * w * adj(J) adj(J)^T / |det J|
       {
-        T Vx[1 << DIM][DIM];
-        if (valid)
-          {
-            const long nv0 = nc0 + 1, nv1 = nc1 + 1;
-#pragma unroll
-            for (int v = 0; v < (1 << DIM); ++v)
-              {
-                const long vx = cx + (v & 1), vy = cy + ((v >> 1) & 1);
-                const long vz = (DIM == 3) ? cz + ((v >> 2) & 1) : 0;
-                const long id = vx + nv0 * (vy + nv1 * vz);
-#pragma unroll
-                for (int d = 0; d < DIM; ++d)
-                  Vx[v][d] = T(A.vertices[id * DIM + d]);
-              }
-          }
-        else
-          {
-#pragma unroll
-            for (int v = 0; v < (1 << DIM); ++v)
-#pragma unroll
-              for (int d = 0; d < DIM; ++d)
-                Vx[v][d] = T((v >> d) & 1);
-          }
+        // vertex v, coordinate d of the cell (LDS)
+        auto Vx = [&](int v, int d) -> T { return Vl[v * DIM + d]; };
         const T xa = Tb::X(N, ia), xb = Tb::X(N, ib);
 #pragma unroll
         for (int k = 0; k < N; ++k)
@@ -330,9 +329,9 @@ namespace stfem
 #pragma unroll
                       for (int f = 0; f < 2; ++f)
                         {
-                          s0 += (Vx[1 + 2 * e + 4 * f][d] - Vx[0 + 2 * e + 4 * f][d]) * wy[e] * wz[f];
-                          s1 += (Vx[e + 2 + 4 * f][d] - Vx[e + 4 * f][d]) * wx[e] * wz[f];
-                          s2 += (Vx[e + 2 * f + 4][d] - Vx[e + 2 * f][d]) * wx[e] * wy[f];
+                          s0 += (Vx(1 + 2 * e + 4 * f, d) - Vx(0 + 2 * e + 4 * f, d)) * wy[e] * wz[f];
+                          s1 += (Vx(e + 2 + 4 * f, d) - Vx(e + 4 * f, d)) * wx[e] * wz[f];
+                          s2 += (Vx(e + 2 * f + 4, d) - Vx(e + 2 * f, d)) * wx[e] * wy[f];
                         }
                     J[d][0] = s0;
                     J[d][1] = s1;
@@ -347,8 +346,8 @@ namespace stfem
 #pragma unroll
                 for (int d = 0; d < 2; ++d)
                   {
-                    J[d][0] = (Vx[1][d] - Vx[0][d]) * wy[0] + (Vx[3][d] - Vx[2][d]) * wy[1];
-                    J[d][1] = (Vx[2][d] - Vx[0][d]) * wx[0] + (Vx[3][d] - Vx[1][d]) * wx[1];
+                    J[d][0] = (Vx(1, d) - Vx(0, d)) * wy[0] + (Vx(3, d) - Vx(2, d)) * wy[1];
+                    J[d][1] = (Vx(2, d) - Vx(0, d)) * wx[0] + (Vx(3, d) - Vx(1, d)) * wx[1];
                   }
                 wq = Tb::W(N, ia) * Tb::W(N, k);
               }

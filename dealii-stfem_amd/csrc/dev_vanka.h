@@ -229,6 +229,24 @@ namespace stfem
                                           : (NL == 4 ? 2 : NL == 9 ? 3 : NL == 16 ? 4 : NL == 25 ? 5 : 6);
       static_assert((dim == 3 ? n * n * n : n * n) == NL, "unsupported patch size");
     };
+    // global node and inverse valence of local node a, patch shape known at
+    // compile time (the divisions by n become multiplications)
+    template <int NL>
+    __device__ __forceinline__ long node_ct(const PatchMap &M, int cx, int cy, int cz, int a,
+                                            float &inv_val)
+    {
+      using Sh         = V2Shape<NL>;
+      constexpr int n  = Sh::n, p = n - 1;
+      const int     la = a % n, lb = (a / n) % n, lc = (Sh::dim == 3) ? a / (n * n) : 0;
+      const long    gx = (long)cx * p + la, gy = (long)cy * p + lb, gz = (long)cz * p + lc;
+      int m = ((la == 0 && gx > 0) || (la == p && gx < M.nn0 - 1) ? 2 : 1) *
+              ((lb == 0 && gy > 0) || (lb == p && gy < M.nn1 - 1) ? 2 : 1);
+      if (Sh::dim == 3)
+        m *= ((lc == 0 && gz > 0) || (lc == p && gz < M.nn2 - 1) ? 2 : 1);
+      inv_val = 1.0f / float(m);
+      return gx + M.nn0 * (gy + M.nn1 * gz);
+    }
+
     template <int NL>
     constexpr int v2_groups()
     {
@@ -495,7 +513,7 @@ namespace stfem
               {
                 const int  a = l / NT, i = l - a * NT;
                 float      iv;
-                const long gn = M.node(cx, cy, cz, a, iv);
+                const long gn = node_ct<NL>(M, cx, cy, cz, a, iv);
                 v             = rhs[gn * NT + i];
                 if (ax)
                   v -= ax[gn * NT + i];
@@ -505,9 +523,22 @@ namespace stfem
         if (tid < NL)
           {
             const T *Wk = W + ((size_t)cell * NL + tid) * NT * NT;
+            if constexpr ((NT * NT) % VW == 0)
+              {
+                // W_k blocks are 16-byte aligned when NT*NT is a multiple of VW
 #pragma unroll
-            for (int e = 0; e < NT * NT; ++e)
-              b.w[e] = Wk[e];
+                for (int e = 0; e < NT * NT / VW; ++e)
+                  {
+                    const vec_t v = reinterpret_cast<const vec_t *>(Wk)[e];
+#pragma unroll
+                    for (int f = 0; f < VW; ++f)
+                      b.w[e * VW + f] = v[f];
+                  }
+              }
+            else
+#pragma unroll
+              for (int e = 0; e < NT * NT; ++e)
+                b.w[e] = Wk[e];
           }
       };
       auto process = [&](const Buf &b, long cell) {
@@ -624,7 +655,7 @@ namespace stfem
             M.cell_xyz(cell, cx, cy, cz);
             const int a = tid;
             float     iv;
-            (void)M.node(cx, cy, cz, a, iv);
+            (void)node_ct<NL>(M, cx, cy, cz, a, iv);
             const T   sc = omega * T(iv);
             const int la = a % n, lb = (a / n) % n, lc = (Sh::dim == 3) ? a / (n * n) : 0;
 #pragma unroll

@@ -25,8 +25,9 @@ namespace stfem
       using Ly = Layout<DIM, N>;
       __shared__ T lds0[Ly::CPB * Ly::NC];
       __shared__ T lds1[Ly::CPB * Ly::NC];
+      __shared__ T vlds[Ly::CPB * (1 << DIM) * DIM];
       st_apply_cells<DIM, N, JB, T, TO_CELLS>(A, src, dst, j0, jn, (long)blockIdx.x,
-                                              (int)threadIdx.x, lds0, lds1);
+                                              (int)threadIdx.x, lds0, lds1, vlds);
     }
 
     template <int DIM, int N, int NT, typename T, bool SKIP_DIR>
@@ -77,7 +78,8 @@ namespace stfem
     {
       using Ly              = Layout<DIM, N>;
       const int  nto        = a.nt_out;
-      const int  chunks     = (nto + 3) / 4;
+      const int  mj         = (a.max_jb >= 1 && a.max_jb <= 4) ? a.max_jb : 4;
+      const int  chunks     = (nto + mj - 1) / mj;
       const int  jb         = (nto + chunks - 1) / chunks;
       const long blocks     = (a.g.n_cells + Ly::CPB - 1) / Ly::CPB;
       const dim3 grid{(unsigned)blocks, 1, 1};

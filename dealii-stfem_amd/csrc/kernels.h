@@ -34,6 +34,7 @@ namespace stfem
     const T      *alpha;     // nt_out x nt_in (stiffness weights), device
     const T      *beta;      // nt_out x nt_in (mass weights), device
     int           nt_in, nt_out;
+    int           max_jb = 4; // output components per cell pass (1..4)
   };
   // cells != null: two-pass form with the cell-major scratch `cells` of
   // st_cells_size() values; cells == null: one pass with atomic adds.
@@ -57,6 +58,12 @@ namespace stfem
   template <typename T>
   void launch_vanka_explicit(const GridDesc &g, int nt, const T *binv, const T *rhs, const T *ax,
                              T *u, T omega, hipStream_t s);
+  // row stride of the factored Vanka's S blocks: rows padded to 16 bytes
+  inline int vanka_s_ld(int nloc, size_t tsize)
+  {
+    const int vw = int(16 / tsize);
+    return (nloc + vw - 1) / vw * vw;
+  }
   // Factored form: Binv_c = (I x S_c) blockdiag_k (W_{c,k}) (I x S_c^T),
   // S_c (n_c x n_c, row-major with the row stride vanka_s_ld(n_c, sizeof(T)),
   // padding zero), W_{c,k} (nt x nt row-major) per mode k.
@@ -64,15 +71,9 @@ namespace stfem
   // compile-time patch shapes, mode 0 u = correction, 1 u += correction;
   // otherwise (and for the other shapes) u += correction with atomics
   // (mode 0 then zero-fills u first).  variant: 0 automatic, 2 the
-  // LDS-staged kernel, 3 the register-tiled kernel (FP32, patches of at
-  // most 128 nodes), 4 its software-pipelined persistent form; shapes a
+  // LDS-staged kernel, 3 the register-tiled kernel (patches of at most 128
+  // nodes), 4 its software-pipelined persistent form (FP32); shapes a
   // variant does not cover fall back to the others.
-  // row stride of the factored Vanka's S blocks: rows padded to 16 bytes
-  inline int vanka_s_ld(int nloc, size_t tsize)
-  {
-    const int vw = int(16 / tsize);
-    return (nloc + vw - 1) / vw * vw;
-  }
   template <typename T>
   void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *W, const T *rhs,
                              const T *ax, T *u, T omega, T *cells, int mode, int variant,

@@ -488,6 +488,7 @@ namespace stfem
     args.beta       = b.get();
     args.nt_in      = nt_in_;
     args.nt_out     = nt_out_;
+    args.max_jb     = max_jb_;
     launch_st_apply<T>(args, src, dst, cells_.get(), mode, rhs, ctx_.stream);
   }
   template <typename T>
@@ -1732,6 +1733,8 @@ namespace stfem
       ps.problem = (kv["problem"] == "wave" || kv["problem"] == "1") ? 1 : 0;
     geti("vanka_device_setup", ps.vanka_device_setup);
     geti("vanka_kernel", ps.vanka_kernel);
+    geti("op_jb", ps.op_jb);
+    geti("op_jb64", ps.op_jb64);
     geti("smoothing_steps", ps.mg.smoothing_steps);
     getb("variable", ps.mg.variable);
     getb("estimate_relaxation", ps.mg.estimate_relaxation);
@@ -1837,6 +1840,7 @@ namespace stfem
         else
           {
             fine_ = std::make_unique<SystemMatrix<double>>(ctx_, F, w.Alpha, w.Beta, tp);
+            fine_->set_max_jb(spec.op_jb64);
             const DMat rK = (spec.type == TimeStepType::CGP) ? w.Gamma : zero;
             const DMat rM = (spec.type == TimeStepType::CGP) ? w.Zeta : w.Gamma;
             rhs_ = std::make_unique<SystemMatrix<double>>(ctx_, F, rK, rM, tp);
@@ -1940,6 +1944,7 @@ namespace stfem
             const MeshLevel &m = mesh_[sref[l]];
             levels[l].A  = std::make_unique<SystemMatrix<T>>(ctx_, m, w.Alpha, w.Beta,
                                                              spec.atomic_scatter == 0);
+            levels[l].A->set_max_jb(sizeof(T) == 4 ? spec.op_jb : spec.op_jb64);
             levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form,
                                                                   spec.vanka_device_setup != 0);
             levels[l].V->set_variant(spec.vanka_kernel);

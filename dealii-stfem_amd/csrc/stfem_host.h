@@ -163,6 +163,9 @@ namespace stfem
     void residual(T *dst, const T *rhs, const T *src) const;
     // replaces the coefficient by one value per cell (host array, n_cells)
     void set_cell_coefficient(const double *kappa);
+    // output time components per cell pass of the kernel (1..4): fewer
+    // means fewer registers and more passes over the inputs
+    void set_max_jb(int j) { max_jb_ = j; }
     long        n_nodes() const { return grid_.n_nodes; }
     int         n_blocks() const { return nt_out_; }
     const double *vertices() const { return verts_.get(); }
@@ -184,6 +187,7 @@ namespace stfem
     DBuf<T>      coef_;
     DMat         A_, B_;
     int          nt_in_, nt_out_;
+    int          max_jb_ = 4;
     DBuf<T>      dA_, dB_, dAt_, dBt_;
     mutable DBuf<T> cells_; // cell-major results of the two-pass form
   };
@@ -431,6 +435,7 @@ namespace stfem
     int          coarse_coef = 0; // coarse-level coefficient: 0 sampled, 1 geometric, 2 harmonic, 3 arithmetic mean
     int          vanka_device_setup = 1; // factored Vanka set up on the GPU
     int          vanka_kernel = 0; // factored sweep kernel: 0 automatic, 2 LDS-staged, 3 register-tiled
+    int          op_jb = 4, op_jb64 = 4; // output components per operator cell pass (levels, FP64 fine)
     GMGOptions   mg;
     SolverControlData solver;
     int          extrapolate = 1;
