@@ -62,11 +62,13 @@ def workload_large():
     """Spec, vertices and coefficient of the large 3D heat workload: 64^3
     cells perturbed by up to 0.1h, Q4 space, dG(3) time (4 time dofs per
     slab, 67.9M space-time dofs per slab), T = 1 in 4 slabs, cellwise
-    log-uniform coefficient in [1e-3, 1e3], seeded Gaussian bumps.  Two pre-
-    and two post-smoothing steps; the coarse levels take the arithmetic mean
-    of the finest-level coefficient over each quadrature point's share of
-    the cell (with the point-sampled coefficient the multigrid does not
-    converge for this contrast)."""
+    log-uniform coefficient in [1e-3, 1e3], seeded Gaussian bumps.  Three pre-
+    and three post-smoothing steps (35 iterations per slab instead of 51 with
+    two, 6 % less time); the coarse levels take the arithmetic mean of the
+    finest-level coefficient over each quadrature point's share of the cell
+    (with the point-sampled coefficient the multigrid does not converge for
+    this contrast); the V-cycle from the 4^3-cell level down (19,652 DoFs) is
+    one dense operator."""
     import numpy as np
     dim, refine = 3, 6
     nc = [1 << refine] * dim
@@ -75,7 +77,8 @@ def workload_large():
     kappa = np.exp(rng.uniform(np.log(1e-3), np.log(1e3), int(np.prod(nc))))
     spec = dict(dim=3, refine=refine, p=4, type=2, r=3, nts=1, tau=0.25, end_time=1.0,
                 source=3, n_bumps=4, seed=7, reltol=1e-8, abstol=1e-300, maxiter=200,
-                basis=30, vanka_factored=1, smoothing_steps=2, coarse_coef=3)
+                basis=30, vanka_factored=1, smoothing_steps=3, coarse_coef=3,
+                dense_coarse_dofs=20000)
     return spec, X, kappa
 
 
@@ -123,8 +126,10 @@ def run_wave(spec, X, kappa, centre, steps, warmup=1):
     v = torch.zeros_like(u)
     u0 = torch.tensor(wave_initial(g, centre), dtype=torch.float64, device=dev)
     pu, pv = u0.clone(), torch.zeros(n, dtype=torch.float64, device=dev)
-    for _ in range(warmup):
+    for _ in range(warmup):  # the loop body once, device code loaded outside the timing
         g.solve_slab_wave(u, v, pu, pv, 0.0)
+        pu.copy_(u.view(n, nt)[:, nt - 1])
+        pv.copy_(v.view(n, nt)[:, nt - 1])
     pu.copy_(u0)
     pv.zero_()
     its, ok = 0, True
