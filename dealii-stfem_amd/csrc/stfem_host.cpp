@@ -723,15 +723,15 @@ namespace stfem
   template <typename T>
   PreconditionVanka<T>::PreconditionVanka(Context &ctx, const MeshLevel &m, const Shape1D &sh,
                                           const DMat &Alpha, const DMat &Beta, Form form,
-                                          bool device_setup)
+                                          int device_setup)
     : ctx_(ctx)
     , grid_(m.g)
     , nt_(Alpha.m)
     , form_(form)
   {
-    if (form_ == Form::factored && device_setup && nt_ <= 8)
+    if (form_ == Form::factored && device_setup != 0 && nt_ <= 8)
       {
-        setup_factored_on_device(m, sh, Alpha, Beta);
+        setup_factored_on_device(m, sh, Alpha, Beta, device_setup == 2);
         return;
       }
     std::vector<double> Kp, Mp;
@@ -907,7 +907,8 @@ namespace stfem
   // Beta)^{-1} in the level precision.
   template <typename T>
   void PreconditionVanka<T>::setup_factored_on_device(const MeshLevel &m, const Shape1D &sh,
-                                                      const DMat &Alpha, const DMat &Beta)
+                                                      const DMat &Alpha, const DMat &Beta,
+                                                      bool divide_conquer)
   {
     const GridDesc &g  = m.g;
     const int       nl = (g.dim == 3) ? sh.n * sh.n * sh.n : sh.n * sh.n;
@@ -935,7 +936,7 @@ namespace stfem
     const int  lper     = int(std::max<long>(1, std::min<long>(nlay, target / layer)));
     const long emax     = std::min<long>(nlay, lper + 2) * layer, pmax = long(lper) * layer;
     DBuf<double> Me{size_t(emax) * bs}, Ke{size_t(emax) * bs}, Mp{size_t(pmax) * bs},
-      Kp{size_t(pmax) * bs}, lam{size_t(pmax) * nl}, res{size_t(pmax)};
+      Kp{size_t(pmax) * bs}, lam{size_t(pmax) * nl}, res{size_t(pmax) * nl};
     DBuf<int>     sweeps{size_t(pmax)}, info{size_t(pmax)};
     rocblas_handle h = nullptr;
     if (rocblas_create_handle(&h) != rocblas_status_success)
@@ -954,10 +955,19 @@ namespace stfem
                                 sh.Sd.data(), sh.qx.data(), sh.qw.data(), e0, ne, Me.get(),
                                 Ke.get(), s);
         launch_patch_matrices(g, p0, np, e0, Me.get(), Ke.get(), Mp.get(), Kp.get(), s);
-        const rocblas_status st = rocsolver_dsygvj_strided_batched(
-          h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, nl, Kp.get(), nl,
-          rocblas_stride(bs), Mp.get(), nl, rocblas_stride(bs), 0.0, res.get(), 100,
-          sweeps.get(), lam.get(), nl, info.get(), rocblas_int(np));
+        // K_c z = lambda M_c z with z^T M_c z = 1: eigenvectors overwrite Kp
+        const rocblas_status st =
+          divide_conquer ?
+            rocsolver_dsygvd_strided_batched(h, rocblas_eform_ax, rocblas_evect_original,
+                                             rocblas_fill_upper, nl, Kp.get(), nl,
+                                             rocblas_stride(bs), Mp.get(), nl, rocblas_stride(bs),
+                                             lam.get(), nl, res.get(), nl, info.get(),
+                                             rocblas_int(np)) :
+            rocsolver_dsygvj_strided_batched(h, rocblas_eform_ax, rocblas_evect_original,
+                                             rocblas_fill_upper, nl, Kp.get(), nl,
+                                             rocblas_stride(bs), Mp.get(), nl, rocblas_stride(bs),
+                                             0.0, res.get(), 100, sweeps.get(), lam.get(), nl,
+                                             info.get(), rocblas_int(np));
         if (st != rocblas_status_success)
           throw Error(STATUS_ERR_INTERNAL,
                       std::string("rocSOLVER sygvj: ") + rocblas_status_to_string(st));
@@ -1946,7 +1956,7 @@ namespace stfem
                                                              spec.atomic_scatter == 0);
             levels[l].A->set_max_jb(sizeof(T) == 4 ? spec.op_jb : spec.op_jb64);
             levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form,
-                                                                  spec.vanka_device_setup != 0);
+                                                                  spec.vanka_device_setup);
             levels[l].V->set_variant(spec.vanka_kernel);
             levels[l].r  = rl[l];
             levels[l].nts = nl[l];

@@ -203,10 +203,10 @@ namespace stfem
       factored
     };
     // device_setup: the factored form is set up on the GPU (element and
-    // patch matrices, batched generalized eigensolver); otherwise, and for
-    // the explicit inverse, on the host
+    // patch matrices, batched generalized eigensolver: 1 Jacobi, 2 divide
+    // and conquer); 0, and the explicit inverse: on the host
     PreconditionVanka(Context &ctx, const MeshLevel &m, const Shape1D &sh, const DMat &Alpha,
-                      const DMat &Beta, Form form, bool device_setup = true);
+                      const DMat &Beta, Form form, int device_setup = 2);
     // dst = sum_c (1/valence) B_c^{-1} src|_c   (gmg.h:476-514)
     void   vmult(T *dst, const T *src) const;
     // u += omega * Vanka(rhs - ax); ax may be null.  One relaxation sweep of
@@ -221,7 +221,7 @@ namespace stfem
 
   private:
     void     setup_factored_on_device(const MeshLevel &m, const Shape1D &sh, const DMat &Alpha,
-                                      const DMat &Beta);
+                                      const DMat &Beta, bool divide_conquer);
     Context &ctx_;
     GridDesc grid_;
     int      nt_;
@@ -433,7 +433,7 @@ namespace stfem
     int          atomic_scatter = 0; // 1: one-pass operator with atomic adds
     int          problem = 0; // 0 heat, 1 wave (first order in time, tp_01.cc:167-197)
     int          coarse_coef = 0; // coarse-level coefficient: 0 sampled, 1 geometric, 2 harmonic, 3 arithmetic mean
-    int          vanka_device_setup = 1; // factored Vanka set up on the GPU
+    int          vanka_device_setup = 2; // factored Vanka on the GPU: 1 Jacobi, 2 divide-and-conquer eigensolver
     int          vanka_kernel = 0; // factored sweep kernel: 0 automatic, 2 LDS-staged, 3 register-tiled
     int          op_jb = 4, op_jb64 = 4; // output components per operator cell pass (levels, FP64 fine)
     GMGOptions   mg;

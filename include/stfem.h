@@ -86,8 +86,13 @@ int stfem_time_mg_sequence(int n_sp_lvl, int k_max, int k_min, int nts, int nts_
  * source, 1 none, 2 load vector set by stfem_set_load_vector, 3 seeded
  * Gaussian bumps with n_bumps, seed), dirichlet (face bit mask), problem
  * (heat, or wave: the first-order-in-time wave system of tp_01.cc:167-197,
- * solved with stfem_solve_slab_wave), atomic_scatter, vanka_device_setup,
- * dense_coarse_dofs.
+ * solved with stfem_solve_slab_wave), atomic_scatter, vanka_device_setup
+ * (0 host, 1 device with the batched Jacobi generalized eigensolver, 2
+ * device with divide and conquer), dense_coarse_dofs, coarse_coef (0 the
+ * coarse-level coefficient sampled at the quadrature points, 1/2/3 its
+ * geometric/harmonic/arithmetic mean over each point's share of the cell),
+ * vanka_kernel (0 automatic, 2 LDS-staged, 3 register-tiled, 4 pipelined),
+ * op_jb/op_jb64 (output components per operator pass, levels/FP64 fine).
  * vertices: finest-level vertex coordinates ((nc+1)^dim points, dim each,
  * x fastest) or NULL for the uniform box; cell_coef: finest-level per-cell
  * coefficient or NULL for 1. */
