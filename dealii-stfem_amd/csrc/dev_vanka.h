@@ -78,6 +78,100 @@ namespace stfem
       return m;
     }
 
+    // z = (lam Alpha + Beta)^{-1} y for one eigenmode of the factored
+    // inverse (W_k applied without storing it): Gaussian elimination with
+    // partial pivoting on the NT x NT time block, fully unrolled so that the
+    // block stays in registers (row exchanges are selects).
+    template <typename T, int NT>
+    __device__ __forceinline__ void solve_time_block(T lam, const T *__restrict__ ta,
+                                                     const T *__restrict__ tb, const T *y, T *z)
+    {
+      T M[NT][NT + 1];
+#pragma unroll
+      for (int i = 0; i < NT; ++i)
+        {
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            M[i][j] = lam * ta[i * NT + j] + tb[i * NT + j];
+          M[i][NT] = y[i];
+        }
+#pragma unroll
+      for (int c = 0; c < NT; ++c)
+        {
+#pragma unroll
+          for (int q = c + 1; q < NT; ++q)
+            {
+              const bool sw = fabs(M[q][c]) > fabs(M[c][c]);
+#pragma unroll
+              for (int j = c; j <= NT; ++j)
+                {
+                  const T a = M[c][j], b = M[q][j];
+                  M[c][j] = sw ? b : a;
+                  M[q][j] = sw ? a : b;
+                }
+            }
+          const T inv = T(1) / M[c][c];
+#pragma unroll
+          for (int q = c + 1; q < NT; ++q)
+            {
+              const T f = M[q][c] * inv;
+#pragma unroll
+              for (int j = c + 1; j <= NT; ++j)
+                M[q][j] -= f * M[c][j];
+            }
+        }
+#pragma unroll
+      for (int i = NT - 1; i >= 0; --i)
+        {
+          T s = M[i][NT];
+#pragma unroll
+          for (int j = i + 1; j < NT; ++j)
+            s -= M[i][j] * z[j];
+          z[i] = s / M[i][i];
+        }
+    }
+    // the same for a run-time block size nt <= 16 (fallback shapes)
+    template <typename T>
+    __device__ inline void solve_time_block_rt(int nt, T lam, const T *__restrict__ ta,
+                                               const T *__restrict__ tb, const T *y, T *z)
+    {
+      T M[16][17];
+      for (int i = 0; i < nt; ++i)
+        {
+          for (int j = 0; j < nt; ++j)
+            M[i][j] = lam * ta[i * nt + j] + tb[i * nt + j];
+          M[i][nt] = y[i];
+        }
+      for (int c = 0; c < nt; ++c)
+        {
+          int r = c;
+          for (int q = c + 1; q < nt; ++q)
+            if (fabs(M[q][c]) > fabs(M[r][c]))
+              r = q;
+          if (r != c)
+            for (int j = c; j <= nt; ++j)
+              {
+                const T t = M[c][j];
+                M[c][j]   = M[r][j];
+                M[r][j]   = t;
+              }
+          const T inv = T(1) / M[c][c];
+          for (int q = c + 1; q < nt; ++q)
+            {
+              const T f = M[q][c] * inv;
+              for (int j = c + 1; j <= nt; ++j)
+                M[q][j] -= f * M[c][j];
+            }
+        }
+      for (int i = nt - 1; i >= 0; --i)
+        {
+          T s = M[i][nt];
+          for (int j = i + 1; j < nt; ++j)
+            s -= M[i][j] * z[j];
+          z[i] = s / M[i][i];
+        }
+    }
+
     inline int patch_size(const GridDesc &g)
     {
       return (g.dim == 3) ? (g.p + 1) * (g.p + 1) * (g.p + 1) : (g.p + 1) * (g.p + 1);
@@ -141,7 +235,9 @@ namespace stfem
     template <typename T, bool STAGE>
     __device__ __forceinline__ void vanka_factored_cell(const PatchMap &M, int nt, int nloc,
                                                         int ld, const T *__restrict__ S,
-                                                        const T *__restrict__ W,
+                                                        const T *__restrict__ lam,
+                                                        const T *__restrict__ ta,
+                                                        const T *__restrict__ tb,
                                                         const T *__restrict__ rhs,
                                                         const T *__restrict__ ax,
                                                         T *__restrict__ u, T omega, long cell,
@@ -163,7 +259,7 @@ namespace stfem
       __syncthreads();
       const T  *Sc  = STAGE ? ss : Sg;
       const int sst = STAGE ? nloc : ld;
-      const T *Wc = W + (size_t)cell * nloc * nt * nt;
+      const T *lc = lam + (size_t)cell * nloc;
       // y_{i,k} = sum_a S[a][k] x_{i,a}  then  z_{:,k} = W_k y_{:,k}
       for (int k = tid; k < nloc; k += nthr)
         {
@@ -176,14 +272,10 @@ namespace stfem
               for (int i = 0; i < nt; ++i)
                 yk[i] += s * xs[i * nloc + a];
             }
-          const T *Wk = Wc + (size_t)k * nt * nt;
+          T zk[16];
+          solve_time_block_rt<T>(nt, lc[k], ta, tb, yk, zk);
           for (int i = 0; i < nt; ++i)
-            {
-              T z = T(0);
-              for (int j = 0; j < nt; ++j)
-                z += Wk[i * nt + j] * yk[j];
-              ys[i * nloc + k] = z;
-            }
+            ys[i * nloc + k] = zk[i];
         }
       __syncthreads();
       // out_{i,a} = sum_k S[a][k] z_{i,k}
@@ -213,8 +305,9 @@ namespace stfem
     // stored with rows padded to 16 bytes and copied to LDS with 16-byte
     // loads, all of a thread's loads issued before its first LDS store; the
     // LDS row stride is odd (NL, or NL+1 for even NL) so that the strided
-    // row reads of S z are conflict-free.  W_k is prefetched into registers
-    // with S.  The
+    // row reads of S z are conflict-free.  The eigenvalue lambda_k is loaded
+    // with S; W_k = (lambda_k Alpha + Beta)^{-1} is applied by solving the
+    // time block.  The
     // local vectors are node-major ([a][i]) so a node's NT values are one
     // broadcast read.  Four barriers.
     // TWO_PASS: the corrections omega/valence * B_c^{-1} r_c go to the
@@ -272,7 +365,9 @@ namespace stfem
 
     template <typename T, int NL, int NT, bool TWO_PASS>
     __device__ __forceinline__ void vanka_v2_cell(const PatchMap &M, const T *__restrict__ S,
-                                                  const T *__restrict__ W,
+                                                  const T *__restrict__ lam,
+                                                  const T *__restrict__ ta,
+                                                  const T *__restrict__ tb,
                                                   const T *__restrict__ rhs,
                                                   const T *__restrict__ ax, T *__restrict__ u,
                                                   T *__restrict__ cells, T omega, long cell,
@@ -290,15 +385,8 @@ namespace stfem
 
       int cx, cy, cz;
       M.cell_xyz(cell, cx, cy, cz);
-      // W_k of this thread's k
-      T wk[NT * NT];
-      if (tid < NL)
-        {
-          const T *Wk = W + ((size_t)cell * NL + tid) * NT * NT;
-#pragma unroll
-          for (int e = 0; e < NT * NT; ++e)
-            wk[e] = Wk[e];
-        }
+      // eigenvalue of this thread's mode k
+      const T lk = (tid < NL) ? lam[(size_t)cell * NL + tid] : T(0);
       // S into LDS: 16-byte loads of the row-padded block (row stride LDG),
       // all of a thread's loads issued before its first LDS store
       {
@@ -381,15 +469,11 @@ namespace stfem
                 s += red[(q * NT + i) * NL + k];
               y[i] = s;
             }
+          T z[NT];
+          solve_time_block<T, NT>(lk, ta, tb, y, z);
 #pragma unroll
           for (int i = 0; i < NT; ++i)
-            {
-              T z = T(0);
-#pragma unroll
-              for (int j = 0; j < NT; ++j)
-                z += wk[i * NT + j] * y[j];
-              ys[k * NT + i] = z;
-            }
+            ys[k * NT + i] = z[i];
         }
       __syncthreads();
       // partial out_{a,i} = sum_{k in slice} S[a][k] z_{k,i}
@@ -441,7 +525,7 @@ namespace stfem
     // from the row-padded layout (16 lanes read 256 contiguous bytes of a
     // row); S takes no LDS, so several cells share a CU.  !PIPE: one cell
     // per workgroup.  PIPE: a persistent workgroup walks a contiguous range
-    // of cells and loads the next cell's S tile, residual values and W_k
+    // of cells and loads the next cell's S tile, residual values and lambda_k
     // into a second register set while the current cell is processed (twice
     // the registers, one workgroup per CU).  Partial sums over the 16 tiles of a column
     // (of a row) meet in LDS laid out [tile][i][index], which makes the
@@ -466,7 +550,9 @@ namespace stfem
     template <typename T, int NL, int NT, bool PIPE>
     __device__ __forceinline__ void vanka_v3_range(const PatchMap &M, long n_cells,
                                                    const T *__restrict__ S,
-                                                   const T *__restrict__ W,
+                                                   const T *__restrict__ lam,
+                                                   const T *__restrict__ talpha,
+                                                   const T *__restrict__ tbeta,
                                                    const T *__restrict__ rhs,
                                                    const T *__restrict__ ax, T *__restrict__ cells,
                                                    T omega, int tid, T *smem)
@@ -487,7 +573,7 @@ namespace stfem
       {
         vec_t s[R][H];
         T     x[NX];
-        T     w[NT * NT];
+        T     lk; // eigenvalue of mode k = tid
       };
       auto load = [&](Buf &b, long cell) {
         const T *Sg = S + (size_t)cell * NL * LDG;
@@ -520,26 +606,7 @@ namespace stfem
               }
             b.x[j] = v;
           }
-        if (tid < NL)
-          {
-            const T *Wk = W + ((size_t)cell * NL + tid) * NT * NT;
-            if constexpr ((NT * NT) % VW == 0)
-              {
-                // W_k blocks are 16-byte aligned when NT*NT is a multiple of VW
-#pragma unroll
-                for (int e = 0; e < NT * NT / VW; ++e)
-                  {
-                    const vec_t v = reinterpret_cast<const vec_t *>(Wk)[e];
-#pragma unroll
-                    for (int f = 0; f < VW; ++f)
-                      b.w[e * VW + f] = v[f];
-                  }
-              }
-            else
-#pragma unroll
-              for (int e = 0; e < NT * NT; ++e)
-                b.w[e] = Wk[e];
-          }
+        b.lk = (tid < NL) ? lam[(size_t)cell * NL + tid] : T(0);
       };
       auto process = [&](const Buf &b, long cell) {
 #pragma unroll
@@ -604,15 +671,11 @@ namespace stfem
                   acc += red[(q * NT + i) * KP + k];
                 y[i] = acc;
               }
+            T z[NT];
+            solve_time_block<T, NT>(b.lk, talpha, tbeta, y, z);
 #pragma unroll
             for (int i = 0; i < NT; ++i)
-              {
-                T z = T(0);
-#pragma unroll
-                for (int j = 0; j < NT; ++j)
-                  z += b.w[i * NT + j] * y[j];
-                ys[i * KP + k] = z;
-              }
+              ys[i * KP + k] = z[i];
           }
         else if (tid < KP)
 #pragma unroll

@@ -222,15 +222,15 @@ namespace stfem
         }
     }
 
-    // S (row-major, eigenvector k in column k) and W_k = (lambda_k Alpha +
-    // Beta)^{-1} (Gauss-Jordan with partial pivoting) of the cells of a
-    // chunk; Z: column-major eigenvectors, lam: eigenvalues; one thread per
-    // (cell, k).
+    // S (row-major, eigenvector k in column k) and lambda_k in the level
+    // precision for the cells of a chunk, with a check that the time block
+    // lambda_k Alpha + Beta is invertible; Z: column-major eigenvectors, lam:
+    // eigenvalues; one thread per (cell, k).
     template <typename T>
     __global__ void k_pack_factors(long ncell, int nl, int ld, int nt, const double *__restrict__ Z,
                                    const double *__restrict__ lam, const double *__restrict__ Alpha,
                                    const double *__restrict__ Beta, T *__restrict__ S,
-                                   T *__restrict__ W, int *__restrict__ singular)
+                                   T *__restrict__ lam_out, int *__restrict__ singular)
     {
       const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
       if (id >= ncell * nl)
@@ -245,14 +245,14 @@ namespace stfem
         for (int i = 0; i < nl; ++i)
           for (int kk = nl; kk < ld; ++kk)
             Sc[size_t(i) * ld + kk] = T(0);
-      double       A[8][8], I[8][8];
       const double l = lam[size_t(c) * nl + k];
+      lam_out[size_t(c) * nl + k] = T(l);
+      // the time block lambda_k Alpha + Beta must be invertible: elimination
+      // with partial pivoting, a zero pivot flags the patch
+      double A[8][8];
       for (int i = 0; i < nt; ++i)
         for (int j = 0; j < nt; ++j)
-          {
-            A[i][j] = l * Alpha[i * nt + j] + Beta[i * nt + j];
-            I[i][j] = (i == j) ? 1.0 : 0.0;
-          }
+          A[i][j] = l * Alpha[i * nt + j] + Beta[i * nt + j];
       for (int col = 0; col < nt; ++col)
         {
           int    r    = col;
@@ -268,36 +268,17 @@ namespace stfem
           if (r != col)
             for (int j = 0; j < nt; ++j)
               {
-                double t = A[r][j];
-                A[r][j]  = A[col][j];
-                A[col][j] = t;
-                t         = I[r][j];
-                I[r][j]   = I[col][j];
-                I[col][j] = t;
+                const double t = A[r][j];
+                A[r][j]         = A[col][j];
+                A[col][j]       = t;
               }
-          const double d = 1.0 / A[col][col];
-          for (int j = 0; j < nt; ++j)
+          for (int q = col + 1; q < nt; ++q)
             {
-              A[col][j] *= d;
-              I[col][j] *= d;
+              const double f = A[q][col] / A[col][col];
+              for (int j = col; j < nt; ++j)
+                A[q][j] -= f * A[col][j];
             }
-          for (int q = 0; q < nt; ++q)
-            if (q != col)
-              {
-                const double f = A[q][col];
-                if (f == 0.0)
-                  continue;
-                for (int j = 0; j < nt; ++j)
-                  {
-                    A[q][j] -= f * A[col][j];
-                    I[q][j] -= f * I[col][j];
-                  }
-              }
         }
-      T *Wk = W + (size_t(c) * nl + k) * nt * nt;
-      for (int i = 0; i < nt; ++i)
-        for (int j = 0; j < nt; ++j)
-          Wk[i * nt + j] = T(I[i][j]);
     }
 
     template <int DIM, int N>
@@ -377,15 +358,15 @@ namespace stfem
 
   template <typename T>
   void launch_pack_factors(long ncell, int nl, int nt, const double *Z, const double *lam,
-                           const double *Alpha, const double *Beta, T *S, T *W, int *singular,
-                           hipStream_t s)
+                           const double *Alpha, const double *Beta, T *S, T *lam_out,
+                           int *singular, hipStream_t s)
   {
     STFEM_REQUIRE(nt >= 1 && nt <= 8, "pack factors: at most 8 time dofs");
     const long n = ncell * nl;
     if (n == 0)
       return;
     k_pack_factors<T><<<dim3((unsigned)((n + 127) / 128)), 128, 0, s>>>(
-      ncell, nl, vanka_s_ld(nl, sizeof(T)), nt, Z, lam, Alpha, Beta, S, W, singular);
+      ncell, nl, vanka_s_ld(nl, sizeof(T)), nt, Z, lam, Alpha, Beta, S, lam_out, singular);
     STFEM_HIP(hipGetLastError());
   }
   template void launch_pack_factors<double>(long, int, int, const double *, const double *,

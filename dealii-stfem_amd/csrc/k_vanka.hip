@@ -8,8 +8,8 @@
 // Explicit form: one workgroup per cell, B_c^{-1} stored column-major so a
 // column is read by consecutive lanes (coalesced); the gathered patch sits in
 // LDS.  Factored form: B_c^{-1} = (I x S)(blockdiag_k W_k)(I x S^T) with
-// S^T M_c S = I, S^T K_c S = diag(lambda), W_k = (lambda_k Alpha + Beta)^{-1};
-// S is staged into LDS with coalesced loads.  The per-cell device code is in
+// S^T M_c S = I, S^T K_c S = diag(lambda), W_k = (lambda_k Alpha + Beta)^{-1}
+// applied by solving the nt x nt time block from lambda_k.  The per-cell device code is in
 // dev_vanka.h.
 #include "dev_vanka.h"
 
@@ -35,23 +35,27 @@ namespace stfem
     template <typename T, bool STAGE>
     __global__ void __launch_bounds__(256)
       k_vanka_factored(const PatchMap M, int nt, int nloc, int ld, const T *__restrict__ S,
-                       const T *__restrict__ W, const T *__restrict__ rhs,
+                       const T *__restrict__ lam, const T *__restrict__ ta,
+                       const T *__restrict__ tb, const T *__restrict__ rhs,
                        const T *__restrict__ ax, T *__restrict__ u, T omega)
     {
       extern __shared__ unsigned char smem_raw[];
-      vanka_factored_cell<T, STAGE>(M, nt, nloc, ld, S, W, rhs, ax, u, omega, (long)blockIdx.x,
+      vanka_factored_cell<T, STAGE>(M, nt, nloc, ld, S, lam, ta, tb, rhs, ax, u, omega,
+                                    (long)blockIdx.x,
                                     (int)threadIdx.x, (int)blockDim.x,
                                     reinterpret_cast<T *>(smem_raw));
     }
 
     template <typename T, int NL, int NT, bool TWO_PASS>
     __global__ void __launch_bounds__(256)
-      k_vanka_factored_v2(const PatchMap M, const T *__restrict__ S, const T *__restrict__ W,
+      k_vanka_factored_v2(const PatchMap M, const T *__restrict__ S, const T *__restrict__ lam,
+                          const T *__restrict__ ta, const T *__restrict__ tb,
                           const T *__restrict__ rhs, const T *__restrict__ ax,
                           T *__restrict__ u, T *__restrict__ cells, T omega)
     {
       extern __shared__ __align__(16) unsigned char smem_raw[];
-      vanka_v2_cell<T, NL, NT, TWO_PASS>(M, S, W, rhs, ax, u, cells, omega, (long)blockIdx.x,
+      vanka_v2_cell<T, NL, NT, TWO_PASS>(M, S, lam, ta, tb, rhs, ax, u, cells, omega,
+                                         (long)blockIdx.x,
                                          (int)threadIdx.x, reinterpret_cast<T *>(smem_raw));
     }
 
@@ -59,11 +63,13 @@ namespace stfem
     template <typename T, int NL, int NT, bool PIPE>
     __global__ void __launch_bounds__(256, PIPE ? 1 : (sizeof(T) == 4 ? 4 : 2))
       k_vanka_factored_v3(const PatchMap M, long n_cells, const T *__restrict__ S,
-                          const T *__restrict__ W, const T *__restrict__ rhs,
+                          const T *__restrict__ lam, const T *__restrict__ ta,
+                          const T *__restrict__ tb, const T *__restrict__ rhs,
                           const T *__restrict__ ax, T *__restrict__ cells, T omega)
     {
       extern __shared__ __align__(16) unsigned char smem_raw[];
-      vanka_v3_range<T, NL, NT, PIPE>(M, n_cells, S, W, rhs, ax, cells, omega, (int)threadIdx.x,
+      vanka_v3_range<T, NL, NT, PIPE>(M, n_cells, S, lam, ta, tb, rhs, ax, cells, omega,
+                                      (int)threadIdx.x,
                                       reinterpret_cast<T *>(smem_raw));
     }
 
@@ -98,9 +104,9 @@ namespace stfem
   }
 
   template <typename T>
-  void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *W, const T *rhs,
-                             const T *ax, T *u, T omega, T *cells, int mode, int variant,
-                             hipStream_t s)
+  void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *lam, const T *ta,
+                             const T *tb, const T *rhs, const T *ax, T *u, T omega, T *cells,
+                             int mode, int variant, hipStream_t s)
   {
     if (g.n_cells == 0)
       return;
@@ -143,11 +149,11 @@ namespace stfem
                 }
               const long grid = std::min<long>(g.n_cells, resident);
               k_vanka_factored_v3<T, NL, NT, true><<<dim3((unsigned)grid), dim3(256), lds, s>>>(
-                M, g.n_cells, S, W, rhs, ax, cells, omega);
+                M, g.n_cells, S, lam, ta, tb, rhs, ax, cells, omega);
             }
           else
             k_vanka_factored_v3<T, NL, NT, false><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
-              M, g.n_cells, S, W, rhs, ax, cells, omega);
+              M, g.n_cells, S, lam, ta, tb, rhs, ax, cells, omega);
           STFEM_HIP(hipGetLastError());
           launch_gather_cells<T>(g, NT, cells, u, nullptr, mode, false, s);
           done = true;
@@ -167,7 +173,7 @@ namespace stfem
             STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v2<T, NL, NT, true>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
           k_vanka_factored_v2<T, NL, NT, true><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
-            M, S, W, rhs, ax, u, cells, omega);
+            M, S, lam, ta, tb, rhs, ax, u, cells, omega);
           STFEM_HIP(hipGetLastError());
           launch_gather_cells<T>(g, NT, cells, u, nullptr, mode, false, s);
         }
@@ -179,7 +185,7 @@ namespace stfem
             STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v2<T, NL, NT, false>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
           k_vanka_factored_v2<T, NL, NT, false><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
-            M, S, W, rhs, ax, u, cells, omega);
+            M, S, lam, ta, tb, rhs, ax, u, cells, omega);
         }
       done = true;
     };
@@ -213,11 +219,11 @@ namespace stfem
           STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored<T, true>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(full)));
         k_vanka_factored<T, true><<<dim3((unsigned)g.n_cells), dim3((unsigned)thr), full, s>>>(
-          M, nt, nloc, vanka_s_ld(nloc, sizeof(T)), S, W, rhs, ax, u, omega);
+          M, nt, nloc, vanka_s_ld(nloc, sizeof(T)), S, lam, ta, tb, rhs, ax, u, omega);
       }
     else
       k_vanka_factored<T, false><<<dim3((unsigned)g.n_cells), dim3((unsigned)thr), base, s>>>(
-        M, nt, nloc, vanka_s_ld(nloc, sizeof(T)), S, W, rhs, ax, u, omega);
+        M, nt, nloc, vanka_s_ld(nloc, sizeof(T)), S, lam, ta, tb, rhs, ax, u, omega);
     STFEM_HIP(hipGetLastError());
   }
 
@@ -228,8 +234,10 @@ namespace stfem
                                              const float *, float *, float, hipStream_t);
   template void launch_vanka_factored<double>(const GridDesc &, int, const double *,
                                               const double *, const double *, const double *,
-                                              double *, double, double *, int, int, hipStream_t);
+                                              const double *, const double *, double *, double,
+                                              double *, int, int, hipStream_t);
   template void launch_vanka_factored<float>(const GridDesc &, int, const float *, const float *,
-                                             const float *, const float *, float *, float, float *,
-                                             int, int, hipStream_t);
+                                             const float *, const float *, const float *,
+                                             const float *, float *, float, float *, int, int,
+                                             hipStream_t);
 } // namespace stfem

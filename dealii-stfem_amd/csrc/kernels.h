@@ -66,7 +66,9 @@ namespace stfem
   }
   // Factored form: Binv_c = (I x S_c) blockdiag_k (W_{c,k}) (I x S_c^T),
   // S_c (n_c x n_c, row-major with the row stride vanka_s_ld(n_c, sizeof(T)),
-  // padding zero), W_{c,k} (nt x nt row-major) per mode k.
+  // padding zero), W_{c,k} = (lam_{c,k} Alpha + Beta)^{-1} applied from the
+  // eigenvalues lam (n_c per cell) and the level's time matrices ta = Alpha,
+  // tb = Beta (nt x nt row-major, device).
   // cells (n_cells * n_c * nt values) non-null: two-pass form for the
   // compile-time patch shapes, mode 0 u = correction, 1 u += correction;
   // otherwise (and for the other shapes) u += correction with atomics
@@ -75,9 +77,9 @@ namespace stfem
   // nodes), 4 its software-pipelined persistent form (FP32); shapes a
   // variant does not cover fall back to the others.
   template <typename T>
-  void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *W, const T *rhs,
-                             const T *ax, T *u, T omega, T *cells, int mode, int variant,
-                             hipStream_t s);
+  void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *lam, const T *ta,
+                             const T *tb, const T *rhs, const T *ax, T *u, T omega, T *cells,
+                             int mode, int variant, hipStream_t s);
   // Node gather of a cell-major buffer (cell_slot layout, nto components):
   // mode 0 dst = sum, 1 dst += sum, 2 dst = rhs - sum; skip_dirichlet:
   // Dirichlet nodes are left out (zero for modes 0, 2, unchanged for 1).
@@ -151,12 +153,13 @@ namespace stfem
   void launch_patch_matrices(const GridDesc &g, long p0, long np, long e0, const double *Me,
                              const double *Ke, double *Mp, double *Kp, hipStream_t s);
   // S_c (row-major with the row stride vanka_s_ld, eigenvector k = column k)
-  // from the column-major eigenvectors Z, and W_{c,k} = (lam_k Alpha + Beta)^{-1} (nt <= 8); sets
-  // *singular when a time block cannot be inverted.
+  // from the column-major eigenvectors Z and the eigenvalues in the level
+  // precision (lam_out, n_c per cell); sets *singular when a time block
+  // lam_k Alpha + Beta cannot be inverted (nt <= 8).
   template <typename T>
   void launch_pack_factors(long ncell, int nl, int nt, const double *Z, const double *lam,
-                           const double *Alpha, const double *Beta, T *S, T *W, int *singular,
-                           hipStream_t s);
+                           const double *Alpha, const double *Beta, T *S, T *lam_out,
+                           int *singular, hipStream_t s);
 
   // ------------------------------------------------------- adjoint
   // grad[c] -= kappa[c] * sum_j lam_j^T K_c w_j with K_c the unit-coefficient

@@ -729,9 +729,21 @@ namespace stfem
     , nt_(Alpha.m)
     , form_(form)
   {
+    if (form_ == Form::factored)
+      {
+        // W_k = (lambda_k Alpha + Beta)^{-1} is applied from these in the sweep
+        std::vector<T> a(Alpha.a.begin(), Alpha.a.end()), b(Beta.a.begin(), Beta.a.end());
+        ta_.upload(a.data(), a.size(), ctx.stream);
+        tb_.upload(b.data(), b.size(), ctx.stream);
+        ctx.sync();
+      }
     if (form_ == Form::factored && device_setup != 0 && nt_ <= 8)
       {
-        setup_factored_on_device(m, sh, Alpha, Beta, device_setup == 2);
+        // rocSOLVER's batched divide and conquer is the faster generalized
+        // eigensolver for 125-node patches (3D Q4), Jacobi for 64 and fewer
+        const int nl = (m.g.dim == 3) ? sh.n * sh.n * sh.n : sh.n * sh.n;
+        setup_factored_on_device(m, sh, Alpha, Beta,
+                                 device_setup == 2 || (device_setup == 3 && nl > 64));
         return;
       }
     std::vector<double> Kp, Mp;
@@ -790,7 +802,7 @@ namespace stfem
         const int    ld  = vanka_s_ld(nl, sizeof(T)); // row-padded S blocks
         const size_t bsp = size_t(nl) * ld;
         S_.alloc(size_t(nc) * bsp);
-        W_.alloc(size_t(nc) * nl * nt * nt);
+        lam_.alloc(size_t(nc) * nl);
         cells_.alloc(size_t(nc) * nl * nt);
         const long     ch = std::max<long>(1, std::min<long>(nc, 4096));
         std::vector<T> sS, sW;
@@ -798,7 +810,7 @@ namespace stfem
           {
             const long c1 = std::min(nc, c0 + ch);
             sS.assign(size_t(c1 - c0) * bsp, T(0));
-            sW.resize(size_t(c1 - c0) * nl * nt * nt);
+            sW.resize(size_t(c1 - c0) * nl);
             bool bad = false;
 #pragma omp parallel
             {
@@ -874,8 +886,8 @@ namespace stfem
                   for (int i = 0; i < nl; ++i)
                     for (int col = 0; col < nl; ++col)
                       Sd[size_t(i) * ld + col] = T(Y[size_t(i) * nl + col]);
-                  // W_k = (lambda_k Alpha + Beta)^{-1}
-                  T *Wd = sW.data() + size_t(c - c0) * nl * nt * nt;
+                  // lambda_k; W_k = (lambda_k Alpha + Beta)^{-1} must exist
+                  T *Wd = sW.data() + size_t(c - c0) * nl;
                   std::vector<double> Bk(size_t(nt) * nt), wk;
                   for (int k = 0; k < nl; ++k)
                     {
@@ -884,8 +896,7 @@ namespace stfem
                           Bk[size_t(i) * nt + j] = lam[k] * Alpha(i, j) + Beta(i, j);
                       if (!gj_invert(Bk.data(), nt, wk))
                         bad = true;
-                      for (int i = 0; i < nt * nt; ++i)
-                        Wd[size_t(k) * nt * nt + i] = T(Bk[i]);
+                      Wd[k] = T(lam[k]);
                     }
                 }
             }
@@ -893,8 +904,8 @@ namespace stfem
               throw Error(STATUS_ERR_INTERNAL, "Vanka factored: patch not positive definite");
             STFEM_HIP(hipMemcpy(S_.get() + size_t(c0) * bsp, sS.data(), sS.size() * sizeof(T),
                                 hipMemcpyHostToDevice));
-            STFEM_HIP(hipMemcpy(W_.get() + size_t(c0) * nl * nt * nt, sW.data(),
-                                sW.size() * sizeof(T), hipMemcpyHostToDevice));
+            STFEM_HIP(hipMemcpy(lam_.get() + size_t(c0) * nl, sW.data(), sW.size() * sizeof(T),
+                                hipMemcpyHostToDevice));
           }
       }
   }
@@ -918,7 +929,7 @@ namespace stfem
     hipStream_t     s  = ctx_.stream;
     const size_t bsp = size_t(nl) * vanka_s_ld(nl, sizeof(T)); // row-padded S block
     S_.alloc(size_t(nc) * bsp);
-    W_.alloc(size_t(nc) * nl * nt * nt);
+    lam_.alloc(size_t(nc) * nl);
     cells_.alloc(size_t(nc) * nl * nt);
     DBuf<double> X, coef, dA, dB;
     X.upload(m.X.data(), m.X.size(), s);
@@ -972,7 +983,7 @@ namespace stfem
           throw Error(STATUS_ERR_INTERNAL,
                       std::string("rocSOLVER sygvj: ") + rocblas_status_to_string(st));
         launch_pack_factors<T>(np, nl, nt, Kp.get(), lam.get(), dA.get(), dB.get(),
-                               S_.get() + size_t(p0) * bsp, W_.get() + size_t(p0) * nl * nt * nt,
+                               S_.get() + size_t(p0) * bsp, lam_.get() + size_t(p0) * nl,
                                flag.get(), s);
         STFEM_HIP(hipMemcpyAsync(hinfo.data(), info.get(), size_t(np) * sizeof(int),
                                  hipMemcpyDeviceToHost, s));
@@ -992,8 +1003,8 @@ namespace stfem
   void PreconditionVanka<T>::vmult(T *dst, const T *src) const
   {
     if (form_ == Form::factored)
-      return launch_vanka_factored<T>(grid_, nt_, S_.get(), W_.get(), src, nullptr, dst, T(1),
-                                      cells_.get(), 0, variant_, ctx_.stream);
+      return launch_vanka_factored<T>(grid_, nt_, S_.get(), lam_.get(), ta_.get(), tb_.get(), src,
+                                      nullptr, dst, T(1), cells_.get(), 0, variant_, ctx_.stream);
     launch_fill_zero<T>(dst, size_t(grid_.n_nodes) * nt_, ctx_.stream);
     relax(dst, src, nullptr, T(1));
   }
@@ -1003,8 +1014,8 @@ namespace stfem
     if (form_ == Form::explicit_inverse)
       launch_vanka_explicit<T>(grid_, nt_, binv_.get(), rhs, ax, u, omega, ctx_.stream);
     else
-      launch_vanka_factored<T>(grid_, nt_, S_.get(), W_.get(), rhs, ax, u, omega, cells_.get(), 1,
-                               variant_, ctx_.stream);
+      launch_vanka_factored<T>(grid_, nt_, S_.get(), lam_.get(), ta_.get(), tb_.get(), rhs, ax, u,
+                               omega, cells_.get(), 1, variant_, ctx_.stream);
   }
   template <typename T>
   double PreconditionVanka<T>::sweep_bytes() const
@@ -1014,7 +1025,7 @@ namespace stfem
     const double nst = double(grid_.n_nodes) * nt_;
     if (form_ == Form::explicit_inverse)
       return s * grid_.n_cells * (nt_ * nl) * (nt_ * nl) + 3 * s * nst;
-    return s * grid_.n_cells * (nl * nl + nl * nt_ * nt_) + 3 * s * nst;
+    return s * grid_.n_cells * (nl * nl + nl) + 3 * s * nst;
   }
 
   // ===================================================== time transfer

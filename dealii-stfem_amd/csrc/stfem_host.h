@@ -204,9 +204,9 @@ namespace stfem
     };
     // device_setup: the factored form is set up on the GPU (element and
     // patch matrices, batched generalized eigensolver: 1 Jacobi, 2 divide
-    // and conquer); 0, and the explicit inverse: on the host
+    // and conquer, 3 by patch size); 0, and the explicit inverse: on the host
     PreconditionVanka(Context &ctx, const MeshLevel &m, const Shape1D &sh, const DMat &Alpha,
-                      const DMat &Beta, Form form, int device_setup = 2);
+                      const DMat &Beta, Form form, int device_setup = 3);
     // dst = sum_c (1/valence) B_c^{-1} src|_c   (gmg.h:476-514)
     void   vmult(T *dst, const T *src) const;
     // u += omega * Vanka(rhs - ax); ax may be null.  One relaxation sweep of
@@ -217,7 +217,7 @@ namespace stfem
     // factored sweep kernel (0 automatic, 2 LDS-staged, 3 register-tiled)
     void   set_variant(int v) { variant_ = v; }
     double sweep_bytes() const;
-    size_t storage_bytes() const { return (binv_.size() + S_.size() + W_.size()) * sizeof(T); }
+    size_t storage_bytes() const { return (binv_.size() + S_.size() + lam_.size()) * sizeof(T); }
 
   private:
     void     setup_factored_on_device(const MeshLevel &m, const Shape1D &sh, const DMat &Alpha,
@@ -226,7 +226,8 @@ namespace stfem
     GridDesc grid_;
     int      nt_;
     Form     form_;
-    DBuf<T>  binv_, S_, W_;
+    DBuf<T>  binv_, S_, lam_; // explicit inverses; eigenvectors and eigenvalues
+    DBuf<T>  ta_, tb_;        // the level's time matrices (factored form)
     mutable DBuf<T> cells_; // cell-major corrections of the two-pass sweep
     int             variant_ = 0;
   };
@@ -433,7 +434,7 @@ namespace stfem
     int          atomic_scatter = 0; // 1: one-pass operator with atomic adds
     int          problem = 0; // 0 heat, 1 wave (first order in time, tp_01.cc:167-197)
     int          coarse_coef = 0; // coarse-level coefficient: 0 sampled, 1 geometric, 2 harmonic, 3 arithmetic mean
-    int          vanka_device_setup = 2; // factored Vanka on the GPU: 1 Jacobi, 2 divide-and-conquer eigensolver
+    int          vanka_device_setup = 3; // factored Vanka on the GPU: 1 Jacobi, 2 divide and conquer, 3 by patch size
     int          vanka_kernel = 0; // factored sweep kernel: 0 automatic, 2 LDS-staged, 3 register-tiled
     int          op_jb = 4, op_jb64 = 4; // output components per operator cell pass (levels, FP64 fine)
     GMGOptions   mg;
