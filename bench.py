@@ -377,9 +377,11 @@ def main():
         dofs64 = a.steps * nst / (ms64 * 1e-3)
         g64.close()
 
-    # dominant kernel of an FGMRES iteration on the finest level: two Vanka
-    # sweeps (pre/post), three level vmults and one FP64 vmult
-    if 2 * vk_ms >= 3 * vm32_ms + vm64_ms:
+    # dominant kernel of an FGMRES iteration on the finest level: 2s Vanka
+    # sweeps (s pre- and s post-smoothing steps), about as many FP32 level
+    # vmults, one FP64 vmult
+    s_steps = int(spec.get("smoothing_steps", 1))
+    if 2 * s_steps * vk_ms >= max(2 * s_steps * vm32_ms, vm64_ms):
         dom = dict(kernel="vanka_factored_fp32", ms=vk_ms, bytes=vk_bytes, only="vanka",
                    symbol=r"k_vanka_factored_v")
     else:
