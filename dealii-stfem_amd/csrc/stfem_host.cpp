@@ -739,11 +739,12 @@ namespace stfem
       }
     if (form_ == Form::factored && device_setup != 0 && nt_ <= 8)
       {
-        // rocSOLVER's batched divide and conquer is the faster generalized
-        // eigensolver for 125-node patches (3D Q4), Jacobi for 64 and fewer
+        // batched generalized eigensolver: rocSOLVER's Jacobi is the faster
+        // for patches of up to 64 nodes, the reduction to a standard problem
+        // with batched BLAS-3 and divide and conquer above
         const int nl = (m.g.dim == 3) ? sh.n * sh.n * sh.n : sh.n * sh.n;
         setup_factored_on_device(m, sh, Alpha, Beta,
-                                 device_setup == 2 || (device_setup == 3 && nl > 64));
+                                 device_setup == 3 ? (nl > 64 ? 4 : 1) : device_setup);
         return;
       }
     std::vector<double> Kp, Mp;
@@ -919,7 +920,7 @@ namespace stfem
   template <typename T>
   void PreconditionVanka<T>::setup_factored_on_device(const MeshLevel &m, const Shape1D &sh,
                                                       const DMat &Alpha, const DMat &Beta,
-                                                      bool divide_conquer)
+                                                      int method)
   {
     const GridDesc &g  = m.g;
     const int       nl = (g.dim == 3) ? sh.n * sh.n * sh.n : sh.n * sh.n;
@@ -967,21 +968,61 @@ namespace stfem
                                 Ke.get(), s);
         launch_patch_matrices(g, p0, np, e0, Me.get(), Ke.get(), Mp.get(), Kp.get(), s);
         // K_c z = lambda M_c z with z^T M_c z = 1: eigenvectors overwrite Kp
-        const rocblas_status st =
-          divide_conquer ?
-            rocsolver_dsygvd_strided_batched(h, rocblas_eform_ax, rocblas_evect_original,
-                                             rocblas_fill_upper, nl, Kp.get(), nl,
-                                             rocblas_stride(bs), Mp.get(), nl, rocblas_stride(bs),
-                                             lam.get(), nl, res.get(), nl, info.get(),
-                                             rocblas_int(np)) :
-            rocsolver_dsygvj_strided_batched(h, rocblas_eform_ax, rocblas_evect_original,
-                                             rocblas_fill_upper, nl, Kp.get(), nl,
-                                             rocblas_stride(bs), Mp.get(), nl, rocblas_stride(bs),
-                                             0.0, res.get(), 100, sweeps.get(), lam.get(), nl,
-                                             info.get(), rocblas_int(np));
+        rocblas_status st;
+        if (method == 4)
+          {
+            // reduction to a standard problem with batched BLAS-3:
+            // M = L L^T, C = L^{-1} K L^{-T}, C Q = Q Lambda, Z = L^{-T} Q
+            const double   one = 1.0;
+            const rocblas_stride sb = rocblas_stride(bs);
+            const rocblas_int    nb = rocblas_int(np);
+            st = rocsolver_dpotrf_strided_batched(h, rocblas_fill_lower, nl, Mp.get(), nl, sb,
+                                                  info.get(), nb);
+            if (st == rocblas_status_success)
+              st = rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower,
+                                                 rocblas_operation_none, rocblas_diagonal_non_unit,
+                                                 nl, nl, &one, Mp.get(), nl, sb, Kp.get(), nl, sb,
+                                                 nb);
+            if (st == rocblas_status_success)
+              st = rocblas_dtrsm_strided_batched(h, rocblas_side_right, rocblas_fill_lower,
+                                                 rocblas_operation_transpose,
+                                                 rocblas_diagonal_non_unit, nl, nl, &one,
+                                                 Mp.get(), nl, sb, Kp.get(), nl, sb, nb);
+            if (st == rocblas_status_success)
+              {
+                STFEM_HIP(hipMemcpyAsync(hinfo.data(), info.get(), size_t(np) * sizeof(int),
+                                         hipMemcpyDeviceToHost, s));
+                ctx_.sync();
+                for (long k = 0; k < np; ++k)
+                  if (hinfo[k] != 0)
+                    throw Error(STATUS_ERR_INTERNAL, "Vanka factored: patch mass not positive definite");
+                st = rocsolver_dsyevd_strided_batched(h, rocblas_evect_original,
+                                                      rocblas_fill_lower, nl, Kp.get(), nl, sb,
+                                                      lam.get(), nl, res.get(), nl, info.get(),
+                                                      nb);
+              }
+            if (st == rocblas_status_success)
+              st = rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower,
+                                                 rocblas_operation_transpose,
+                                                 rocblas_diagonal_non_unit, nl, nl, &one,
+                                                 Mp.get(), nl, sb, Kp.get(), nl, sb, nb);
+          }
+        else if (method == 2)
+          st = rocsolver_dsygvd_strided_batched(h, rocblas_eform_ax, rocblas_evect_original,
+                                                rocblas_fill_upper, nl, Kp.get(), nl,
+                                                rocblas_stride(bs), Mp.get(), nl,
+                                                rocblas_stride(bs), lam.get(), nl, res.get(), nl,
+                                                info.get(), rocblas_int(np));
+        else
+          st = rocsolver_dsygvj_strided_batched(h, rocblas_eform_ax, rocblas_evect_original,
+                                                rocblas_fill_upper, nl, Kp.get(), nl,
+                                                rocblas_stride(bs), Mp.get(), nl,
+                                                rocblas_stride(bs), 0.0, res.get(), 100,
+                                                sweeps.get(), lam.get(), nl, info.get(),
+                                                rocblas_int(np));
         if (st != rocblas_status_success)
           throw Error(STATUS_ERR_INTERNAL,
-                      std::string("rocSOLVER sygvj: ") + rocblas_status_to_string(st));
+                      std::string("Vanka eigensolver: ") + rocblas_status_to_string(st));
         launch_pack_factors<T>(np, nl, nt, Kp.get(), lam.get(), dA.get(), dB.get(),
                                S_.get() + size_t(p0) * bsp, lam_.get() + size_t(p0) * nl,
                                flag.get(), s);

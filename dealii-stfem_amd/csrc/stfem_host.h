@@ -221,7 +221,7 @@ namespace stfem
 
   private:
     void     setup_factored_on_device(const MeshLevel &m, const Shape1D &sh, const DMat &Alpha,
-                                      const DMat &Beta, bool divide_conquer);
+                                      const DMat &Beta, int method);
     Context &ctx_;
     GridDesc grid_;
     int      nt_;

@@ -88,8 +88,9 @@ int stfem_time_mg_sequence(int n_sp_lvl, int k_max, int k_min, int nts, int nts_
  * (heat, or wave: the first-order-in-time wave system of tp_01.cc:167-197,
  * solved with stfem_solve_slab_wave), atomic_scatter, vanka_device_setup
  * (0 host, 1 device with the batched Jacobi generalized eigensolver, 2
- * device with divide and conquer, 3 (default) divide and conquer for
- * patches above 64 nodes, Jacobi otherwise), dense_coarse_dofs, coarse_coef (0 the
+ * device with divide and conquer, 4 device with a Cholesky reduction to a
+ * standard problem and divide and conquer, 3 (default) mode 4 for patches
+ * above 64 nodes, Jacobi otherwise), dense_coarse_dofs, coarse_coef (0 the
  * coarse-level coefficient sampled at the quadrature points, 1/2/3 its
  * geometric/harmonic/arithmetic mean over each point's share of the cell),
  * vanka_kernel (0 automatic, 2 LDS-staged, 3 register-tiled, 4 pipelined),

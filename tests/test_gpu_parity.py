@@ -275,13 +275,21 @@ def test_dense_coarse_operator(name, dense):
     h.close()
 
 
-@pytest.mark.parametrize("name", ["3d_q3_dg2_factored", "3d_q3_dg2_fp32_factored"])
-def test_vanka_device_setup_matches_host(name):
+@pytest.mark.parametrize("name,method", [
+    ("3d_q3_dg2_factored", 3), ("3d_q3_dg2_fp32_factored", 3), ("3d_q3_dg2_factored", 1),
+    ("3d_q3_dg2_factored", 2), ("3d_q3_dg2_factored", 4), ("3d_q4_dg3_fp32_factored", 3)])
+def test_vanka_device_setup_matches_host(name, method):
     """The factored Vanka set up on the GPU (element and patch matrices,
     batched generalized eigensolver, time blocks) applies the same operator
-    as the one set up on the host, on every level."""
-    spec, o, g = problems(name)
+    as the one set up on the host, on every level, for each eigensolver:
+    1 Jacobi, 2 divide and conquer, 4 Cholesky reduction + divide and
+    conquer, 3 the automatic choice."""
+    spec, o, _ = problems(name)
     c = CASES[name]
+    g = st.Problem(dict(spec, vanka_device_setup=method),
+                   vertices=box_vertices(spec["dim"], [1 << spec["refine"]] * spec["dim"],
+                                         perturb=c["perturb"], seed=7),
+                   kappa=log_uniform((1 << spec["refine"]) ** spec["dim"], 1e-2, 1e2, seed=11))
     dim = spec["dim"]
     nc = [1 << spec["refine"]] * dim
     X = box_vertices(dim, nc, perturb=c["perturb"], seed=7)
@@ -293,12 +301,13 @@ def test_vanka_device_setup_matches_host(name):
         info = g.level_info(l)
         x = torch.tensor(rng.standard_normal(info["n_nodes"] * info["nt"]), dtype=dt, device=DEV)
         y1, y2 = torch.zeros_like(x), torch.zeros_like(x)
-        print(f"[case {name}] level {l} vanka device vs host setup", flush=True)
+        print(f"[case {name}] level {l} vanka device setup {method} vs host", flush=True)
         g.vanka(l, y1, x)
         h.vanka(l, y2, x)
         tol = 1e-5 if g.level_size == 4 else 1e-11
         assert rel_err(y1.cpu().double().numpy(), y2.cpu().double().numpy()) < tol, l
     h.close()
+    g.close()
 
 
 def test_single_cell_levels_shortcut():
