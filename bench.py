@@ -68,7 +68,8 @@ def workload_large():
     finest-level coefficient over each quadrature point's share of the cell
     (with the point-sampled coefficient the multigrid does not converge for
     this contrast); the V-cycle from the 4^3-cell level down (19,652 DoFs) is
-    one dense operator."""
+    one dense operator; an FGMRES basis of 60 vectors (no restart within a
+    slab: 34 instead of 35.5 iterations, 3 % less time)."""
     import numpy as np
     dim, refine = 3, 6
     nc = [1 << refine] * dim
@@ -77,7 +78,7 @@ def workload_large():
     kappa = np.exp(rng.uniform(np.log(1e-3), np.log(1e3), int(np.prod(nc))))
     spec = dict(dim=3, refine=refine, p=4, type=2, r=3, nts=1, tau=0.25, end_time=1.0,
                 source=3, n_bumps=4, seed=7, reltol=1e-8, abstol=1e-300, maxiter=200,
-                basis=30, vanka_factored=1, smoothing_steps=3, coarse_coef=3,
+                basis=60, vanka_factored=1, smoothing_steps=3, coarse_coef=3,
                 dense_coarse_dofs=20000)
     return spec, X, kappa
 

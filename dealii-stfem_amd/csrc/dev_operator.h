@@ -311,14 +311,68 @@ namespace stfem
         // vertex v, coordinate d of the cell (LDS)
         auto Vx = [&](int v, int d) -> T { return Vl[v * DIM + d]; };
         const T xa = Tb::X(N, ia), xb = Tb::X(N, ib);
+        // The Q1 Jacobian along the owned line (last reference coordinate
+        // xi_l varying): the columns of the other directions are linear in
+        // xi_l and the last column does not depend on it, so
+        // J = Jc + xi_l Jl with Jc, Jl computed once per thread (FP64; in
+        // FP32 the 15 extra live registers cost more than the arithmetic
+        // they save, and J is formed from the vertices at every point).
+        constexpr bool HOIST = sizeof(T) == 8;
+        T              Jc[DIM][DIM], Jl[DIM][DIM];
+        if constexpr (HOIST && DIM == 3)
+          {
+            const T wx[2] = {T(1) - xa, xa}, wy[2] = {T(1) - xb, xb};
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+              {
+                T z0 = T(0), z1 = T(0), y0 = T(0), y1 = T(0), s2 = T(0);
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                  {
+                    z0 += (Vx(1 + 2 * e, d) - Vx(2 * e, d)) * wy[e];
+                    z1 += (Vx(5 + 2 * e, d) - Vx(4 + 2 * e, d)) * wy[e];
+                    y0 += (Vx(e + 2, d) - Vx(e, d)) * wx[e];
+                    y1 += (Vx(e + 6, d) - Vx(e + 4, d)) * wx[e];
+#pragma unroll
+                    for (int f = 0; f < 2; ++f)
+                      s2 += (Vx(e + 2 * f + 4, d) - Vx(e + 2 * f, d)) * (wx[e] * wy[f]);
+                  }
+                Jc[d][0] = z0;
+                Jl[d][0] = z1 - z0;
+                Jc[d][1] = y0;
+                Jl[d][1] = y1 - y0;
+                Jc[d][2] = s2;
+                Jl[d][2] = T(0);
+              }
+          }
+        else if constexpr (HOIST)
+          {
+#pragma unroll
+            for (int d = 0; d < 2; ++d)
+              {
+                const T a0 = Vx(1, d) - Vx(0, d), a1 = Vx(3, d) - Vx(2, d);
+                Jc[d][0] = a0;
+                Jl[d][0] = a1 - a0;
+                Jc[d][1] = (Vx(2, d) - Vx(0, d)) * (T(1) - xa) + (Vx(3, d) - Vx(1, d)) * xa;
+                Jl[d][1] = T(0);
+              }
+          }
+        const T wab = (DIM == 3) ? Tb::W(N, ia) * Tb::W(N, ib) : Tb::W(N, ia);
 #pragma unroll
         for (int k = 0; k < N; ++k)
           {
-            T J[DIM][DIM];
-            T wq;
-            if constexpr (DIM == 3)
+            const T xc = Tb::X(N, k);
+            T       J[DIM][DIM];
+            if constexpr (HOIST)
               {
-                const T xc = Tb::X(N, k);
+#pragma unroll
+                for (int d = 0; d < DIM; ++d)
+#pragma unroll
+                  for (int e = 0; e < DIM; ++e)
+                    J[d][e] = (e == DIM - 1) ? Jc[d][e] : Jc[d][e] + xc * Jl[d][e];
+              }
+            else if constexpr (DIM == 3)
+              {
                 const T wy[2] = {T(1) - xb, xb}, wz[2] = {T(1) - xc, xc}, wx[2] = {T(1) - xa, xa};
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
@@ -329,7 +383,7 @@ namespace stfem
 #pragma unroll
                       for (int f = 0; f < 2; ++f)
                         {
-                          s0 += (Vx(1 + 2 * e + 4 * f, d) - Vx(0 + 2 * e + 4 * f, d)) * wy[e] * wz[f];
+                          s0 += (Vx(1 + 2 * e + 4 * f, d) - Vx(2 * e + 4 * f, d)) * wy[e] * wz[f];
                           s1 += (Vx(e + 2 + 4 * f, d) - Vx(e + 4 * f, d)) * wx[e] * wz[f];
                           s2 += (Vx(e + 2 * f + 4, d) - Vx(e + 2 * f, d)) * wx[e] * wy[f];
                         }
@@ -337,11 +391,9 @@ namespace stfem
                     J[d][1] = s1;
                     J[d][2] = s2;
                   }
-                wq = Tb::W(N, ia) * Tb::W(N, ib) * Tb::W(N, k);
               }
             else
               {
-                const T xc = Tb::X(N, k);
                 const T wy[2] = {T(1) - xc, xc}, wx[2] = {T(1) - xa, xa};
 #pragma unroll
                 for (int d = 0; d < 2; ++d)
@@ -349,8 +401,8 @@ namespace stfem
                     J[d][0] = (Vx(1, d) - Vx(0, d)) * wy[0] + (Vx(3, d) - Vx(2, d)) * wy[1];
                     J[d][1] = (Vx(2, d) - Vx(0, d)) * wx[0] + (Vx(3, d) - Vx(1, d)) * wx[1];
                   }
-                wq = Tb::W(N, ia) * Tb::W(N, k);
               }
+            const T wq = wab * Tb::W(N, k);
             T adj[DIM][DIM];
             T det;
             if constexpr (DIM == 3)
@@ -382,17 +434,18 @@ namespace stfem
               kap = valid ? A.coef[cell * NC + Ly::idx(ia, ib, k)] : T(0);
             const T JxW = adet * wq;
             const T gs  = kap * wq / adet;
-            T       G[DIM][DIM];
+            T       G[DIM][DIM]; // symmetric
 #pragma unroll
             for (int e = 0; e < DIM; ++e)
 #pragma unroll
-              for (int f = 0; f < DIM; ++f)
+              for (int f = e; f < DIM; ++f)
                 {
                   T s = T(0);
 #pragma unroll
                   for (int d = 0; d < DIM; ++d)
                     s += adj[e][d] * adj[f][d];
                   G[e][f] = s * gs;
+                  G[f][e] = G[e][f];
                 }
 #pragma unroll
             for (int jj = 0; jj < JB; ++jj)

@@ -19,13 +19,15 @@ namespace stfem
   {
     template <int DIM, int N, int JB, typename T, bool TO_CELLS>
     __global__ void __launch_bounds__(256)
-      k_st_apply(const StOpArgs<T> A, const T *__restrict__ src, T *__restrict__ dst, int j0,
-                 int jn)
+      k_st_apply(const StOpArgs<T> A, const T *__restrict__ src, T *__restrict__ dst, int jb)
     {
       using Ly = Layout<DIM, N>;
       __shared__ T lds0[Ly::CPB * Ly::NC];
       __shared__ T lds1[Ly::CPB * Ly::NC];
       __shared__ T vlds[Ly::CPB * (1 << DIM) * DIM];
+      // output components [j0, jn) of this grid row
+      const int j0 = (int)blockIdx.y * jb;
+      const int jn = (j0 + jb < A.nt_out) ? j0 + jb : A.nt_out;
       st_apply_cells<DIM, N, JB, T, TO_CELLS>(A, src, dst, j0, jn, (long)blockIdx.x,
                                               (int)threadIdx.x, lds0, lds1, vlds);
     }
@@ -79,29 +81,21 @@ namespace stfem
       using Ly              = Layout<DIM, N>;
       const int  nto        = a.nt_out;
       const int  mj         = (a.max_jb >= 1 && a.max_jb <= 4) ? a.max_jb : 4;
-      const int  chunks     = (nto + mj - 1) / mj;
-      const int  jb         = (nto + chunks - 1) / chunks;
       const long blocks     = (a.g.n_cells + Ly::CPB - 1) / Ly::CPB;
-      const dim3 grid{(unsigned)blocks, 1, 1};
+      int        chunks     = (nto + mj - 1) / mj;
+      if (blocks < a.split_below)
+        chunks = nto;
+      const int  jb         = (nto + chunks - 1) / chunks;
+      chunks                = (nto + jb - 1) / jb;
+      // the chunks of output components run side by side as grid rows
+      const dim3 grid{(unsigned)blocks, (unsigned)chunks, 1};
       const dim3 block{(unsigned)(Ly::CPB * Ly::TPC), 1, 1};
-      for (int j0 = 0; j0 < nto; j0 += jb)
+      switch (jb)
         {
-          const int jn = (j0 + jb < nto) ? j0 + jb : nto;
-          switch (jb)
-            {
-              case 1:
-                k_st_apply<DIM, N, 1, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
-                break;
-              case 2:
-                k_st_apply<DIM, N, 2, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
-                break;
-              case 3:
-                k_st_apply<DIM, N, 3, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
-                break;
-              default:
-                k_st_apply<DIM, N, 4, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, j0, jn);
-                break;
-            }
+          case 1: k_st_apply<DIM, N, 1, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, jb); break;
+          case 2: k_st_apply<DIM, N, 2, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, jb); break;
+          case 3: k_st_apply<DIM, N, 3, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, jb); break;
+          default: k_st_apply<DIM, N, 4, T, TO_CELLS><<<grid, block, 0, s>>>(a, src, dst, jb); break;
         }
       STFEM_HIP(hipGetLastError());
     }

@@ -35,6 +35,11 @@ namespace stfem
     const T      *beta;      // nt_out x nt_in (mass weights), device
     int           nt_in, nt_out;
     int           max_jb = 4; // output components per cell pass (1..4)
+    // grids of fewer cell workgroups than this compute one output component
+    // per workgroup (grid.y = nt_out): more workgroups in flight and a
+    // shorter chain per workgroup on small levels, at the price of
+    // evaluating the inputs once per component (0: off)
+    int           split_below = 0;
   };
   // cells != null: two-pass form with the cell-major scratch `cells` of
   // st_cells_size() values; cells == null: one pass with atomic adds.

@@ -488,7 +488,8 @@ namespace stfem
     args.beta       = b.get();
     args.nt_in      = nt_in_;
     args.nt_out     = nt_out_;
-    args.max_jb     = max_jb_;
+    args.max_jb     = max_jb_ > 0 ? max_jb_ : (sizeof(T) == 8 && grid_.p >= 4 ? 2 : 4);
+    args.split_below = split_below_;
     launch_st_apply<T>(args, src, dst, cells_.get(), mode, rhs, ctx_.stream);
   }
   template <typename T>
@@ -739,12 +740,12 @@ namespace stfem
       }
     if (form_ == Form::factored && device_setup != 0 && nt_ <= 8)
       {
-        // batched generalized eigensolver: rocSOLVER's Jacobi is the faster
-        // for patches of up to 64 nodes, the reduction to a standard problem
-        // with batched BLAS-3 and divide and conquer above
+        // batched generalized eigensolver: reduction to a standard problem
+        // with batched BLAS-3, then Jacobi for patches of up to 64 nodes and
+        // divide and conquer above (the faster of the two on each side)
         const int nl = (m.g.dim == 3) ? sh.n * sh.n * sh.n : sh.n * sh.n;
         setup_factored_on_device(m, sh, Alpha, Beta,
-                                 device_setup == 3 ? (nl > 64 ? 4 : 1) : device_setup);
+                                 device_setup == 3 ? (nl > 64 ? 4 : 5) : device_setup);
         return;
       }
     std::vector<double> Kp, Mp;
@@ -969,10 +970,11 @@ namespace stfem
         launch_patch_matrices(g, p0, np, e0, Me.get(), Ke.get(), Mp.get(), Kp.get(), s);
         // K_c z = lambda M_c z with z^T M_c z = 1: eigenvectors overwrite Kp
         rocblas_status st;
-        if (method == 4)
+        if (method >= 4)
           {
             // reduction to a standard problem with batched BLAS-3:
             // M = L L^T, C = L^{-1} K L^{-T}, C Q = Q Lambda, Z = L^{-T} Q
+            // (C Q = Q Lambda by divide and conquer, method 4, or Jacobi, 5)
             const double   one = 1.0;
             const rocblas_stride sb = rocblas_stride(bs);
             const rocblas_int    nb = rocblas_int(np);
@@ -996,10 +998,16 @@ namespace stfem
                 for (long k = 0; k < np; ++k)
                   if (hinfo[k] != 0)
                     throw Error(STATUS_ERR_INTERNAL, "Vanka factored: patch mass not positive definite");
-                st = rocsolver_dsyevd_strided_batched(h, rocblas_evect_original,
-                                                      rocblas_fill_lower, nl, Kp.get(), nl, sb,
-                                                      lam.get(), nl, res.get(), nl, info.get(),
-                                                      nb);
+                st = method == 4 ?
+                       rocsolver_dsyevd_strided_batched(h, rocblas_evect_original,
+                                                        rocblas_fill_lower, nl, Kp.get(), nl, sb,
+                                                        lam.get(), nl, res.get(), nl, info.get(),
+                                                        nb) :
+                       rocsolver_dsyevj_strided_batched(h, rocblas_esort_ascending,
+                                                        rocblas_evect_original,
+                                                        rocblas_fill_lower, nl, Kp.get(), nl, sb,
+                                                        0.0, res.get(), 100, sweeps.get(),
+                                                        lam.get(), nl, info.get(), nb);
               }
             if (st == rocblas_status_success)
               st = rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_lower,
@@ -1797,6 +1805,7 @@ namespace stfem
     geti("vanka_kernel", ps.vanka_kernel);
     geti("op_jb", ps.op_jb);
     geti("op_jb64", ps.op_jb64);
+    geti("op_split", ps.op_split);
     geti("smoothing_steps", ps.mg.smoothing_steps);
     getb("variable", ps.mg.variable);
     getb("estimate_relaxation", ps.mg.estimate_relaxation);
@@ -1903,6 +1912,7 @@ namespace stfem
           {
             fine_ = std::make_unique<SystemMatrix<double>>(ctx_, F, w.Alpha, w.Beta, tp);
             fine_->set_max_jb(spec.op_jb64);
+            fine_->set_split_below(spec.op_split);
             const DMat rK = (spec.type == TimeStepType::CGP) ? w.Gamma : zero;
             const DMat rM = (spec.type == TimeStepType::CGP) ? w.Zeta : w.Gamma;
             rhs_ = std::make_unique<SystemMatrix<double>>(ctx_, F, rK, rM, tp);
@@ -2007,6 +2017,7 @@ namespace stfem
             levels[l].A  = std::make_unique<SystemMatrix<T>>(ctx_, m, w.Alpha, w.Beta,
                                                              spec.atomic_scatter == 0);
             levels[l].A->set_max_jb(sizeof(T) == 4 ? spec.op_jb : spec.op_jb64);
+            levels[l].A->set_split_below(spec.op_split);
             levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form,
                                                                   spec.vanka_device_setup);
             levels[l].V->set_variant(spec.vanka_kernel);

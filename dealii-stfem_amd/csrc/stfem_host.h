@@ -166,6 +166,9 @@ namespace stfem
     // output time components per cell pass of the kernel (1..4): fewer
     // means fewer registers and more passes over the inputs
     void set_max_jb(int j) { max_jb_ = j; }
+    // grids of fewer cell workgroups than this compute one output
+    // component per workgroup (StOpArgs::split_below)
+    void set_split_below(int b) { split_below_ = b; }
     long        n_nodes() const { return grid_.n_nodes; }
     int         n_blocks() const { return nt_out_; }
     const double *vertices() const { return verts_.get(); }
@@ -187,7 +190,7 @@ namespace stfem
     DBuf<T>      coef_;
     DMat         A_, B_;
     int          nt_in_, nt_out_;
-    int          max_jb_ = 4;
+    int          max_jb_ = 4, split_below_ = 0;
     DBuf<T>      dA_, dB_, dAt_, dBt_;
     mutable DBuf<T> cells_; // cell-major results of the two-pass form
   };
@@ -436,7 +439,10 @@ namespace stfem
     int          coarse_coef = 0; // coarse-level coefficient: 0 sampled, 1 geometric, 2 harmonic, 3 arithmetic mean
     int          vanka_device_setup = 3; // factored Vanka on the GPU: 1 Jacobi, 2 divide and conquer, 3 by patch size
     int          vanka_kernel = 0; // factored sweep kernel: 0 automatic, 2 LDS-staged, 3 register-tiled
-    int          op_jb = 4, op_jb64 = 4; // output components per operator cell pass (levels, FP64 fine)
+    // output components per operator cell pass (FP32 levels, FP64 operators;
+    // 0: 2 for FP64 from Q4 on, where four do not fit two waves per SIMD, else 4)
+    int          op_jb = 4, op_jb64 = 0;
+    int          op_split = 128;         // StOpArgs::split_below of every operator
     GMGOptions   mg;
     SolverControlData solver;
     int          extrapolate = 1;

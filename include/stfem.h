@@ -88,13 +88,16 @@ int stfem_time_mg_sequence(int n_sp_lvl, int k_max, int k_min, int nts, int nts_
  * (heat, or wave: the first-order-in-time wave system of tp_01.cc:167-197,
  * solved with stfem_solve_slab_wave), atomic_scatter, vanka_device_setup
  * (0 host, 1 device with the batched Jacobi generalized eigensolver, 2
- * device with divide and conquer, 4 device with a Cholesky reduction to a
- * standard problem and divide and conquer, 3 (default) mode 4 for patches
- * above 64 nodes, Jacobi otherwise), dense_coarse_dofs, coarse_coef (0 the
+ * device with divide and conquer, 4/5 device with a Cholesky reduction to a
+ * standard problem and divide and conquer/Jacobi, 3 (default) mode 4 for
+ * patches above 64 nodes, 5 otherwise), dense_coarse_dofs, coarse_coef (0 the
  * coarse-level coefficient sampled at the quadrature points, 1/2/3 its
  * geometric/harmonic/arithmetic mean over each point's share of the cell),
  * vanka_kernel (0 automatic, 2 LDS-staged, 3 register-tiled, 4 pipelined),
- * op_jb/op_jb64 (output components per operator pass, levels/FP64 fine).
+ * op_jb/op_jb64 (output components per operator pass, FP32 levels/FP64
+ * operators; op_jb64=0, the default, picks 2 from Q4 on and 4 below),
+ * op_split (operators with fewer cell workgroups than this, 128 by
+ * default, compute one output time component per workgroup).
  * vertices: finest-level vertex coordinates ((nc+1)^dim points, dim each,
  * x fastest) or NULL for the uniform box; cell_coef: finest-level per-cell
  * coefficient or NULL for 1. */

@@ -121,6 +121,46 @@ def test_level_operators(name):
         assert rel_err(nodes_to_blocks(y.cpu().double().numpy(), nt), ref) < tol_of(g) * 100
 
 
+@pytest.mark.parametrize("name", ["3d_q3_dg2", "3d_q4_dg3_fp32", "2d_dg1_nts2", "3d_q2_cg2",
+                                  "2d_dg2_distorted"])
+@pytest.mark.parametrize("chunking", [dict(op_split=0, op_jb=4, op_jb64=4),
+                                      dict(op_split=0, op_jb=2, op_jb64=2),
+                                      dict(op_split=0, op_jb=1, op_jb64=3),
+                                      dict(op_split=1 << 30, op_jb=4, op_jb64=4)])
+def test_operator_output_chunking(name, chunking):
+    """The operator kernel computes the output time components in chunks of
+    op_jb (FP32 levels) / op_jb64 (FP64) per workgroup, side by side as grid
+    rows, and one component per workgroup on grids below op_split
+    workgroups: every split gives the oracle's level and fine operators."""
+    spec, o, _ = problems(name)
+    c = CASES[name]
+    dim = spec["dim"]
+    nc = [1 << spec["refine"]] * dim
+    X = box_vertices(dim, nc, perturb=c["perturb"], seed=7) if c["perturb"] > 0 else None
+    kap = log_uniform(int(np.prod(nc)), 1e-2, 1e2, seed=11) if c["kappa"] else None
+    g = st.Problem(dict(spec, **chunking), vertices=X, kappa=kap)
+    rng = np.random.default_rng(13)
+    dt = dtype_of(g)
+    for l in range(g.n_levels):
+        info = g.level_info(l)
+        n, nt = info["n_nodes"], info["nt"]
+        x = rng.standard_normal(n * nt)
+        xg = torch.tensor(blocks_to_nodes(x, nt), dtype=dt, device=DEV)
+        y = torch.zeros_like(xg)
+        print(f"[case {name}] {chunking} level {l} vmult", flush=True)
+        g.vmult(l, y, xg)
+        ref = o.level_apply(l, 0, x)
+        assert rel_err(nodes_to_blocks(y.cpu().double().numpy(), nt), ref) < tol_of(g) * 10, l
+    x = rng.standard_normal(g.n_nodes * g.nt)
+    xg = torch.tensor(blocks_to_nodes(x, g.nt), dtype=torch.float64, device=DEV)
+    y = torch.zeros_like(xg)
+    for tr in (False, True):
+        print(f"[case {name}] {chunking} fine vmult transpose={tr}", flush=True)
+        (g.Tvmult if tr else g.vmult)(st.FINE, y, xg)
+        assert rel_err(nodes_to_blocks(y.cpu().numpy(), g.nt), o.fine_vmult(x, transpose=tr)) < 1e-12
+    g.close()
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_transfers(name):
     spec, o, g = problems(name)
@@ -277,13 +317,14 @@ def test_dense_coarse_operator(name, dense):
 
 @pytest.mark.parametrize("name,method", [
     ("3d_q3_dg2_factored", 3), ("3d_q3_dg2_fp32_factored", 3), ("3d_q3_dg2_factored", 1),
-    ("3d_q3_dg2_factored", 2), ("3d_q3_dg2_factored", 4), ("3d_q4_dg3_fp32_factored", 3)])
+    ("3d_q3_dg2_factored", 2), ("3d_q3_dg2_factored", 4), ("3d_q3_dg2_factored", 5),
+    ("3d_q4_dg3_fp32_factored", 3)])
 def test_vanka_device_setup_matches_host(name, method):
     """The factored Vanka set up on the GPU (element and patch matrices,
     batched generalized eigensolver, time blocks) applies the same operator
     as the one set up on the host, on every level, for each eigensolver:
-    1 Jacobi, 2 divide and conquer, 4 Cholesky reduction + divide and
-    conquer, 3 the automatic choice."""
+    1 Jacobi, 2 divide and conquer, 4/5 Cholesky reduction + divide and
+    conquer/Jacobi, 3 the automatic choice."""
     spec, o, _ = problems(name)
     c = CASES[name]
     g = st.Problem(dict(spec, vanka_device_setup=method),
