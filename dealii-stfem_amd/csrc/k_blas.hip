@@ -115,6 +115,123 @@ namespace stfem
       x[i] = s;
     }
 
+    // ---- Gram-Schmidt with one reduction pass per Krylov step
+    // Stage 1 of the dots (v_i, x) and (v_i, y), i < m, and (x, y): the
+    // partial sums of workgroup b go to partial[d * RB + b] with d = dx + i,
+    // dy + i and dxy (dxy < 0: no (x, y) dot).
+    constexpr int MD = 16; // left vectors per pass
+    struct DotList
+    {
+      const double *v[MD];
+      int           m, dx, dy, dxy;
+    };
+    __global__ void __launch_bounds__(RT)
+      k_multi_dot(long n, DotList L, const double *__restrict__ x, const double *__restrict__ y,
+                  double *__restrict__ partial)
+    {
+      double ax[MD], ay[MD], axy = 0.0;
+#pragma unroll
+      for (int i = 0; i < MD; ++i)
+        ax[i] = ay[i] = 0.0;
+      for (long e = (long)blockIdx.x * RT + threadIdx.x; e < n; e += (long)RB * RT)
+        {
+          const double xe = x[e], ye = y[e];
+          axy += xe * ye;
+#pragma unroll
+          for (int i = 0; i < MD; ++i)
+            if (i < L.m)
+              {
+                const double ve = L.v[i][e];
+                ax[i] += ve * xe;
+                ay[i] += ve * ye;
+              }
+        }
+      // wavefront sums, then the four wavefront sums in order
+      __shared__ double red[RT / 64][2 * MD + 1];
+      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+      for (int d = 0; d < 2 * MD + 1; ++d)
+        {
+          if (d < 2 * MD && (d % MD) >= L.m)
+            continue;
+          double v = d < MD ? ax[d] : (d < 2 * MD ? ay[d - MD] : axy);
+          for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_down(v, off, 64);
+          if (lane == 0)
+            red[w][d] = v;
+        }
+      __syncthreads();
+      const int d = threadIdx.x;
+      if (d < 2 * MD + 1 && (d == 2 * MD ? L.dxy >= 0 : (d % MD) < L.m))
+        {
+          double s = 0.0;
+          for (int i = 0; i < RT / 64; ++i)
+            s += red[i][d];
+          const int o = d < MD ? L.dx + d : (d < 2 * MD ? L.dy + d - MD : L.dxy);
+          partial[(long)o * RB + blockIdx.x] = s;
+        }
+    }
+
+    // stage 2 of nd dots (one workgroup each, the order of k_finish)
+    __global__ void __launch_bounds__(RT) k_finish_many(const double *partial, double *out)
+    {
+      const double *p = partial + (long)blockIdx.x * RB;
+      double        s = 0.0;
+      for (int i = threadIdx.x; i < RB; i += RT)
+        s += p[i];
+      s = block_sum(s);
+      if (threadIdx.x == 0)
+        out[blockIdx.x] = s;
+    }
+
+    // Coefficients of step j from dots = [(v_0,x) .. (v_j,x), (v_0,v_j) ..
+    // (v_{j-1},v_j)]: column j of the Gram matrix G (ld ldg) is stored, then
+    // h_i = (v_i, x) - sum_{k<i} G(k,i) h_k, which equals the coefficient
+    // modified Gram-Schmidt computes from x - sum_{k<i} h_k v_k.
+    __global__ void k_gs_coefficients(const double *dots, int j, double *G, int ldg, double *h)
+    {
+      if (threadIdx.x != 0)
+        return;
+      for (int i = 0; i < j; ++i)
+        G[i + (long)j * ldg] = dots[j + 1 + i];
+      for (int i = 0; i <= j; ++i)
+        {
+          double s = dots[i];
+          for (int k = 0; k < i; ++k)
+            s -= G[k + (long)i * ldg] * h[k];
+          h[i] = s;
+        }
+    }
+
+    // x -= sum_i h_i v_i (h on the device), then the partial sums of (x, x)
+    constexpr int MA = 64;
+    struct AxpyList
+    {
+      const double *v[MA];
+      int           m;
+    };
+    __global__ void __launch_bounds__(RT)
+      k_multi_axpy_norm(long n, AxpyList L, const double *__restrict__ h, double *__restrict__ x,
+                        double *__restrict__ pnorm)
+    {
+      __shared__ double hs[MA];
+      if (threadIdx.x < L.m)
+        hs[threadIdx.x] = h[threadIdx.x];
+      __syncthreads();
+      double s = 0.0;
+      for (long e = (long)blockIdx.x * RT + threadIdx.x; e < n; e += (long)RB * RT)
+        {
+          double w = x[e];
+          for (int i = 0; i < L.m; ++i)
+            w -= hs[i] * L.v[i][e];
+          x[e] = w;
+          s += w * w;
+        }
+      s = block_sum(s);
+      if (threadIdx.x == 0)
+        pnorm[blockIdx.x] = s;
+    }
+
     template <typename TD, typename TS>
     __global__ void k_convert(long n, const TS *__restrict__ s, TD *__restrict__ d)
     {
@@ -306,6 +423,35 @@ namespace stfem
     for (int g = 0; g < RT / 64; ++g)
       r += w[g];
     return r;
+  }
+
+  void launch_gs_step(long n, int j, const double *const *v, double *x, double *G, int ldg,
+                      double *partial, double *dots, double *h, double *pnorm, hipStream_t s)
+  {
+    STFEM_REQUIRE(j >= 0 && j < MA && j < ldg, "Gram-Schmidt step: basis too large");
+    const double *vj = v[j];
+    // (v_i, x) -> d = i, (v_i, v_j) -> d = j + 1 + i for i < j; (v_j, x) -> d = j
+    for (int i0 = 0; i0 < j || i0 == 0; i0 += MD)
+      {
+        DotList L;
+        L.m   = std::min(MD, j - i0);
+        L.dx  = i0;
+        L.dy  = j + 1 + i0;
+        L.dxy = (i0 == 0) ? j : -1;
+        for (int i = 0; i < L.m; ++i)
+          L.v[i] = v[i0 + i];
+        k_multi_dot<<<RB, RT, 0, s>>>(n, L, x, vj, partial);
+        if (j == 0)
+          break;
+      }
+    k_finish_many<<<2 * j + 1, RT, 0, s>>>(partial, dots);
+    k_gs_coefficients<<<1, 64, 0, s>>>(dots, j, G, ldg, h);
+    AxpyList A;
+    A.m = j + 1;
+    for (int i = 0; i <= j; ++i)
+      A.v[i] = v[i];
+    k_multi_axpy_norm<<<RB, RT, 0, s>>>(n, A, h, x, pnorm);
+    STFEM_HIP(hipGetLastError());
   }
 
   template <typename T>

@@ -125,6 +125,14 @@ namespace stfem
                              const double *z, double *pout, double *hout, hipStream_t s);
   // stage 2 on the host, in the same order as the device stage 2
   double sum_partials(const double *partial_host);
+  // Gram-Schmidt step j of a Krylov basis v[0..j] (j < 64) with one
+  // reduction pass: the dots of x and v_j with the basis, the coefficients
+  // h[0..j] of modified Gram-Schmidt from them through the Gram matrix G
+  // (column j stored, leading dimension ldg > j), x -= sum h_i v_i, and the
+  // partial sums of (x, x) into pnorm.  partial: (2j + 1) * reduce_slots()
+  // doubles, dots: 2j + 1 doubles, h: j + 1 doubles (device).
+  void launch_gs_step(long n, int j, const double *const *v, double *x, double *G, int ldg,
+                      double *partial, double *dots, double *h, double *pnorm, hipStream_t s);
   template <typename T>
   void launch_axpby(long n, T a, const T *x, T b, T *y, hipStream_t s); // y = a x + b y
   void launch_multi_axpy(long n, int m, const double *const *z, const double *coef,

@@ -1615,6 +1615,13 @@ namespace stfem
         v_.emplace_back(size_t(n_));
         z_.emplace_back(size_t(n_));
       }
+    if (c_.orthogonalization == 1 && gs_G_.size() < size_t(m + 1) * (m + 1))
+      {
+        gs_part_.alloc(size_t(2 * m + 1) * reduce_slots());
+        gs_dots_.alloc(size_t(2 * m + 1));
+        gs_h_.alloc(size_t(m + 1));
+        gs_G_.alloc(size_t(m + 1) * (m + 1));
+      }
     hist_.clear();
     double       *aux = aux_.get();
     double       *dH  = ctx_.scalars.get();
@@ -1667,20 +1674,36 @@ namespace stfem
               v_[j].zero(ctx_.stream);
             P.vmult(z_[j].get(), v_[j].get());
             A.vmult(aux, z_[j].get());
-            // modified Gram-Schmidt: every step takes its coefficient from
-            // the previous step's partial sums (all on the device) and
-            // stores it to dH
-            launch_dot_partials<double>(n_, aux, v_[0].get(), pA, ctx_.stream);
-            double *pin = pA, *pout = pB;
-            for (int i = 1; i <= j + 1; ++i)
+            double *pin = pA;
+            if (c_.orthogonalization == 1)
               {
-                launch_axpy_dot_chain(n_, aux, pin, -1.0, v_[i - 1].get(),
-                                      i <= j ? v_[i].get() : nullptr, pout, dH + i - 1,
-                                      ctx_.stream);
-                std::swap(pin, pout);
+                // modified Gram-Schmidt coefficients from one pass of dots
+                // (the Gram matrix correction, all on the device)
+                std::vector<const double *> vp(j + 1);
+                for (int i = 0; i <= j; ++i)
+                  vp[i] = v_[i].get();
+                launch_gs_step(n_, j, vp.data(), aux, gs_G_.get(), m + 1, gs_part_.get(),
+                               gs_dots_.get(), gs_h_.get(), pA, ctx_.stream);
+                STFEM_HIP(hipMemcpyAsync(hcol.data(), gs_h_.get(), (j + 1) * sizeof(double),
+                                         hipMemcpyDeviceToHost, ctx_.stream));
               }
-            STFEM_HIP(hipMemcpyAsync(hcol.data(), dH, (j + 1) * sizeof(double),
-                                     hipMemcpyDeviceToHost, ctx_.stream));
+            else
+              {
+                // modified Gram-Schmidt: every step takes its coefficient
+                // from the previous step's partial sums (all on the device)
+                // and stores it to dH
+                launch_dot_partials<double>(n_, aux, v_[0].get(), pA, ctx_.stream);
+                double *pout = pB;
+                for (int i = 1; i <= j + 1; ++i)
+                  {
+                    launch_axpy_dot_chain(n_, aux, pin, -1.0, v_[i - 1].get(),
+                                          i <= j ? v_[i].get() : nullptr, pout, dH + i - 1,
+                                          ctx_.stream);
+                    std::swap(pin, pout);
+                  }
+                STFEM_HIP(hipMemcpyAsync(hcol.data(), dH, (j + 1) * sizeof(double),
+                                         hipMemcpyDeviceToHost, ctx_.stream));
+              }
             const double hn = std::sqrt(host_sum(pin)); // also waits for hcol
             for (int i = 0; i <= j; ++i)
               H[i][j] = hcol[i];
@@ -1821,6 +1844,7 @@ namespace stfem
     getd("abstol", ps.solver.abstol);
     getd("reltol", ps.solver.reltol);
     geti("basis", ps.solver.basis);
+    geti("orthogonalization", ps.solver.orthogonalization);
     geti("extrapolate", ps.extrapolate);
     getd("end_time", ps.end_time);
     getd("frequency", ps.frequency);

@@ -389,6 +389,10 @@ namespace stfem
     double abstol  = 1e-12;
     double reltol  = 1e-12;
     int    basis   = 30;
+    // 0: modified Gram-Schmidt as a chain of axpy+dot passes; 1: the same
+    // coefficients from one pass of dots against the whole basis corrected
+    // with the basis' Gram matrix (two passes over the basis per step)
+    int    orthogonalization = 1;
   };
 
   // Right-preconditioned flexible GMRES (deal.II SolverFGMRES semantics:
@@ -409,6 +413,7 @@ namespace stfem
     SolverControlData         c_;
     std::vector<DBuf<double>> v_, z_;
     DBuf<double>              aux_;
+    DBuf<double>              gs_part_, gs_dots_, gs_h_, gs_G_; // one-reduction Gram-Schmidt
     int                       last_step_ = 0;
     std::vector<double>       hist_;
   };

@@ -249,6 +249,45 @@ def test_slab_solves(name):
         time += spec["nts"] * g.tau
 
 
+@pytest.mark.parametrize("name", ["3d_q3_dg2", "2d_dg1_nts2", "3d_q2_cg2", "2d_dg2_distorted",
+                                  "3d_q3_dg2_fp32_factored", "3d_q4_dg3_fp32"])
+@pytest.mark.parametrize("ortho", [0, 1])
+def test_fgmres_residual_history(name, ortho):
+    """FGMRES with the GMG preconditioner from x = 0 on b = A x_r (x_r
+    seeded): with either Gram-Schmidt form (0: chain of axpy+dot passes, 1:
+    one pass of dots corrected by the Gram matrix) the GPU solver takes the
+    oracle's iterations and its residual history (SolverFGMRES's projected
+    residuals) agrees within 1e-10 of the initial residual; FP32 multigrid:
+    within one iteration and 1e-6."""
+    spec, o, _ = problems(name)
+    c = CASES[name]
+    dim = spec["dim"]
+    nc = [1 << spec["refine"]] * dim
+    X = box_vertices(dim, nc, perturb=c["perturb"], seed=7) if c["perturb"] > 0 else None
+    kap = log_uniform(int(np.prod(nc)), 1e-2, 1e2, seed=11) if c["kappa"] else None
+    g = st.Problem(dict(spec, orthogonalization=ortho), vertices=X, kappa=kap)
+    rng = np.random.default_rng(29)
+    n, nt = g.n_nodes, g.nt
+    b = o.fine_vmult(rng.standard_normal(n * nt))
+    rc_o, xo, its_o, ho = o.fgmres(b, np.zeros(n * nt))
+    xg = torch.zeros(n * nt, dtype=torch.float64, device=DEV)
+    bg = torch.tensor(blocks_to_nodes(b, nt), dtype=torch.float64, device=DEV)
+    print(f"[case {name}] fgmres orthogonalization {ortho}", flush=True)
+    rc_g, its_g, hg = g.fgmres(xg, bg)
+    assert rc_o == 0 and rc_g == 0, (rc_o, rc_g)
+    fp64 = g.level_size == 8
+    if fp64:
+        assert its_g == its_o, (its_g, its_o)
+    else:
+        assert abs(its_g - its_o) <= 1, (its_g, its_o)
+    k = min(len(hg), len(ho))
+    dev = np.abs(hg[:k] - ho[:k]).max() / ho[0]
+    assert dev < (1e-10 if fp64 else 1e-6), dev
+    err = rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo)
+    assert err < (1e-9 if fp64 else 1e-6), err
+    g.close()
+
+
 def test_graph_replay_matches_eager():
     """The V-cycle replayed from a captured HIP graph equals the eagerly
     launched V-cycle on every replay, with other device work in between."""
