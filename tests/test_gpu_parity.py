@@ -406,10 +406,11 @@ def test_single_cell_levels_shortcut():
 
 @pytest.mark.parametrize("name", ["3d_q3_dg2_fp32_factored", "3d_q4_dg3_fp32_factored",
                                   "3d_q3_dg2_factored"])
-@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("variant", [2, 3, 4])
 def test_vanka_kernel_variants(name, variant):
-    """Both factored sweep kernels (LDS-staged S, register-tiled S) on every
-    level against the oracle's PreconditionVanka::vmult."""
+    """The factored sweep kernels (2 LDS-staged S, 3 register-tiled S, 4 its
+    software-pipelined persistent form in FP32) on every level against the
+    oracle's PreconditionVanka::vmult."""
     spec, o, _ = problems(name)
     c = CASES[name]
     dim = spec["dim"]
