@@ -36,43 +36,16 @@ namespace stfem
       return cell_slot<DIM, N>(g.nc[0], g.nc[1], nto, cx, cy, cz, a, b, c, j);
     }
 
-    // Second pass of the two-pass forms: node `node` of the level sums its
-    // (one to 2^dim) cell-local results from the cell-major buffer, in a
-    // fixed order (cells by increasing x, then y, then z), and stores
-    //   mode 0: dst = sum,  mode 1: dst += sum,  mode 2: dst = rhs - sum.
-    // SKIP_DIR (operator): Dirichlet nodes get zero (modes 0, 2) or stay
-    // unchanged (mode 1); otherwise (Vanka, whose patches cover them) they
-    // are summed like every other node.
-    // NT > 0: compile-time number of components (all loads issued before
-    // the sums); NT == 0: nto at run time.
-    template <int DIM, int N, int NT, typename T, bool SKIP_DIR = true>
-    __device__ __forceinline__ void gather_node(const GridDesc &g, int nto,
-                                                const T *__restrict__ cells, T *__restrict__ dst,
-                                                const T *__restrict__ rhs, int mode, long node)
+    // The (one to 2^dim) positions of node (gx, gy, gz)'s cell-local
+    // values, component 0, in the order of summation (cells by increasing
+    // x, then y, then z); returns how many there are.
+    template <int DIM, int N>
+    __device__ __forceinline__ int node_slots(const GridDesc &g, int nt, long gx, long gy, long gz,
+                                              long slot[1 << DIM])
     {
-      constexpr int p   = N - 1;
-      const long    nn0 = g.nn[0], nn1 = g.nn[1];
-      const long    gx  = node % nn0, r = node / nn0;
-      const long    gy  = (DIM == 3) ? r % nn1 : r;
-      const long    gz  = (DIM == 3) ? r / nn1 : 0;
-      const int     d   = g.dirichlet;
-      bool bnd = (gx == 0 && (d & 1)) || (gx == nn0 - 1 && (d & 2)) || (gy == 0 && (d & 4)) ||
-                 (gy == nn1 - 1 && (d & 8));
-      if (DIM == 3)
-        bnd = bnd || (gz == 0 && (d & 16)) || (gz == g.nn[2] - 1 && (d & 32));
-      if (!SKIP_DIR)
-        bnd = false;
-      const int nt  = NT > 0 ? NT : nto;
-      T *const  out = dst + node * nt;
-      if (bnd)
-        {
-          if (mode != 1)
-            for (int j = 0; j < nt; ++j)
-              out[j] = T(0);
-          return;
-        }
+      constexpr int p = N - 1;
       // the cells sharing the node and its local index in each, per direction
-      int cs[3][2] = {}, ls[3][2] = {}, nsh[3] = {1, 1, 1};
+      int        cs[3][2] = {}, ls[3][2] = {}, nsh[3] = {1, 1, 1};
       const long gc[3] = {gx, gy, gz};
 #pragma unroll
       for (int e = 0; e < DIM; ++e)
@@ -95,9 +68,7 @@ namespace stfem
               ls[e][0] = l;
             }
         }
-      constexpr int MAXS = 1 << DIM;
-      long          slot[MAXS];
-      int           ns = 0;
+      int ns = 0;
 #pragma unroll
       for (int kz = 0; kz < (DIM == 3 ? 2 : 1); ++kz)
 #pragma unroll
@@ -113,6 +84,72 @@ namespace stfem
                                                           ls[0][kx], ls[1][ky], 0, 0);
               ns += use ? 1 : 0;
             }
+      return ns;
+    }
+
+    // Whether node (gx, gy, gz) lies on a Dirichlet face.
+    template <int DIM>
+    __device__ __forceinline__ bool dirichlet_node(const GridDesc &g, long gx, long gy, long gz)
+    {
+      const int d = g.dirichlet;
+      bool bnd = (gx == 0 && (d & 1)) || (gx == g.nn[0] - 1 && (d & 2)) || (gy == 0 && (d & 4)) ||
+                 (gy == g.nn[1] - 1 && (d & 8));
+      if (DIM == 3)
+        bnd = bnd || (gz == 0 && (d & 16)) || (gz == g.nn[2] - 1 && (d & 32));
+      return bnd;
+    }
+
+    // Component j of node (gx, gy, gz) summed from a cell-major buffer of
+    // nto components, zero on Dirichlet nodes: the value gather_node stores
+    // in mode 0 (same order of summation, so the same bits), computed where
+    // it is consumed.
+    template <int DIM, int N, typename T>
+    __device__ __forceinline__ T node_sum(const GridDesc &g, int nto, const T *__restrict__ cells,
+                                          long gx, long gy, long gz, int j)
+    {
+      if (dirichlet_node<DIM>(g, gx, gy, gz))
+        return T(0);
+      long      slot[1 << DIM];
+      const int ns = node_slots<DIM, N>(g, nto, gx, gy, gz, slot);
+      const long L = long(g.nc[0]) * N; // component stride of cell_slot
+      T          s = T(0);
+      for (int k = 0; k < ns; ++k)
+        s += cells[slot[k] + j * L];
+      return s;
+    }
+
+    // Second pass of the two-pass forms: node `node` of the level sums its
+    // (one to 2^dim) cell-local results from the cell-major buffer, in a
+    // fixed order (cells by increasing x, then y, then z), and stores
+    //   mode 0: dst = sum,  mode 1: dst += sum,  mode 2: dst = rhs - sum.
+    // SKIP_DIR (operator): Dirichlet nodes get zero (modes 0, 2) or stay
+    // unchanged (mode 1); otherwise (Vanka, whose patches cover them) they
+    // are summed like every other node.
+    // NT > 0: compile-time number of components (all loads issued before
+    // the sums); NT == 0: nto at run time.
+    template <int DIM, int N, int NT, typename T, bool SKIP_DIR = true>
+    __device__ __forceinline__ void gather_node(const GridDesc &g, int nto,
+                                                const T *__restrict__ cells, T *__restrict__ dst,
+                                                const T *__restrict__ rhs, int mode, long node)
+    {
+      constexpr int p   = N - 1;
+      const long    nn0 = g.nn[0], nn1 = g.nn[1];
+      const long    gx  = node % nn0, r = node / nn0;
+      const long    gy  = (DIM == 3) ? r % nn1 : r;
+      const long    gz  = (DIM == 3) ? r / nn1 : 0;
+      const bool    bnd = SKIP_DIR && dirichlet_node<DIM>(g, gx, gy, gz);
+      const int nt  = NT > 0 ? NT : nto;
+      T *const  out = dst + node * nt;
+      if (bnd)
+        {
+          if (mode != 1)
+            for (int j = 0; j < nt; ++j)
+              out[j] = T(0);
+          return;
+        }
+      constexpr int MAXS = 1 << DIM;
+      long          slot[MAXS];
+      const int     ns = node_slots<DIM, N>(g, nt, gx, gy, gz, slot);
       const long L = long(g.nc[0]) * N; // component stride of cell_slot
       if constexpr (NT > 0)
         {

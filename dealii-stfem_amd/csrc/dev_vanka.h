@@ -547,7 +547,10 @@ namespace stfem
                                                     size_t(16) * NT * KP : size_t(16) * RS));
     };
 
-    template <typename T, int NL, int NT, bool PIPE>
+    // AXC: A x at the patch nodes is summed here from the operator's
+    // cell-major buffer opc (the value its node gather would store), which
+    // saves the gather launch of the residual; ax is not read.
+    template <typename T, int NL, int NT, bool PIPE, bool AXC = false>
     __device__ __forceinline__ void vanka_v3_range(const PatchMap &M, long n_cells,
                                                    const T *__restrict__ S,
                                                    const T *__restrict__ lam,
@@ -555,7 +558,9 @@ namespace stfem
                                                    const T *__restrict__ tbeta,
                                                    const T *__restrict__ rhs,
                                                    const T *__restrict__ ax, T *__restrict__ cells,
-                                                   T omega, int tid, T *smem)
+                                                   T omega, int tid, T *smem,
+                                                   const GridDesc *gd = nullptr,
+                                                   const T *__restrict__ opc = nullptr)
     {
       using C = V3<T, NL, NT>;
       constexpr int VW = C::VW, LDG = C::LDG, R = C::R, H = C::H, KP = C::KP, AP = C::AP,
@@ -601,7 +606,14 @@ namespace stfem
                 float      iv;
                 const long gn = node_ct<NL>(M, cx, cy, cz, a, iv);
                 v             = rhs[gn * NT + i];
-                if (ax)
+                if constexpr (AXC)
+                  {
+                    const int la = a % n, lb = (a / n) % n, lc = (Sh::dim == 3) ? a / (n * n) : 0;
+                    v -= node_sum<Sh::dim, n, T>(*gd, NT, opc, (long)cx * (n - 1) + la,
+                                                 (long)cy * (n - 1) + lb, (long)cz * (n - 1) + lc,
+                                                 i);
+                  }
+                else if (ax)
                   v -= ax[gn * NT + i];
               }
             b.x[j] = v;

@@ -110,12 +110,18 @@ namespace stfem
 
   namespace
   {
-    // mode 0: dst = A src, 1: dst += A src, 2: dst = rhs - A src
+    // mode 0: dst = A src, 1: dst += A src, 2: dst = rhs - A src, 3: the
+    // cell pass only
     template <int DIM, int N, typename T>
     void apply_dn(const StOpArgs<T> &a, const T *src, T *dst, T *cells, const T *rhs, int mode,
                   hipStream_t s)
     {
       const long m = a.g.n_nodes * a.nt_out;
+      if (mode == 3)
+        {
+          STFEM_REQUIRE(cells, "st_apply: the cell pass alone needs the two-pass operator");
+          return dispatch_jb<DIM, N, T, true>(a, src, cells, s);
+        }
       if (!cells)
         {
           if (mode == 2)

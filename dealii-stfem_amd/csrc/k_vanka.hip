@@ -72,6 +72,19 @@ namespace stfem
                                       (int)threadIdx.x,
                                       reinterpret_cast<T *>(smem_raw));
     }
+    // the same sweep with A x summed from the operator's cell buffer opc
+    template <typename T, int NL, int NT>
+    __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2)
+      k_vanka_factored_v3_opc(const PatchMap M, const GridDesc g, const T *__restrict__ S,
+                              const T *__restrict__ lam, const T *__restrict__ ta,
+                              const T *__restrict__ tb, const T *__restrict__ rhs,
+                              const T *__restrict__ opc, T *__restrict__ cells, T omega)
+    {
+      extern __shared__ __align__(16) unsigned char smem_raw[];
+      vanka_v3_range<T, NL, NT, false, true>(M, g.n_cells, S, lam, ta, tb, rhs, nullptr, cells,
+                                             omega, (int)threadIdx.x,
+                                             reinterpret_cast<T *>(smem_raw), &g, opc);
+    }
 
     size_t max_lds_per_block()
     {
@@ -103,13 +116,54 @@ namespace stfem
     STFEM_HIP(hipGetLastError());
   }
 
+  namespace
+  {
+    // the compile-time patch shapes of the factored kernels, f(NL, NT)
+    template <typename F>
+    void for_each_shape(F &&f)
+    {
+      auto for_nt = [&](auto nl_c) {
+        f(nl_c, std::integral_constant<int, 1>{});
+        f(nl_c, std::integral_constant<int, 2>{});
+        f(nl_c, std::integral_constant<int, 3>{});
+        f(nl_c, std::integral_constant<int, 4>{});
+        f(nl_c, std::integral_constant<int, 6>{});
+        f(nl_c, std::integral_constant<int, 8>{});
+      };
+      for_nt(std::integral_constant<int, 4>{});
+      for_nt(std::integral_constant<int, 8>{});
+      for_nt(std::integral_constant<int, 9>{});
+      for_nt(std::integral_constant<int, 16>{});
+      for_nt(std::integral_constant<int, 25>{});
+      for_nt(std::integral_constant<int, 27>{});
+      for_nt(std::integral_constant<int, 64>{});
+      for_nt(std::integral_constant<int, 125>{});
+      for_nt(std::integral_constant<int, 36>{});
+    }
+  } // namespace
+
+  template <typename T>
+  bool vanka_fused_residual_supported(const GridDesc &g, int nt)
+  {
+    const int nloc = patch_size(g);
+    bool      ok   = false;
+    for_each_shape([&](auto nl_c, auto nt_c) {
+      constexpr int NL = decltype(nl_c)::value, NT = decltype(nt_c)::value;
+      if constexpr (NL <= 128 && V3<T, NL, NT>::lds <= 80 * 1024)
+        if (nloc == NL && nt == NT && V3<T, NL, NT>::lds <= max_lds_per_block())
+          ok = true;
+    });
+    return ok;
+  }
+
   template <typename T>
   void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *lam, const T *ta,
                              const T *tb, const T *rhs, const T *ax, T *u, T omega, T *cells,
-                             int mode, int variant, hipStream_t s)
+                             int mode, int variant, hipStream_t s, const T *opc)
   {
     if (g.n_cells == 0)
       return;
+    STFEM_REQUIRE(!opc || cells, "vanka factored: the fused residual needs the two-pass form");
     STFEM_REQUIRE(mode == 0 || mode == 1, "vanka factored: mode must be 0 or 1");
     STFEM_REQUIRE(nt <= 16, "vanka factored: at most 16 time dofs per patch");
     const int    nloc = patch_size(g);
@@ -125,7 +179,7 @@ namespace stfem
         {
           if (done || nloc != NL || nt != NT || !cells)
             return;
-          const bool use = variant == 3 || (variant == 4 && sizeof(T) == 4) ||
+          const bool use = opc || variant == 3 || (variant == 4 && sizeof(T) == 4) ||
                            (variant == 0 && NL >= 64);
           if (!use)
             return;
@@ -133,9 +187,16 @@ namespace stfem
           if (lds > max_lds_per_block())
             return;
           if (lds > 64 * 1024)
-            STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v3<T, NL, NT, false>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-          if (variant == 4)
+            {
+              STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v3<T, NL, NT, false>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+              STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v3_opc<T, NL, NT>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            }
+          if (opc)
+            k_vanka_factored_v3_opc<T, NL, NT><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
+              M, g, S, lam, ta, tb, rhs, opc, cells, omega);
+          else if (variant == 4)
             {
               static int resident = 0;
               if (!resident)
@@ -189,23 +250,14 @@ namespace stfem
         }
       done = true;
     };
-    auto for_nt = [&](auto nl_c) {
-      try_v2(nl_c, std::integral_constant<int, 1>{});
-      try_v2(nl_c, std::integral_constant<int, 2>{});
-      try_v2(nl_c, std::integral_constant<int, 3>{});
-      try_v2(nl_c, std::integral_constant<int, 4>{});
-      try_v2(nl_c, std::integral_constant<int, 6>{});
-      try_v2(nl_c, std::integral_constant<int, 8>{});
-    };
-    for_nt(std::integral_constant<int, 4>{});
-    for_nt(std::integral_constant<int, 8>{});
-    for_nt(std::integral_constant<int, 9>{});
-    for_nt(std::integral_constant<int, 16>{});
-    for_nt(std::integral_constant<int, 25>{});
-    for_nt(std::integral_constant<int, 27>{});
-    for_nt(std::integral_constant<int, 64>{});
-    for_nt(std::integral_constant<int, 125>{});
-    for_nt(std::integral_constant<int, 36>{});
+    if (opc)
+      {
+        for_each_shape(try_v3);
+        STFEM_REQUIRE(done, "vanka factored: no fused-residual kernel for this patch shape");
+        STFEM_HIP(hipGetLastError());
+        return;
+      }
+    for_each_shape(try_v2);
     if (done)
       {
         STFEM_HIP(hipGetLastError());
@@ -235,9 +287,11 @@ namespace stfem
   template void launch_vanka_factored<double>(const GridDesc &, int, const double *,
                                               const double *, const double *, const double *,
                                               const double *, const double *, double *, double,
-                                              double *, int, int, hipStream_t);
+                                              double *, int, int, hipStream_t, const double *);
   template void launch_vanka_factored<float>(const GridDesc &, int, const float *, const float *,
                                              const float *, const float *, const float *,
                                              const float *, float *, float, float *, int, int,
-                                             hipStream_t);
+                                             hipStream_t, const float *);
+  template bool vanka_fused_residual_supported<double>(const GridDesc &, int);
+  template bool vanka_fused_residual_supported<float>(const GridDesc &, int);
 } // namespace stfem

@@ -44,7 +44,8 @@ namespace stfem
   // cells != null: two-pass form with the cell-major scratch `cells` of
   // st_cells_size() values; cells == null: one pass with atomic adds.
   // mode 0: dst = A src;  1: dst += A src;  2: dst = rhs - A src (two-pass
-  // form only).
+  // form only); 3: only the cell pass into `cells` (dst unused), for a
+  // consumer that sums the node values itself.
   template <typename T>
   void launch_st_apply(const StOpArgs<T> &a, const T *src, T *dst, T *cells, int mode,
                        const T *rhs, hipStream_t s);
@@ -81,10 +82,16 @@ namespace stfem
   // LDS-staged kernel, 3 the register-tiled kernel (patches of at most 128
   // nodes), 4 its software-pipelined persistent form (FP32); shapes a
   // variant does not cover fall back to the others.
+  // opc non-null (two-pass form only, shapes with
+  // vanka_fused_residual_supported): A x at the patch nodes is summed from
+  // the operator's cell-major buffer opc (StOpArgs's level, nt components)
+  // instead of read from ax, which saves the operator's node gather.
   template <typename T>
   void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *lam, const T *ta,
                              const T *tb, const T *rhs, const T *ax, T *u, T omega, T *cells,
-                             int mode, int variant, hipStream_t s);
+                             int mode, int variant, hipStream_t s, const T *opc = nullptr);
+  template <typename T>
+  bool vanka_fused_residual_supported(const GridDesc &g, int nt);
   // Node gather of a cell-major buffer (cell_slot layout, nto components):
   // mode 0 dst = sum, 1 dst += sum, 2 dst = rhs - sum; skip_dirichlet:
   // Dirichlet nodes are left out (zero for modes 0, 2, unchanged for 1).

@@ -506,6 +506,12 @@ namespace stfem
     apply(dst, src, dA_, dB_, 0);
   }
   template <typename T>
+  void SystemMatrix<T>::vmult_cells(const T *src) const
+  {
+    STFEM_REQUIRE(cells_.get(), "vmult_cells: the operator has no cell buffer");
+    apply(nullptr, src, dA_, dB_, 3);
+  }
+  template <typename T>
   void SystemMatrix<T>::set_cell_coefficient(const double *kappa)
   {
     std::vector<T> c(kappa, kappa + grid_.n_cells);
@@ -1058,13 +1064,34 @@ namespace stfem
     relax(dst, src, nullptr, T(1));
   }
   template <typename T>
-  void PreconditionVanka<T>::relax(T *u, const T *rhs, const T *ax, T omega) const
+  void PreconditionVanka<T>::relax(T *u, const T *rhs, const T *ax, T omega, bool assign) const
   {
     if (form_ == Form::explicit_inverse)
-      launch_vanka_explicit<T>(grid_, nt_, binv_.get(), rhs, ax, u, omega, ctx_.stream);
+      {
+        if (assign)
+          launch_fill_zero<T>(u, size_t(grid_.n_nodes) * nt_, ctx_.stream);
+        launch_vanka_explicit<T>(grid_, nt_, binv_.get(), rhs, ax, u, omega, ctx_.stream);
+      }
     else
       launch_vanka_factored<T>(grid_, nt_, S_.get(), lam_.get(), ta_.get(), tb_.get(), rhs, ax, u,
-                               omega, cells_.get(), 1, variant_, ctx_.stream);
+                               omega, cells_.get(), assign ? 0 : 1, variant_, ctx_.stream);
+  }
+  template <typename T>
+  bool PreconditionVanka<T>::fused_residual_ok() const
+  {
+    // the register-tiled kernel has the fused form (chosen automatically
+    // from 64-node patches on); the others keep the separate gather
+    const int nl = grid_.dim == 3 ? (grid_.p + 1) * (grid_.p + 1) * (grid_.p + 1) :
+                                    (grid_.p + 1) * (grid_.p + 1);
+    return form_ == Form::factored && cells_.get() &&
+           (variant_ == 3 || (variant_ == 0 && nl >= 64)) &&
+           vanka_fused_residual_supported<T>(grid_, nt_);
+  }
+  template <typename T>
+  void PreconditionVanka<T>::relax_opc(T *u, const T *rhs, const T *opc, T omega) const
+  {
+    launch_vanka_factored<T>(grid_, nt_, S_.get(), lam_.get(), ta_.get(), tb_.get(), rhs, nullptr,
+                             u, omega, cells_.get(), 1, variant_, ctx_.stream, opc);
   }
   template <typename T>
   double PreconditionVanka<T>::sweep_bytes() const
@@ -1492,10 +1519,16 @@ namespace stfem
   {
     const auto &L  = lv_[l];
     const long  sz = L.A->m();
+    (void)sz;
     if (first)
       {
-        launch_fill_zero<T>(u, size_t(sz), ctx_.stream);
-        L.V->relax(u, rhs, nullptr, omega);
+        L.V->relax(u, rhs, nullptr, omega, true);
+        return;
+      }
+    if (L.V->n_cells() <= opt_.fuse_residual && L.A->cell_buffer() && L.V->fused_residual_ok())
+      {
+        L.A->vmult_cells(u);
+        L.V->relax_opc(u, rhs, L.A->cell_buffer(), omega);
         return;
       }
     T *t1 = t1_[l].get();
@@ -1838,6 +1871,8 @@ namespace stfem
     getb("vanka_factored", ps.mg.vanka_factored);
     getb("exact_single_cell", ps.mg.exact_single_cell);
     getb("use_graph", ps.mg.use_graph);
+    if (kv.count("fuse_residual"))
+      ps.mg.fuse_residual = std::stol(kv["fuse_residual"]);
     if (kv.count("dense_coarse_dofs"))
       ps.mg.dense_coarse_dofs = std::stol(kv["dense_coarse_dofs"]);
     geti("maxiter", ps.solver.maxiter);

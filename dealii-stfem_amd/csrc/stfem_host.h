@@ -154,6 +154,9 @@ namespace stfem
                  bool two_pass = true);
     // dst = (Alpha x K + Beta x M) src   (operators.h:94-122)
     void vmult(T *dst, const T *src) const;
+    // the cell pass of vmult alone, into cell_buffer() (two-pass form)
+    void vmult_cells(const T *src) const;
+    const T *cell_buffer() const { return cells_.get(); }
     // dst = (Alpha^T x K + Beta^T x M) src   (operators.h:124-152)
     void Tvmult(T *dst, const T *src) const;
     // dst(all blocks) += (Alpha(:,0) x K + Beta(:,0) x M) src(one block)
@@ -212,9 +215,15 @@ namespace stfem
                       const DMat &Beta, Form form, int device_setup = 3);
     // dst = sum_c (1/valence) B_c^{-1} src|_c   (gmg.h:476-514)
     void   vmult(T *dst, const T *src) const;
-    // u += omega * Vanka(rhs - ax); ax may be null.  One relaxation sweep of
-    // PreconditionRelaxation with this preconditioner, in one kernel.
-    void   relax(T *u, const T *rhs, const T *ax, T omega) const;
+    // u += omega * Vanka(rhs - ax); ax may be null; assign: u = omega *
+    // Vanka(rhs - ax) (u need not be initialised).  One relaxation sweep of
+    // PreconditionRelaxation with this preconditioner.
+    void   relax(T *u, const T *rhs, const T *ax, T omega, bool assign = false) const;
+    // the same update with A x summed from the cell buffer of the level
+    // operator's cell pass (SystemMatrix::vmult_cells), when
+    // fused_residual_ok(): one node gather fewer per sweep
+    bool   fused_residual_ok() const;
+    void   relax_opc(T *u, const T *rhs, const T *opc, T omega) const;
     long   n_cells() const { return grid_.n_cells; }
     Form   form() const { return form_; }
     // factored sweep kernel (0 automatic, 2 LDS-staged, 3 register-tiled)
@@ -323,6 +332,12 @@ namespace stfem
     // that map as one dense matrix, built at setup from the level kernels
     // and applied as one matrix-vector product per V-cycle; 0 disables.
     long   dense_coarse_dofs       = 8192;
+    // relaxation sweeps on levels of at most this many cells take A u at
+    // the patch nodes from the operator's cell buffer (one node gather
+    // launch fewer per sweep, at the price of summing the node values in
+    // the sweep kernel) where that kernel supports it; 0 (default)
+    // disables: on the benchmark levels the two effects cancel
+    long   fuse_residual           = 0;
   };
 
   // Level data of the hierarchy (precision T).
