@@ -37,8 +37,10 @@ namespace stfem
     }
 
     // The (one to 2^dim) positions of node (gx, gy, gz)'s cell-local
-    // values, component 0, in the order of summation (cells by increasing
-    // x, then y, then z); returns how many there are.
+    // values, component 0, in the order of summation of gather_node (cells
+    // by increasing x, then y, then z); returns how many there are.
+    // gather_node keeps its own copy of this logic: built on this function
+    // the FP64 gather ran at half its speed.
     template <int DIM, int N>
     __device__ __forceinline__ int node_slots(const GridDesc &g, int nt, long gx, long gy, long gz,
                                               long slot[1 << DIM])
@@ -137,7 +139,13 @@ namespace stfem
       const long    gx  = node % nn0, r = node / nn0;
       const long    gy  = (DIM == 3) ? r % nn1 : r;
       const long    gz  = (DIM == 3) ? r / nn1 : 0;
-      const bool    bnd = SKIP_DIR && dirichlet_node<DIM>(g, gx, gy, gz);
+      const int     d   = g.dirichlet;
+      bool bnd = (gx == 0 && (d & 1)) || (gx == nn0 - 1 && (d & 2)) || (gy == 0 && (d & 4)) ||
+                 (gy == nn1 - 1 && (d & 8));
+      if (DIM == 3)
+        bnd = bnd || (gz == 0 && (d & 16)) || (gz == g.nn[2] - 1 && (d & 32));
+      if (!SKIP_DIR)
+        bnd = false;
       const int nt  = NT > 0 ? NT : nto;
       T *const  out = dst + node * nt;
       if (bnd)
@@ -147,9 +155,48 @@ namespace stfem
               out[j] = T(0);
           return;
         }
+      // the cells sharing the node and its local index in each, per direction
+      int cs[3][2] = {}, ls[3][2] = {}, nsh[3] = {1, 1, 1};
+      const long gc[3] = {gx, gy, gz};
+#pragma unroll
+      for (int e = 0; e < DIM; ++e)
+        {
+          int c = int(gc[e] / p), l = int(gc[e] - long(c) * p);
+          if (c == g.nc[e])
+            c = g.nc[e] - 1, l = p;
+          // lower neighbour first: fixed summation order
+          if (l == 0 && c > 0)
+            {
+              cs[e][0] = c - 1;
+              ls[e][0] = p;
+              cs[e][1] = c;
+              ls[e][1] = 0;
+              nsh[e]   = 2;
+            }
+          else
+            {
+              cs[e][0] = c;
+              ls[e][0] = l;
+            }
+        }
       constexpr int MAXS = 1 << DIM;
       long          slot[MAXS];
-      const int     ns = node_slots<DIM, N>(g, nt, gx, gy, gz, slot);
+      int           ns = 0;
+#pragma unroll
+      for (int kz = 0; kz < (DIM == 3 ? 2 : 1); ++kz)
+#pragma unroll
+        for (int ky = 0; ky < 2; ++ky)
+#pragma unroll
+          for (int kx = 0; kx < 2; ++kx)
+            {
+              const bool use = kx < nsh[0] && ky < nsh[1] && (DIM == 2 || kz < nsh[2]);
+              if (use)
+                slot[ns] = (DIM == 3) ? cell_slot<DIM, N>(g, nt, cs[0][kx], cs[1][ky], cs[2][kz],
+                                                          ls[0][kx], ls[1][ky], ls[2][kz], 0)
+                                      : cell_slot<DIM, N>(g, nt, cs[0][kx], cs[1][ky], 0,
+                                                          ls[0][kx], ls[1][ky], 0, 0);
+              ns += use ? 1 : 0;
+            }
       const long L = long(g.nc[0]) * N; // component stride of cell_slot
       if constexpr (NT > 0)
         {
