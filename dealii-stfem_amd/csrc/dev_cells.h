@@ -136,9 +136,12 @@ namespace stfem
     {
       constexpr int p   = N - 1;
       const long    nn0 = g.nn[0], nn1 = g.nn[1];
-      const long    gx  = node % nn0, r = node / nn0;
-      const long    gy  = (DIM == 3) ? r % nn1 : r;
-      const long    gz  = (DIM == 3) ? r / nn1 : 0;
+      // 32-bit splits (levels have fewer than 2^31 nodes)
+      const unsigned un  = unsigned(node), u0 = unsigned(nn0), u1 = unsigned(nn1);
+      const unsigned ur  = un / u0;
+      const long     gx  = long(un - ur * u0);
+      const long     gy  = (DIM == 3) ? long(ur % u1) : long(ur);
+      const long     gz  = (DIM == 3) ? long(ur / u1) : 0;
       const int     d   = g.dirichlet;
       bool bnd = (gx == 0 && (d & 1)) || (gx == nn0 - 1 && (d & 2)) || (gy == 0 && (d & 4)) ||
                  (gy == nn1 - 1 && (d & 8));

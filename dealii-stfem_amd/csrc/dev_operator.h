@@ -105,8 +105,10 @@ namespace stfem
       int       cx = 0, cy = 0, cz = 0;
       if (valid)
         {
-          cx     = int(cell % nc0);
-          long r = cell / nc0;
+          // 32-bit splits (levels have fewer than 2^31 cells)
+          const unsigned uc = unsigned(cell), u0 = unsigned(nc0);
+          const unsigned r  = uc / u0;
+          cx                = int(uc - r * u0);
           if (DIM == 3)
             {
               cy = int(r % nc1);

@@ -34,8 +34,10 @@ namespace stfem
 
       __device__ void cell_xyz(long cell, int &cx, int &cy, int &cz) const
       {
-        cx     = int(cell % nc0);
-        long r = cell / nc0;
+        // 32-bit splits (levels have fewer than 2^31 cells)
+        const unsigned uc = unsigned(cell), u0 = unsigned(nc0);
+        const unsigned r  = uc / u0;
+        cx                = int(uc - r * u0);
         if (dim == 3)
           {
             cy = int(r % nc1);

@@ -146,6 +146,9 @@ namespace stfem
             m.g.n_nodes *= m.g.nn[d];
           }
         m.g.dirichlet = dirichlet_mask;
+        // the kernels split node and cell indices with 32-bit arithmetic
+        STFEM_REQUIRE(m.g.n_nodes < (1L << 31) && m.g.n_cells < (1L << 31),
+                      "mesh: at most 2^31 nodes per level");
         const long nv0 = m.g.nc[0] + 1, nv1 = m.g.nc[1] + 1, nv2 = (dim == 3) ? m.g.nc[2] + 1 : 1;
         m.X.resize(size_t(nv0 * nv1 * nv2) * dim);
         for (long k = 0; k < nv2; ++k)
