@@ -3,8 +3,8 @@ oracle on the same seeded inputs.
 
 Operators are compared at the level precision (FP64 to 1e-12 relative, FP32
 to 2e-6); whole slab solves to 1e-10 relative in FP64 with identical
-iteration counts (north-star parity bar: 1e-12 on the solution difference
-up to the solver tolerance, iterations +-1).
+iteration counts (the parity bar: 1e-12 on the solution difference up to
+the solver tolerance, iterations +-1).
 """
 import numpy as np
 import pytest
@@ -30,16 +30,16 @@ CASES = {
     # distorted 2D mesh with a cell coefficient, DG(2), Q3
     "2d_dg2_distorted": dict(spec=dict(dim=2, refine=3, type=2, r=2, nts=1, p=3, r_min=0),
                              perturb=0.2, kappa=True),
-    # configs[1] shape (Q3, dG(2), perturbed 0.2h, log-uniform kappa), 4^3 cells
+    # the 32^3 benchmark's discretisation (Q3, dG(2), perturbed 0.2h, log-uniform kappa), 4^3 cells
     "3d_q3_dg2": dict(spec=dict(dim=3, refine=2, type=2, r=2, nts=1, p=3, r_min=0), perturb=0.2,
                       kappa=True),
     # the same with the factored Vanka inverse (eigen-decomposed patches)
     "3d_q3_dg2_factored": dict(spec=dict(dim=3, refine=2, type=2, r=2, nts=1, p=3, r_min=0,
                                          vanka_factored=1), perturb=0.2, kappa=True),
-    # configs[4] shape (Q4, dG(3)), 2^3 cells, FP32 multigrid
+    # the 64^3 benchmark's discretisation (Q4, dG(3)), 2^3 cells, FP32 multigrid
     "3d_q4_dg3_fp32": dict(spec=dict(dim=3, refine=1, type=2, r=3, nts=1, p=4, r_min=2, vanka_invert_T=0,
                                      mg_float=1), perturb=0.1, kappa=True),
-    # configs[4] shape with the factored Vanka set up on the device
+    # the same with the factored Vanka set up on the device
     "3d_q4_dg3_fp32_factored": dict(spec=dict(dim=3, refine=1, type=2, r=3, nts=1, p=4, r_min=2,
                                               mg_float=1, vanka_factored=1),
                                     perturb=0.1, kappa=True),
@@ -47,7 +47,7 @@ CASES = {
     "3d_q3_dg2_fp32_factored": dict(spec=dict(dim=3, refine=2, type=2, r=2, nts=1, p=3, r_min=0,
                                               mg_float=1, vanka_factored=1, vanka_invert_T=0),
                                     perturb=0.2, kappa=True),
-    # configs[2] time discretisation cG(2) with Q2 (heat operator), 4^3 cells
+    # the wave benchmark's time discretisation cG(2) with Q2 (heat operator), 4^3 cells
     "3d_q2_cg2": dict(spec=dict(dim=3, refine=2, type=1, r=2, nts=2, p=2, r_min=1), perturb=0.0,
                       kappa=False),
     # arithmetic-mean coarse coefficients, Q4 dG(3), FP32 factored Vanka

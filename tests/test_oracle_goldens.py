@@ -16,8 +16,8 @@ The relaxation parameter of the reference comes from deal.II's power
 iteration (gmg.h:792-844), whose start vector and safety factor are
 deal.II-internal: the oracle uses a constant start vector and a factor 1.15
 on the Rayleigh quotient, the best fit to these transcripts.  Iteration
-counts are therefore checked to +-0.25 iterations per slab (the north-star
-bar is +-1), with a floor on the number of exact matches.
+counts are therefore checked to +-0.25 iterations per slab (the parity bar
+for iteration counts is +-1), with a floor on the number of exact matches.
 """
 import json
 import os
