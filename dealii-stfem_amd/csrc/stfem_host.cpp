@@ -1474,7 +1474,10 @@ namespace stfem
     const auto  &L  = lv_[l];
     const long   sz = L.A->m();
     T           *ev = sol_[l].get(), *v1 = t1_[l].get(), *v2 = t2_[l].get();
+    // start vector: ones, zero at the first unknown (node 0, time 0) and at
+    // the Dirichlet nodes
     std::vector<T> ones(sz, T(1));
+    ones[0] = T(0);
     STFEM_HIP(hipMemcpyAsync(ev, ones.data(), sz * sizeof(T), hipMemcpyHostToDevice, ctx_.stream));
     launch_zero_dirichlet<T>(L.A->grid(), L.A->n_blocks(), ev, ctx_.stream);
     double *dsc = ctx_.scalars.get();
@@ -1482,6 +1485,8 @@ namespace stfem
     launch_dot<T>(sz, ev, ev, ctx_.partial.get(), dsc, 1, ctx_.stream);
     STFEM_HIP(hipMemcpyAsync(h, dsc, sizeof(double), hipMemcpyDeviceToHost, ctx_.stream));
     ctx_.sync();
+    if (!(h[0] > 0.0))
+      return 1.0; // no free unknowns on this level (all nodes on Dirichlet faces)
     launch_axpby<T>(sz, T(1.0 / h[0]), ev, T(0), ev, ctx_.stream);
     double est = 0;
     for (int it = 0; it < opt_.eig_iterations; ++it)
@@ -1495,6 +1500,8 @@ namespace stfem
         est = h[0];
         launch_axpby<T>(sz, T(1.0 / h[1]), v1, T(0), ev, ctx_.stream);
       }
+    if (!std::isfinite(est) || est == 0.0)
+      return 1.0;
     const double lmax = std::abs(est) * opt_.eig_safety;
     return 2.0 / (0.9 * lmax + lmax);
   }

@@ -1824,6 +1824,10 @@ namespace orc
             v1[i] = T(v1[i] / n1);
           ev.swap(v1);
         }
+      // a level without free unknowns (every node on a Dirichlet face) has
+      // nothing to estimate
+      if (!std::isfinite(est) || est == 0.0)
+        return 1.0;
       double lmax  = std::abs(est) * spec.eig_safety;
       double alpha = spec.omega_rule == 0 ? 0.9 * lmax : 0.0;
       return spec.omega_scale * 2.0 / (alpha + lmax);

@@ -50,6 +50,15 @@ CASES = {
     # the wave benchmark's time discretisation cG(2) with Q2 (heat operator), 4^3 cells
     "3d_q2_cg2": dict(spec=dict(dim=3, refine=2, type=1, r=2, nts=2, p=2, r_min=1), perturb=0.0,
                       kappa=False),
+    # lowest degrees: Q1 space, dG(0) time with two steps per slab (tau levels
+    # only), perturbed mesh and cell coefficient, factored Vanka, no Dirichlet
+    # faces (with Q1 the one-cell coarse level would have no free node)
+    "3d_q1_dg0_nts2": dict(spec=dict(dim=3, refine=2, type=2, r=0, nts=2, p=1, r_min=0,
+                                     vanka_factored=1, dirichlet=0), perturb=0.2, kappa=True),
+    # the same with Dirichlet faces: the one-cell levels have no free node
+    # (relaxation parameter 1, smoother without effect there)
+    "3d_q1_dg0_dirichlet": dict(spec=dict(dim=3, refine=2, type=2, r=0, nts=2, p=1, r_min=0,
+                                          vanka_factored=1), perturb=0.2, kappa=True),
     # arithmetic-mean coarse coefficients, Q4 dG(3), FP32 factored Vanka
     "3d_q4_dg3_cc3": dict(spec=dict(dim=3, refine=2, type=2, r=3, nts=1, p=4, r_min=2,
                                     mg_float=1, vanka_factored=1, coarse_coef=3,

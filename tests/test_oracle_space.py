@@ -173,3 +173,16 @@ def test_coarse_coefficient_means_on_a_checkerboard():
         energy = (x @ P.level_apply(l, 0, x) - x @ P1.level_apply(l, 0, x)) / A[0, 0] + 1.0
         expect = 2.0 * (lo + hi) * w0 * w0 + (1.0 - 4.0 * w0 * w0) * mean
         assert abs(energy - expect) < 1e-11 * hi, (mode, energy, expect)
+
+
+def test_levels_without_free_nodes():
+    """Q1 on a mesh whose coarsest level is one cell with Dirichlet faces: that
+    level has no free node, its relaxation parameter is 1 (nothing to
+    estimate) and the slab solve still converges."""
+    spec = dict(dim=3, refine=2, type=2, r=0, nts=2, p=1, r_min=0, vanka_factored=1)
+    P = orc.Problem(spec)
+    om = [P.omega(l) for l in range(P.n_levels)]
+    assert all(np.isfinite(om)), om
+    n, nt = P.level_info(P.n_levels - 1)["n_nodes"], P.level_info(P.n_levels - 1)["nt"]
+    rc, x, its = orc.solve_slab(P, np.zeros(n * nt), np.zeros(n), 0.0)
+    assert rc == 0 and np.all(np.isfinite(x)), (rc, its)
