@@ -59,6 +59,10 @@ CASES = {
     # (relaxation parameter 1, smoother without effect there)
     "3d_q1_dg0_dirichlet": dict(spec=dict(dim=3, refine=2, type=2, r=0, nts=2, p=1, r_min=0,
                                           vanka_factored=1), perturb=0.2, kappa=True),
+    # highest space degree: Q5 (216-node patches, the runtime-shape factored
+    # kernel), dG(1), 2^3 cells, FP32 multigrid with device Vanka setup
+    "3d_q5_dg1_fp32": dict(spec=dict(dim=3, refine=1, type=2, r=1, nts=1, p=5, r_min=0,
+                                     mg_float=1, vanka_factored=1), perturb=0.1, kappa=True),
     # arithmetic-mean coarse coefficients, Q4 dG(3), FP32 factored Vanka
     "3d_q4_dg3_cc3": dict(spec=dict(dim=3, refine=2, type=2, r=3, nts=1, p=4, r_min=2,
                                     mg_float=1, vanka_factored=1, coarse_coef=3,
