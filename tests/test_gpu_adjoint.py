@@ -125,3 +125,18 @@ def test_adjoint_config4_central_differences():
         worst = max(worst, rel)
     assert worst < 1e-6, worst
     P.close()
+
+
+@pytest.mark.parametrize("spec,msg", [
+    (dict(dim=2, refine=2, p=2, type=1, r=2, nts=1, r_min=1, tau=0.1), "dG time"),
+    (dict(dim=2, refine=2, p=2, type=1, r=2, nts=1, r_min=1, tau=0.1, problem=1), "heat problem"),
+])
+def test_adjoint_rejects_unsupported_discretisations(spec, msg):
+    """The adjoint covers dG heat slabs; a cG or wave problem gets an error
+    status (no device work), with a message naming the restriction."""
+    P = st.Problem(spec)
+    obs = torch.zeros(2 * P.n_nodes * P.nt, dtype=torch.float64, device=DEV)
+    grad = torch.zeros(P.n_cells, dtype=torch.float64, device=DEV)
+    with pytest.raises(st.StfemError, match=msg):
+        P.adjoint_gradient(obs, 2, grad)
+    P.close()
