@@ -50,6 +50,14 @@ namespace stfem
       constexpr int NL = (DIM == 3) ? N * N * N : N * N, NQ = NL;
       __shared__ double jxw[NQ];
       __shared__ double G[NQ][6]; // kappa JxW J^{-1} J^{-T}, upper triangle
+      // the 1D tables in LDS: the entry loop indexes them per lane, which
+      // from the kernel-argument block would be a memory load per use
+      __shared__ double Sl[N][N], Dl[N][N];
+      for (int e = threadIdx.x; e < N * N; e += blockDim.x)
+        {
+          Sl[e / N][e % N] = tb.S[e / N][e % N];
+          Dl[e / N][e % N] = tb.Sd[e / N][e % N];
+        }
       const long cell = c0 + blockIdx.x;
       int        cx, cy, cz;
       cell_pos(g, cell, cx, cy, cz);
@@ -130,14 +138,14 @@ namespace stfem
           for (int q = 0; q < NQ; ++q)
             {
               const int    qa = q % N, qb = (q / N) % N, qc = (DIM == 3) ? q / (N * N) : 0;
-              const double sxi = tb.S[qa][ai], syi = tb.S[qb][bi];
-              const double szi = (DIM == 3) ? tb.S[qc][ci] : 1.0;
-              const double sxj = tb.S[qa][aj], syj = tb.S[qb][bj];
-              const double szj = (DIM == 3) ? tb.S[qc][cj] : 1.0;
-              const double gi[3] = {tb.Sd[qa][ai] * syi * szi, sxi * tb.Sd[qb][bi] * szi,
-                                    DIM == 3 ? sxi * syi * tb.Sd[qc][ci] : 0.0};
-              const double gj[3] = {tb.Sd[qa][aj] * syj * szj, sxj * tb.Sd[qb][bj] * szj,
-                                    DIM == 3 ? sxj * syj * tb.Sd[qc][cj] : 0.0};
+              const double sxi = Sl[qa][ai], syi = Sl[qb][bi];
+              const double szi = (DIM == 3) ? Sl[qc][ci] : 1.0;
+              const double sxj = Sl[qa][aj], syj = Sl[qb][bj];
+              const double szj = (DIM == 3) ? Sl[qc][cj] : 1.0;
+              const double gi[3] = {Dl[qa][ai] * syi * szi, sxi * Dl[qb][bi] * szi,
+                                    DIM == 3 ? sxi * syi * Dl[qc][ci] : 0.0};
+              const double gj[3] = {Dl[qa][aj] * syj * szj, sxj * Dl[qb][bj] * szj,
+                                    DIM == 3 ? sxj * syj * Dl[qc][cj] : 0.0};
               m += sxi * syi * szi * sxj * syj * szj * jxw[q];
               const double *Gq = G[q];
               k += gi[0] * (Gq[0] * gj[0] + Gq[1] * gj[1] + Gq[2] * gj[2]) +
