@@ -1912,6 +1912,16 @@ namespace stfem
     STFEM_REQUIRE(ps.r >= (ps.type == TimeStepType::DG ? 0 : 1), "spec: bad time degree");
     STFEM_REQUIRE(ps.nts >= 1 && (ps.nts & (ps.nts - 1)) == 0,
                   "spec: n_timesteps_at_once must be a power of two");
+    STFEM_REQUIRE(ps.vanka_device_setup >= 0 && ps.vanka_device_setup <= 5,
+                  "spec: vanka_device_setup must be in [0, 5]");
+    STFEM_REQUIRE(ps.solver.orthogonalization == 0 || ps.solver.orthogonalization == 1,
+                  "spec: orthogonalization must be 0 or 1");
+    STFEM_REQUIRE(ps.op_jb >= 1 && ps.op_jb <= 4 && ps.op_jb64 >= 0 && ps.op_jb64 <= 4,
+                  "spec: op_jb must be in [1, 4], op_jb64 in [0, 4]");
+    STFEM_REQUIRE(ps.vanka_kernel == 0 || (ps.vanka_kernel >= 2 && ps.vanka_kernel <= 4),
+                  "spec: vanka_kernel must be 0, 2, 3 or 4");
+    STFEM_REQUIRE(ps.coarse_coef >= 0 && ps.coarse_coef <= 3, "spec: coarse_coef must be in [0, 3]");
+    STFEM_REQUIRE(ps.solver.basis >= 2 && ps.solver.basis <= 60, "spec: basis must be in [2, 60]");
     return ps;
   }
 

@@ -40,3 +40,19 @@ def test_errors_are_reported():
     assert rc == -1 and not h.value
     rc = L.stfem_problem_create(b"dim=2;nts=3", None, None, 0, ctypes.byref(h))
     assert rc == -1 and b"power of two" in L.stfem_last_error()
+
+
+def test_spec_values_are_checked():
+    """Out-of-range solver and kernel keys are rejected with a message before
+    any device work (these run on a host without a GPU)."""
+    L = st.load()
+    h = ctypes.c_void_p()
+    for spec, msg in [(b"dim=2;vanka_device_setup=9", b"vanka_device_setup"),
+                      (b"dim=2;orthogonalization=2", b"orthogonalization"),
+                      (b"dim=2;op_jb=5", b"op_jb"),
+                      (b"dim=2;vanka_kernel=1", b"vanka_kernel"),
+                      (b"dim=2;coarse_coef=4", b"coarse_coef"),
+                      (b"dim=2;basis=61", b"basis")]:
+        rc = L.stfem_problem_create(spec, None, None, 0, ctypes.byref(h))
+        assert rc == -1 and not h.value, spec
+        assert msg in L.stfem_last_error(), (spec, L.stfem_last_error())
