@@ -393,6 +393,9 @@ def main():
     vm32_bytes = 4.0 * 2 * nst + 4.0 * g32.n_cells + 8.0 * 3 * nv
     vk_ms = g32.time_kernel(2, L, warmup=2, reps=10)
     vk_bytes = g32.bytes_model(1, L)
+    # the transfer between the two finest levels
+    tp_ms, tp_bytes = g32.time_kernel(4, L, warmup=2, reps=10), g32.bytes_model(2, L)
+    tr_ms, tr_bytes = g32.time_kernel(5, L, warmup=2, reps=10), g32.bytes_model(3, L)
     u32 = None if a.no_fp64 else final_state(g32)
     g32.close()
 
@@ -464,6 +467,9 @@ def main():
                        "dofs_per_s": nst / (vm64_ms * 1e-3)},
         "vmult_fp32": {"ms": vm32_ms, "GBps": vm32_bytes / (vm32_ms * 1e-3) / 1e9},
         "vanka_fp32": {"ms": vk_ms, "GBps": vk_bytes / (vk_ms * 1e-3) / 1e9},
+        "transfer_fp32": {"levels": f"{L - 1} <-> {L}",
+                          "prolongate": {"ms": tp_ms, "GBps": tp_bytes / (tp_ms * 1e-3) / 1e9},
+                          "restrict": {"ms": tr_ms, "GBps": tr_bytes / (tr_ms * 1e-3) / 1e9}},
         "roofline": {"kernel": dom["kernel"], "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes"] if traffic else None,

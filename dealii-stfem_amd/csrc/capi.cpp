@@ -425,10 +425,23 @@ extern "C"
       auto &pb = P(p);
       if (what == 0)
         *bytes = pb.fine_vmult_bytes();
-      else
+      else if (what == 1)
         {
           check_level(pb, level, false);
           *bytes = pb.vanka_sweep_bytes(level);
+        }
+      else
+        {
+          // transfer between level-1 and level: read the source, read and
+          // write the destination
+          STFEM_REQUIRE(what == 2 || what == 3, "unknown bytes-model selector");
+          check_level(pb, level, false);
+          STFEM_REQUIRE(level >= 1, "transfers need a level above 0");
+          long fi[12], ci[12];
+          pb.level_info(level, fi);
+          pb.level_info(level - 1, ci);
+          const double s = double(fi[9]), nf = double(fi[0]) * fi[1], nc = double(ci[0]) * ci[1];
+          *bytes = what == 2 ? s * (nc + 2 * nf) : s * (nf + 2 * nc);
         }
     });
   }
@@ -460,9 +473,11 @@ extern "C"
       if (what == 0 || what == 3)
         level = pb.n_levels() - 1;
       check_level(pb, level, false);
+      STFEM_REQUIRE(what < 4 || level >= 1, "transfers need a level above 0");
       pb.level_info(level, li);
       const size_t n  = size_t(li[0]) * li[1];
       const size_t es = (what == 0 || what == 3) ? sizeof(double) : size_t(li[9]);
+      // (transfers: the finer level is the larger of the two vectors)
       DBuf<unsigned char> a(n * es), b(n * es);
       {
         std::mt19937_64                  rng(12345);
@@ -489,6 +504,8 @@ extern "C"
             case 1: pb.level_vmult(level, b.get(), a.get(), false); break;
             case 2: pb.level_vanka(level, b.get(), a.get()); break;
             case 3: pb.gmg_vmult((double *)b.get(), (const double *)a.get()); break;
+            case 4: pb.level_prolongate(level, b.get(), a.get()); break;
+            case 5: pb.level_restrict(level, b.get(), a.get()); break;
             default: throw Error(STATUS_ERR_ARG, "unknown kernel selector");
           }
       };

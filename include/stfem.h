@@ -201,7 +201,8 @@ int stfem_adjoint_gradient(stfem_problem p, const double *obs, int n_slabs, doub
  * multigrid preconditioners keep the coefficient they were built with */
 int stfem_set_cell_coefficient(stfem_problem p, const double *kappa);
 /* algorithmic HBM bytes of one application: what 0 fine vmult, 1 Vanka
- * sweep on `level` */
+ * sweep on `level`, 2 prolongate_and_add into `level`, 3 restrict_and_add
+ * from `level` (source read, destination read and written) */
 int stfem_bytes_model(stfem_problem p, int what, int level, double *bytes);
 /* [t][node] <-> [node][t] on the device (double) */
 int stfem_blocks_to_nodes(long n_nodes, int nt, const double *blocks, double *nodes);
@@ -209,7 +210,8 @@ int stfem_nodes_to_blocks(long n_nodes, int nt, const double *nodes, double *blo
 
 /* ---- measurement (HIP events on the library stream) ----------------- */
 /* average milliseconds of one application over `reps` after `warmup`:
- * what 0 fine FP64 vmult, 1 level vmult, 2 level Vanka, 3 GMG V-cycle */
+ * what 0 fine FP64 vmult, 1 level vmult, 2 level Vanka, 3 GMG V-cycle,
+ * 4 prolongate_and_add into `level`, 5 restrict_and_add from `level` */
 int stfem_time_kernel(stfem_problem p, int what, int level, int warmup, int reps, double *ms);
 /* runs `n_slabs` heat slabs from a zero initial value; returns the summed
  * slab-solve time (ms, HIP events) and the summed FGMRES iterations.  Slabs

@@ -335,6 +335,21 @@ def test_fused_residual_sweeps(name):
     b.close()
 
 
+def test_transfer_timing_and_bytes_model():
+    """The measurement entry points for the transfers between the two finest
+    levels: algorithmic bytes (source read, destination read and written)
+    and HIP-event timings."""
+    spec, o, g = problems("3d_q3_dg2")
+    L = g.n_levels - 1
+    fi, ci = g.level_info(L), g.level_info(L - 1)
+    nf, nc, s = fi["n_nodes"] * fi["nt"], ci["n_nodes"] * ci["nt"], g.level_size
+    assert g.bytes_model(2, L) == s * (nc + 2 * nf)
+    assert g.bytes_model(3, L) == s * (nf + 2 * nc)
+    print("[transfer timing]", flush=True)
+    assert g.time_kernel(4, L, warmup=1, reps=2) > 0
+    assert g.time_kernel(5, L, warmup=1, reps=2) > 0
+
+
 def test_graph_replay_matches_eager():
     """The V-cycle replayed from a captured HIP graph equals the eagerly
     launched V-cycle on every replay, with other device work in between."""
