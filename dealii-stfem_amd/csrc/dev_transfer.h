@@ -18,42 +18,45 @@ namespace stfem
 {
   namespace
   {
-    // entry idx = node*ntd + a of dst: (=|+=) sum_b M(a,b) src[node][b]
-    template <typename T>
-    __device__ __forceinline__ void time_transfer_item(long idx, const T *__restrict__ M, int ntd,
+    // entry idx = node*ntd + a of dst: (=|+=) sum_b M(a,b) src[node][b]; I is the
+    // index type (32-bit unsigned whenever every index fits, which avoids the
+    // emulated 64-bit divisions)
+    template <typename T, typename I>
+    __device__ __forceinline__ void time_transfer_item(I idx, const T *__restrict__ M, int ntd,
                                                        int nts, const T *__restrict__ src,
                                                        T *__restrict__ dst, int add)
     {
-      const long node = idx / ntd;
-      const int  a    = int(idx - node * ntd);
-      T          s    = T(0);
+      const I   node = idx / I(ntd);
+      const int a    = int(idx - node * I(ntd));
+      T         s    = T(0);
       for (int b = 0; b < nts; ++b)
-        s += M[a * nts + b] * src[node * nts + b];
+        s += M[a * nts + b] * src[node * I(nts) + I(b)];
       dst[idx] = add ? dst[idx] + s : s;
     }
 
     // entry idx of the output of a 1D banded operator along direction d:
-    // output dims are n with n[d] replaced by m, nt components interleaved
-    template <typename T>
-    __device__ __forceinline__ void sweep_item(long idx, const T *__restrict__ in,
+    // output dims are n with n[d] replaced by m, nt components interleaved;
+    // I as for time_transfer_item, covering the input and the output
+    template <typename T, typename I>
+    __device__ __forceinline__ void sweep_item(I idx, const T *__restrict__ in,
                                                T *__restrict__ out, int nt, int d, int n0, int n1,
                                                int m, const int *__restrict__ cnt,
                                                const int *__restrict__ col,
                                                const T *__restrict__ w, int K, int add)
     {
-      const long o0 = (d == 0) ? m : n0, o1 = (d == 1) ? m : n1;
-      const long pt   = idx / nt;
-      const int  comp = int(idx - pt * nt);
-      const long i0 = pt % o0, r = pt / o0, i1 = r % o1, i2 = r / o1;
-      const long od = (d == 0) ? i0 : (d == 1) ? i1 : i2;
-      const long stride = (d == 0) ? 1 : (d == 1) ? (long)n0 : (long)n0 * n1;
+      const I   o0 = I((d == 0) ? m : n0), o1 = I((d == 1) ? m : n1);
+      const I   pt   = idx / I(nt);
+      const int comp = int(idx - pt * I(nt));
+      const I   i0 = pt % o0, r = pt / o0, i1 = r % o1, i2 = r / o1;
+      const int od = int((d == 0) ? i0 : (d == 1) ? i1 : i2);
+      const I   stride = (d == 0) ? I(1) : (d == 1) ? I(n0) : I(n0) * I(n1);
       // input index with the d coordinate set to 0
-      const long j0 = (d == 0) ? 0 : i0, j1 = (d == 1) ? 0 : i1, j2 = (d == 2) ? 0 : i2;
-      const long ib = j0 + (long)n0 * (j1 + (long)n1 * j2);
-      const int  c  = cnt[od];
-      T          s  = T(0);
+      const I j0 = (d == 0) ? I(0) : i0, j1 = (d == 1) ? I(0) : i1, j2 = (d == 2) ? I(0) : i2;
+      const I ib = j0 + I(n0) * (j1 + I(n1) * j2);
+      const int c = cnt[od];
+      T         s = T(0);
       for (int k = 0; k < c; ++k)
-        s += w[od * K + k] * in[(ib + (long)col[od * K + k] * stride) * nt + comp];
+        s += w[od * K + k] * in[(ib + I(col[od * K + k]) * stride) * I(nt) + I(comp)];
       out[idx] = add ? out[idx] + s : s;
     }
 

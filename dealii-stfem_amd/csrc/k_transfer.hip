@@ -10,24 +10,28 @@ namespace stfem
 {
   namespace
   {
-    template <typename T>
-    __global__ void k_time_transfer(long total, const T *__restrict__ M, int ntd, int nts,
+    template <typename T, typename I>
+    __global__ void k_time_transfer(I total, const T *__restrict__ M, int ntd, int nts,
                                     const T *__restrict__ src, T *__restrict__ dst, int add)
     {
-      const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      const I idx = I(blockIdx.x) * I(blockDim.x) + I(threadIdx.x);
       if (idx < total)
-        time_transfer_item(idx, M, ntd, nts, src, dst, add);
+        time_transfer_item<T, I>(idx, M, ntd, nts, src, dst, add);
     }
 
-    template <typename T>
-    __global__ void k_sweep(long total, const T *__restrict__ in, T *__restrict__ out, int nt,
+    template <typename T, typename I>
+    __global__ void k_sweep(I total, const T *__restrict__ in, T *__restrict__ out, int nt,
                             int d, int n0, int n1, int m, const int *__restrict__ cnt,
                             const int *__restrict__ col, const T *__restrict__ w, int K, int add)
     {
-      const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      const I idx = I(blockIdx.x) * I(blockDim.x) + I(threadIdx.x);
       if (idx < total)
-        sweep_item(idx, in, out, nt, d, n0, n1, m, cnt, col, w, K, add);
+        sweep_item<T, I>(idx, in, out, nt, d, n0, n1, m, cnt, col, w, K, add);
     }
+
+    // 32-bit indices are used when every index, and every thread index of the
+    // rounded-up grid, stays below this
+    constexpr long k_max_u32 = 4294967295L - 256;
     __global__ void k_wave_velocity(long n_nodes, int nts, int ni, int dg,
                                     const double *__restrict__ AixB,
                                     const double *__restrict__ AixG,
@@ -75,8 +79,13 @@ namespace stfem
     const long total = n_nodes * ntd;
     if (total == 0)
       return;
-    k_time_transfer<T><<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(
-      total, M, ntd, nts, src, dst, add ? 1 : 0);
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (n_nodes * std::max(ntd, nts) <= k_max_u32)
+      k_time_transfer<T, unsigned><<<grid, dim3(256), 0, s>>>(unsigned(total), M, ntd, nts, src,
+                                                             dst, add ? 1 : 0);
+    else
+      k_time_transfer<T, unsigned long><<<grid, dim3(256), 0, s>>>(total, M, ntd, nts, src, dst,
+                                                                  add ? 1 : 0);
     STFEM_HIP(hipGetLastError());
   }
 
@@ -87,8 +96,15 @@ namespace stfem
     const long total = sweep_total(nt, d, n_in, m_out);
     if (total == 0)
       return;
-    k_sweep<T><<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s>>>(
-      total, in, out, nt, d, n_in[0], n_in[1], m_out, cnt, col, w, K, add ? 1 : 0);
+    const long total_in = (long)n_in[0] * n_in[1] * n_in[2] * nt;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (std::max(total, total_in) <= k_max_u32)
+      k_sweep<T, unsigned><<<grid, dim3(256), 0, s>>>(unsigned(total), in, out, nt, d, n_in[0],
+                                                      n_in[1], m_out, cnt, col, w, K, add ? 1 : 0);
+    else
+      k_sweep<T, unsigned long><<<grid, dim3(256), 0, s>>>(total, in, out, nt, d, n_in[0],
+                                                           n_in[1], m_out, cnt, col, w, K,
+                                                           add ? 1 : 0);
     STFEM_HIP(hipGetLastError());
   }
 
