@@ -91,9 +91,9 @@ def test_tp01_transcript_on_gpu():
                                   dict(dim=3, refine=2, type=2, r=1, nts=2)])
 def test_wave_slabs_match_oracle(spec):
     """TimeIntegratorWave::solve on the GPU against the oracle's, slab by
-    slab: displacements and velocities of every block and the FGMRES
-    iteration counts (the right-hand side from prev_v enters from the
-    second slab on, where the velocities are non-zero)."""
+    slab: displacements and velocities of every block to 1e-12 relative and
+    the FGMRES iteration counts (the right-hand side from prev_v enters from
+    the second slab on, where the velocities are non-zero)."""
     spec = dict(spec, problem=1, mg_float=1)
     o, g = orc.Problem(spec), st.Problem(spec)
     n, nt = g.n_nodes, g.nt
@@ -112,8 +112,10 @@ def test_wave_slabs_match_oracle(spec):
         rc_g, its_g = g.solve_slab_wave(xg, vg, pu, pv, time)
         assert rc_o == 0 and rc_g == 0
         assert abs(its_g - its_o) <= 1, (k, its_g, its_o)
-        assert rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo) < 1e-7, k
-        assert rel_err(nodes_to_blocks(vg.cpu().numpy(), nt), vo) < 1e-7, k
+        eu = rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo)
+        ev = rel_err(nodes_to_blocks(vg.cpu().numpy(), nt), vo)
+        print(f"  its {its_g} (oracle {its_o}), rel diff u {eu:.2e} v {ev:.2e}", flush=True)
+        assert eu < 1e-12 and ev < 1e-12, (k, eu, ev)
         time += spec["nts"] * g.tau
     g.close()
 
@@ -146,7 +148,7 @@ def test_wave_layered_pulse_matches_oracle():
     """The 3D wave workload's setting at 4^3 cells (Q2, cG(2), c^2 = 1 | 4
     split at z = 0.5, Gaussian pulse initial displacement, zero velocity, no
     source): three slabs against the oracle's TimeIntegratorWave::solve,
-    displacements and velocities of every block."""
+    displacements and velocities of every block to 1e-12 relative."""
     import sys
     import os
     sys.path.insert(0, os.path.join(os.path.dirname(__file__), ".."))
@@ -169,7 +171,9 @@ def test_wave_layered_pulse_matches_oracle():
         rc_g, its_g = g.solve_slab_wave(xg, vg, pu, pv, k * g.tau)
         assert rc_o == 0 and rc_g == 0
         assert abs(its_g - its_o) <= 1, (k, its_g, its_o)
-        assert rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo) < 1e-7, k
-        assert rel_err(nodes_to_blocks(vg.cpu().numpy(), nt), vo) < 1e-7, k
+        eu = rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo)
+        ev = rel_err(nodes_to_blocks(vg.cpu().numpy(), nt), vo)
+        print(f"  its {its_g} (oracle {its_o}), rel diff u {eu:.2e} v {ev:.2e}", flush=True)
+        assert eu < 1e-12 and ev < 1e-12, (k, eu, ev)
         pu_o, pv_o = xo[(nt - 1) * n:].copy(), vo[(nt - 1) * n:].copy()
     g.close()

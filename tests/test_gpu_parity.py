@@ -2,9 +2,10 @@
 oracle on the same seeded inputs.
 
 Operators are compared at the level precision (FP64 to 1e-12 relative, FP32
-to 2e-6); whole slab solves to 1e-10 relative in FP64 with identical
-iteration counts (the parity bar: 1e-12 on the solution difference up to
-the solver tolerance, iterations +-1).
+to 2e-6); whole slab solves (to a 1e-10 relative residual) to 1e-12 relative
+solution difference, with identical iteration counts in FP64 and within one
+iteration with FP32 multigrid (measured: identical everywhere, differences
+1e-15 to 1.1e-13).
 """
 import numpy as np
 import pytest
@@ -258,7 +259,9 @@ def test_slab_solves(name):
         else:
             assert abs(its_g - its_o) <= 1, (k, its_g, its_o)
         err = rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo)
-        assert err < (1e-9 if g.level_size == 8 else 1e-7), (k, err)
+        print(f"[case {name}] slab {k}: its {its_g} (oracle {its_o}), rel diff {err:.2e}",
+              flush=True)
+        assert err < 1e-12, (k, err)
         time += spec["nts"] * g.tau
 
 
@@ -270,8 +273,8 @@ def test_fgmres_residual_history(name, ortho):
     seeded): with either Gram-Schmidt form (0: chain of axpy+dot passes, 1:
     one pass of dots corrected by the Gram matrix) the GPU solver takes the
     oracle's iterations and its residual history (SolverFGMRES's projected
-    residuals) agrees within 1e-10 of the initial residual; FP32 multigrid:
-    within one iteration and 1e-6."""
+    residuals) agrees within 1e-10 of the initial residual (FP32 multigrid:
+    within one iteration and 1e-6 of it); the solutions agree to 1e-12."""
     spec, o, _ = problems(name)
     c = CASES[name]
     dim = spec["dim"]
@@ -297,7 +300,9 @@ def test_fgmres_residual_history(name, ortho):
     dev = np.abs(hg[:k] - ho[:k]).max() / ho[0]
     assert dev < (1e-10 if fp64 else 1e-6), dev
     err = rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo)
-    assert err < (1e-9 if fp64 else 1e-6), err
+    print(f"[case {name}] its {its_g} (oracle {its_o}), history dev {dev:.2e}, "
+          f"rel diff {err:.2e}", flush=True)
+    assert err < 1e-12, err
     g.close()
 
 
