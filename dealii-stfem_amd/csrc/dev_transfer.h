@@ -36,8 +36,13 @@ namespace stfem
 
     // entry idx of the output of a 1D banded operator along direction d:
     // output dims are n with n[d] replaced by m, nt components interleaved;
-    // I as for time_transfer_item, covering the input and the output
-    template <typename T, typename I>
+    // I as for time_transfer_item, covering the input and the output.  With
+    // KM > 0 (row stride K <= KM) the count and the column and weight loads
+    // of all K taps are issued together, then the input loads of the row's
+    // cnt taps, so that a thread waits for two memory round trips instead of
+    // one per tap (padded taps: column 0, weight 0, never summed); the taps
+    // are summed in order as with KM = 0.
+    template <typename T, typename I, int KM>
     __device__ __forceinline__ void sweep_item(I idx, const T *__restrict__ in,
                                                T *__restrict__ out, int nt, int d, int n0, int n1,
                                                int m, const int *__restrict__ cnt,
@@ -55,8 +60,31 @@ namespace stfem
       const I ib = j0 + I(n0) * (j1 + I(n1) * j2);
       const int c = cnt[od];
       T         s = T(0);
-      for (int k = 0; k < c; ++k)
-        s += w[od * K + k] * in[(ib + I(col[od * K + k]) * stride) * I(nt) + I(comp)];
+      if constexpr (KM == 0)
+        {
+          for (int k = 0; k < c; ++k)
+            s += w[od * K + k] * in[(ib + I(col[od * K + k]) * stride) * I(nt) + I(comp)];
+        }
+      else
+        {
+          int cl[KM];
+          T   wl[KM], xl[KM];
+#pragma unroll
+          for (int k = 0; k < KM; ++k)
+            if (k < K)
+              {
+                cl[k] = col[od * K + k];
+                wl[k] = w[od * K + k];
+              }
+#pragma unroll
+          for (int k = 0; k < KM; ++k)
+            if (k < c)
+              xl[k] = in[(ib + I(cl[k]) * stride) * I(nt) + I(comp)];
+#pragma unroll
+          for (int k = 0; k < KM; ++k)
+            if (k < c)
+              s += wl[k] * xl[k];
+        }
       out[idx] = add ? out[idx] + s : s;
     }
 

@@ -19,14 +19,33 @@ namespace stfem
         time_transfer_item<T, I>(idx, M, ntd, nts, src, dst, add);
     }
 
-    template <typename T, typename I>
+    template <typename T, typename I, int KM>
     __global__ void k_sweep(I total, const T *__restrict__ in, T *__restrict__ out, int nt,
                             int d, int n0, int n1, int m, const int *__restrict__ cnt,
                             const int *__restrict__ col, const T *__restrict__ w, int K, int add)
     {
       const I idx = I(blockIdx.x) * I(blockDim.x) + I(threadIdx.x);
       if (idx < total)
-        sweep_item<T, I>(idx, in, out, nt, d, n0, n1, m, cnt, col, w, K, add);
+        sweep_item<T, I, KM>(idx, in, out, nt, d, n0, n1, m, cnt, col, w, K, add);
+    }
+
+    // k_sweep with the smallest tap bound KM >= K for rows of up to 8 taps
+    // (prolongation: p + 1); the wider restriction rows (up to 4p + 1 taps,
+    // most rows using about half) run faster with the tap loop (KM = 0)
+    template <typename T, typename I>
+    void launch_sweep_k(dim3 grid, hipStream_t s, I total, const T *in, T *out, int nt, int d,
+                        int n0, int n1, int m, const int *cnt, const int *col, const T *w, int K,
+                        int add)
+    {
+      if (K <= 4)
+        k_sweep<T, I, 4><<<grid, dim3(256), 0, s>>>(total, in, out, nt, d, n0, n1, m, cnt, col, w,
+                                                     K, add);
+      else if (K <= 8)
+        k_sweep<T, I, 8><<<grid, dim3(256), 0, s>>>(total, in, out, nt, d, n0, n1, m, cnt, col, w,
+                                                     K, add);
+      else
+        k_sweep<T, I, 0><<<grid, dim3(256), 0, s>>>(total, in, out, nt, d, n0, n1, m, cnt, col, w,
+                                                     K, add);
     }
 
     // 32-bit indices are used when every index, and every thread index of the
@@ -99,12 +118,11 @@ namespace stfem
     const long total_in = (long)n_in[0] * n_in[1] * n_in[2] * nt;
     const dim3 grid((unsigned)((total + 255) / 256));
     if (std::max(total, total_in) <= k_max_u32)
-      k_sweep<T, unsigned><<<grid, dim3(256), 0, s>>>(unsigned(total), in, out, nt, d, n_in[0],
-                                                      n_in[1], m_out, cnt, col, w, K, add ? 1 : 0);
+      launch_sweep_k<T, unsigned>(grid, s, unsigned(total), in, out, nt, d, n_in[0], n_in[1],
+                                  m_out, cnt, col, w, K, add ? 1 : 0);
     else
-      k_sweep<T, unsigned long><<<grid, dim3(256), 0, s>>>(total, in, out, nt, d, n_in[0],
-                                                           n_in[1], m_out, cnt, col, w, K,
-                                                           add ? 1 : 0);
+      launch_sweep_k<T, unsigned long>(grid, s, total, in, out, nt, d, n_in[0], n_in[1], m_out,
+                                       cnt, col, w, K, add ? 1 : 0);
     STFEM_HIP(hipGetLastError());
   }
 
