@@ -1,5 +1,5 @@
-// Device code of the fused matrix-free space-time operator (gfx950),
-// shared by the one-launch-per-level kernel and the coarse-level kernel.
+// Device code of the fused matrix-free space-time operator (gfx950), used by
+// the kernels of k_operator.hip.
 //
 //   dst_j = sum_i Alpha(j,i) K src_i + Beta(j,i) M src_i
 //

@@ -1,5 +1,5 @@
-// Device code of the multigrid transfers (gfx950), shared by the
-// one-launch-per-transfer kernels and the coarse-level kernel.
+// Device code of the multigrid transfers (gfx950), used by the kernels of
+// k_transfer.hip.
 //
 // Time (tau / k) transfers: MGTwoLevelTransferST::prolongate_and_add /
 // restrict_and_add (tests/include/gmg.h:121-135 via tensorproduct_add,

@@ -1,5 +1,5 @@
-// Device code of the cell-patch Vanka smoother (gfx950), shared by the
-// one-launch-per-level kernels and the coarse-level kernel.
+// Device code of the cell-patch Vanka smoother (gfx950), used by the
+// kernels of k_vanka.hip.
 //
 // PreconditionVanka::vmult (tests/include/gmg.h:476-514 of
 // nlsmrg/dealii-stfem), fused with the relaxation update of
