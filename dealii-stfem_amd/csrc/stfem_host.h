@@ -335,9 +335,10 @@ namespace stfem
     // relaxation sweeps on levels of at most this many cells take A u at
     // the patch nodes from the operator's cell buffer (one node gather
     // launch fewer per sweep, at the price of summing the node values in
-    // the sweep kernel) where that kernel supports it; 0 (default)
-    // disables: on the benchmark levels the two effects cancel
-    long   fuse_residual           = 0;
+    // the sweep kernel) where that kernel supports it; 0 disables.  The
+    // default 512 covers levels of up to 8^3 cells, where the saved launch
+    // latency outweighs the extra loads (on larger levels it does not)
+    long   fuse_residual           = 512;
   };
 
   // Level data of the hierarchy (precision T).

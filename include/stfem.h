@@ -96,7 +96,7 @@ int stfem_time_mg_sequence(int n_sp_lvl, int k_max, int k_min, int nts, int nts_
  * vanka_kernel (0 automatic, 2 LDS-staged, 3 register-tiled, 4 pipelined),
  * fuse_residual (relaxation sweeps on levels of at most this many cells sum
  * A u at the patch nodes from the operator's cell buffer instead of a node
- * gather; 0, the default, disables), orthogonalization (1, default: the
+ * gather; default 512, 0 disables), orthogonalization (1, default: the
  * modified Gram-Schmidt coefficients of
  * an FGMRES step from one pass of dots against the basis, corrected with
  * the basis' Gram matrix; 0: the chain of axpy+dot passes),
