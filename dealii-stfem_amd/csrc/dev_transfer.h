@@ -62,6 +62,7 @@ namespace stfem
       T         s = T(0);
       if constexpr (KM == 0)
         {
+#pragma unroll 4
           for (int k = 0; k < c; ++k)
             s += w[od * K + k] * in[(ib + I(col[od * K + k]) * stride) * I(nt) + I(comp)];
         }
