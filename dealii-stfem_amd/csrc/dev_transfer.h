@@ -29,6 +29,7 @@ namespace stfem
       const I   node = idx / I(ntd);
       const int a    = int(idx - node * I(ntd));
       T         s    = T(0);
+#pragma unroll 4
       for (int b = 0; b < nts; ++b)
         s += M[a * nts + b] * src[node * I(nts) + I(b)];
       dst[idx] = add ? dst[idx] + s : s;
@@ -41,7 +42,9 @@ namespace stfem
     // of all K taps are issued together, then the input loads of the row's
     // cnt taps, so that a thread waits for two memory round trips instead of
     // one per tap (padded taps: column 0, weight 0, never summed); the taps
-    // are summed in order as with KM = 0.
+    // are summed in order as with KM = 0.  The KM = 0 loop is unrolled by
+    // four for the same reason: the loads of four taps are then in flight
+    // together.
     template <typename T, typename I, int KM>
     __device__ __forceinline__ void sweep_item(I idx, const T *__restrict__ in,
                                                T *__restrict__ out, int nt, int d, int n0, int n1,
