@@ -184,22 +184,39 @@ namespace stfem
         out[blockIdx.x] = s;
     }
 
-    // Coefficients of step j from dots = [(v_0,x) .. (v_j,x), (v_0,v_j) ..
-    // (v_{j-1},v_j)]: column j of the Gram matrix G (ld ldg) is stored, then
-    // h_i = (v_i, x) - sum_{k<i} G(k,i) h_k, which equals the coefficient
-    // modified Gram-Schmidt computes from x - sum_{k<i} h_k v_k.
-    __global__ void k_gs_coefficients(const double *dots, int j, double *G, int ldg, double *h)
+    // Coefficients of step j (j < MG) from dots = [(v_0,x) .. (v_j,x),
+    // (v_0,v_j) .. (v_{j-1},v_j)]: column j of the Gram matrix G (ld ldg) is
+    // stored, then h_i = (v_i, x) - sum_{k<i} G(k,i) h_k, which equals the
+    // coefficient modified Gram-Schmidt computes from x - sum_{k<i} h_k v_k.
+    // One workgroup of MG threads stages the upper triangle of columns 0..j
+    // in LDS; one thread then runs the recurrence from LDS.
+    constexpr int MG = 64;
+    __global__ void __launch_bounds__(MG)
+      k_gs_coefficients(const double *dots, int j, double *G, int ldg, double *h)
     {
-      if (threadIdx.x != 0)
-        return;
-      for (int i = 0; i < j; ++i)
+      __shared__ double Gs[MG][MG]; // Gs[i][k] = G(k, i), k < i <= j
+      __shared__ double hs[MG];
+      const int         t = threadIdx.x, n = j + 1;
+      for (int i = t; i < j; i += MG)
         G[i + (long)j * ldg] = dots[j + 1 + i];
+      for (int e = t; e < n * n; e += MG)
+        {
+          const int i = e / n, k = e - i * n;
+          if (k < i)
+            Gs[i][k] = (i == j) ? dots[j + 1 + k] : G[k + (long)i * ldg];
+        }
+      if (t < n)
+        hs[t] = dots[t];
+      __syncthreads();
+      if (t != 0)
+        return;
       for (int i = 0; i <= j; ++i)
         {
-          double s = dots[i];
+          double s = hs[i];
           for (int k = 0; k < i; ++k)
-            s -= G[k + (long)i * ldg] * h[k];
-          h[i] = s;
+            s -= Gs[i][k] * hs[k];
+          hs[i] = s;
+          h[i]  = s;
         }
     }
 
@@ -428,7 +445,7 @@ namespace stfem
   void launch_gs_step(long n, int j, const double *const *v, double *x, double *G, int ldg,
                       double *partial, double *dots, double *h, double *pnorm, hipStream_t s)
   {
-    STFEM_REQUIRE(j >= 0 && j < MA && j < ldg, "Gram-Schmidt step: basis too large");
+    STFEM_REQUIRE(j >= 0 && j < MA && j < MG && j < ldg, "Gram-Schmidt step: basis too large");
     const double *vj = v[j];
     // (v_i, x) -> d = i, (v_i, v_j) -> d = j + 1 + i for i < j; (v_j, x) -> d = j
     for (int i0 = 0; i0 < j || i0 == 0; i0 += MD)
@@ -445,7 +462,7 @@ namespace stfem
           break;
       }
     k_finish_many<<<2 * j + 1, RT, 0, s>>>(partial, dots);
-    k_gs_coefficients<<<1, 64, 0, s>>>(dots, j, G, ldg, h);
+    k_gs_coefficients<<<1, MG, 0, s>>>(dots, j, G, ldg, h);
     AxpyList A;
     A.m = j + 1;
     for (int i = 0; i <= j; ++i)
