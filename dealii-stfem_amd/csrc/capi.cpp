@@ -1,6 +1,8 @@
 // extern "C" boundary of libstfem_amd.so (declarations in include/stfem.h).
 #include "../../include/stfem.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <random>
 #include <string>
@@ -63,6 +65,35 @@ namespace
     STFEM_REQUIRE((allow_fine && level == -1) || (level >= 0 && level < pb.n_levels()),
                   "level out of range");
   }
+  // two HIP events, destroyed on every exit path
+  struct EventPair
+  {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    EventPair()
+    {
+      STFEM_HIP(hipEventCreate(&e0));
+      if (hipEventCreate(&e1) != hipSuccess)
+        {
+          (void)hipEventDestroy(e0);
+          throw Error(STATUS_ERR_HIP, "hipEventCreate failed");
+        }
+    }
+    ~EventPair()
+    {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+    EventPair(const EventPair &)            = delete;
+    EventPair &operator=(const EventPair &) = delete;
+    // milliseconds between the two records (waits for the second)
+    double elapsed() const
+    {
+      STFEM_HIP(hipEventSynchronize(e1));
+      float t = 0;
+      STFEM_HIP(hipEventElapsedTime(&t, e0, e1));
+      return t;
+    }
+  };
   // inputs may come from other streams (e.g. a framework allocator)
   void device_barrier(ProblemBase &pb)
   {
@@ -176,6 +207,15 @@ extern "C"
     });
   }
 
+  int stfem_spec_normalize(const char *spec, char *out, int cap)
+  {
+    return guarded([&] {
+      const std::string n = ProblemSpec::parse(spec ? spec : "").to_string();
+      STFEM_REQUIRE(out == nullptr || cap > int(n.size()), "output buffer too small");
+      if (out)
+        std::memcpy(out, n.c_str(), n.size() + 1);
+    });
+  }
   int stfem_problem_create(const char *spec, const double *vertices, const double *cell_coef,
                            int device, stfem_problem *out)
   {
@@ -509,25 +549,30 @@ extern "C"
             default: throw Error(STATUS_ERR_ARG, "unknown kernel selector");
           }
       };
+      // transfers and the V-cycle accumulate into b: start from zero
+      STFEM_HIP(hipMemset(b.get(), 0, n * es));
       for (int i = 0; i < warmup; ++i)
         run();
       ctx.sync();
-      hipEvent_t e0, e1;
-      STFEM_HIP(hipEventCreate(&e0));
-      STFEM_HIP(hipEventCreate(&e1));
-      STFEM_HIP(hipEventRecord(e0, ctx.stream));
+      EventPair ev;
+      STFEM_HIP(hipEventRecord(ev.e0, ctx.stream));
       for (int i = 0; i < reps; ++i)
         run();
-      STFEM_HIP(hipEventRecord(e1, ctx.stream));
-      STFEM_HIP(hipEventSynchronize(e1));
-      float t = 0;
-      STFEM_HIP(hipEventElapsedTime(&t, e0, e1));
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-      *ms = double(t) / reps;
+      STFEM_HIP(hipEventRecord(ev.e1, ctx.stream));
+      *ms = ev.elapsed() / reps;
     });
   }
-
+  int stfem_phase_timings(stfem_problem p, int reset, double *ms, long *calls)
+  {
+    return guarded([&] {
+      auto &pb = P(p);
+      STFEM_REQUIRE(pb.context().timers, "phase timers are off (spec key print_timing=1)");
+      STFEM_HIP(hipSetDevice(pb.context().device));
+      pb.context().timers->read(ms, calls);
+      if (reset)
+        pb.context().timers->reset();
+    });
+  }
   int stfem_time_slabs(stfem_problem p, int n_slabs, double *ms, int *its)
   {
     int rc = STFEM_OK;
@@ -538,32 +583,31 @@ extern "C"
       STFEM_HIP(hipSetDevice(ctx.device));
       const long N  = pb.n_nodes();
       const int  nt = pb.nt;
+      // slabs of one run from t = 0 to end_time; the sequence restarts from
+      // a zero state after that many slabs
+      const double slab_len = pb.spec.nts * pb.tau;
+      const int    period   = std::max(1, int(std::lround(pb.spec.end_time / slab_len)));
       DBuf<double> x{size_t(N) * nt}, prev{size_t(N)}, sel{size_t(nt)};
-      x.zero(ctx.stream);
       std::vector<double> s(nt, 0.0);
       s[nt - 1] = 1.0;
       sel.upload(s.data(), s.size(), ctx.stream);
       ctx.sync();
-      hipEvent_t e0, e1;
-      STFEM_HIP(hipEventCreate(&e0));
-      STFEM_HIP(hipEventCreate(&e1));
-      double total = 0;
-      int    itsum = 0;
-      double time  = 0;
+      EventPair ev;
+      double    total = 0;
+      int       itsum = 0;
       for (int k = 0; k < n_slabs; ++k)
         {
+          if (k % period == 0)
+            x.zero(ctx.stream);
+          const double time = (k % period) * slab_len;
           // prev = last time value of the previous slab (tp_01.cc:637)
           launch_time_transfer<double>(N, sel.get(), 1, nt, x.get(), prev.get(), false, ctx.stream);
-          STFEM_HIP(hipEventRecord(e0, ctx.stream));
+          STFEM_HIP(hipEventRecord(ev.e0, ctx.stream));
           int       it = 0;
           const int r  = pb.solve_slab(x.get(), prev.get(), time, &it);
-          STFEM_HIP(hipEventRecord(e1, ctx.stream));
-          STFEM_HIP(hipEventSynchronize(e1));
-          float t = 0;
-          STFEM_HIP(hipEventElapsedTime(&t, e0, e1));
-          total += t;
+          STFEM_HIP(hipEventRecord(ev.e1, ctx.stream));
+          total += ev.elapsed();
           itsum += it;
-          time += pb.spec.nts * pb.tau;
           // a slab that hits the iteration limit still counts (its time and
           // iterations); the status is reported at the end
           if (r != STFEM_OK && rc == STFEM_OK)
@@ -571,8 +615,6 @@ extern "C"
           if (r != STFEM_OK && r != STFEM_ERR_NO_CONVERGENCE)
             break;
         }
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
       *ms  = total;
       *its = itsum;
     });

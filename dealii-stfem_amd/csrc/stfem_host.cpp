@@ -46,6 +46,106 @@ namespace stfem
     };
   } // namespace
 
+  // ========================================================== PhaseTimers
+  PhaseTimers::~PhaseTimers()
+  {
+    for (auto &p : pending_)
+      {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+      }
+    for (auto e : pool_)
+      (void)hipEventDestroy(e);
+  }
+  hipEvent_t PhaseTimers::get()
+  {
+    if (pool_.empty())
+      {
+        hipEvent_t e;
+        STFEM_HIP(hipEventCreate(&e));
+        return e;
+      }
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t PhaseTimers::start(hipStream_t s)
+  {
+    hipEvent_t a = get();
+    STFEM_HIP(hipEventRecord(a, s));
+    return a;
+  }
+  void PhaseTimers::stop(int phase, hipEvent_t a, hipStream_t s)
+  {
+    // called from destructors: failures drop the sample instead of throwing
+    hipEvent_t b = nullptr;
+    try
+      {
+        b = get();
+      }
+    catch (...)
+      {
+        pool_.push_back(a);
+        return;
+      }
+    if (hipEventRecord(b, s) != hipSuccess)
+      {
+        pool_.push_back(a);
+        pool_.push_back(b);
+        return;
+      }
+    pending_.push_back({phase, a, b});
+    if (pending_.size() >= 4096)
+      {
+        try
+          {
+            resolve();
+          }
+        catch (...)
+          {
+          }
+      }
+  }
+  void PhaseTimers::resolve()
+  {
+    for (auto &p : pending_)
+      {
+        STFEM_HIP(hipEventSynchronize(p.b));
+        float t = 0;
+        STFEM_HIP(hipEventElapsedTime(&t, p.a, p.b));
+        ms_[p.phase] += t;
+        calls_[p.phase] += 1;
+        pool_.push_back(p.a);
+        pool_.push_back(p.b);
+      }
+    pending_.clear();
+  }
+  void PhaseTimers::read(double *ms, long *calls)
+  {
+    resolve();
+    for (int i = 0; i < n_phases; ++i)
+      {
+        if (ms)
+          ms[i] = ms_[i];
+        if (calls)
+          calls[i] = calls_[i];
+      }
+  }
+  void PhaseTimers::reset()
+  {
+    resolve();
+    for (int i = 0; i < n_phases; ++i)
+      {
+        ms_[i]    = 0;
+        calls_[i] = 0;
+      }
+  }
+  const char *PhaseTimers::name(int phase)
+  {
+    static const char *n[n_phases] = {"vmult", "vanka", "gmg", "step"};
+    return (phase >= 0 && phase < n_phases) ? n[phase] : "?";
+  }
+
   // =============================================================== Context
   Context::Context(int dev)
     : device(dev)
@@ -70,6 +170,25 @@ namespace stfem
   }
   Context::~Context()
   {
+    if (timers)
+      {
+        // the reference prints its TimerOutput summary at the end
+        // (printTiming); here when the problem is destroyed
+        double ms[PhaseTimers::n_phases];
+        long   calls[PhaseTimers::n_phases];
+        try
+          {
+            timers->read(ms, calls);
+            std::fprintf(stderr, "[stfem] phase timers (device time):\n");
+            for (int i = 0; i < PhaseTimers::n_phases; ++i)
+              std::fprintf(stderr, "[stfem]   %-6s %10ld calls %12.3f ms\n",
+                           PhaseTimers::name(i), calls[i], ms[i]);
+          }
+        catch (...)
+          {
+          }
+        timers.reset();
+      }
     if (stream)
       (void)hipStreamDestroy(stream);
   }
@@ -481,6 +600,7 @@ namespace stfem
   void SystemMatrix<T>::apply(T *dst, const T *src, const DBuf<T> &a, const DBuf<T> &b, int mode,
                               const T *rhs) const
   {
+    const PhaseScope scope(ctx_, PhaseTimers::vmult);
     StOpArgs<T> args;
     args.g          = grid_;
     args.vertices   = verts_.get();
@@ -1061,14 +1181,18 @@ namespace stfem
   void PreconditionVanka<T>::vmult(T *dst, const T *src) const
   {
     if (form_ == Form::factored)
-      return launch_vanka_factored<T>(grid_, nt_, S_.get(), lam_.get(), ta_.get(), tb_.get(), src,
+      {
+        const PhaseScope scope(ctx_, PhaseTimers::vanka);
+        return launch_vanka_factored<T>(grid_, nt_, S_.get(), lam_.get(), ta_.get(), tb_.get(), src,
                                       nullptr, dst, T(1), cells_.get(), 0, variant_, ctx_.stream);
+      }
     launch_fill_zero<T>(dst, size_t(grid_.n_nodes) * nt_, ctx_.stream);
     relax(dst, src, nullptr, T(1));
   }
   template <typename T>
   void PreconditionVanka<T>::relax(T *u, const T *rhs, const T *ax, T omega, bool assign) const
   {
+    const PhaseScope scope(ctx_, PhaseTimers::vanka);
     if (form_ == Form::explicit_inverse)
       {
         if (assign)
@@ -1093,6 +1217,7 @@ namespace stfem
   template <typename T>
   void PreconditionVanka<T>::relax_opc(T *u, const T *rhs, const T *opc, T omega) const
   {
+    const PhaseScope scope(ctx_, PhaseTimers::vanka);
     launch_vanka_factored<T>(grid_, nt_, S_.get(), lam_.get(), ta_.get(), tb_.get(), rhs, nullptr,
                              u, omega, cells_.get(), 1, variant_, ctx_.stream, opc);
   }
@@ -1110,14 +1235,19 @@ namespace stfem
   // ===================================================== time transfer
   template <typename T>
   MGTwoLevelTransferST<T>::MGTwoLevelTransferST(Context &ctx, TimeStepType type, int r, int nts,
-                                                TimeMGType kind, long n_nodes, bool overwrite)
+                                                TimeMGType kind, long n_nodes, bool overwrite,
+                                                bool restrict_is_transpose)
     : ctx_(ctx)
     , n_nodes_(n_nodes)
     , overwrite_(overwrite)
   {
     P_ = (kind == TimeMGType::k) ? get_time_projection_matrix(type, r - 1, r, nts) :
                                    get_time_prolongation_matrix(type, r, nts);
-    const DMat     R = P_.transposed();
+    // gmg.h:101-113: R = P^T, or the L2 restriction / projection onto r - 1
+    const DMat R = restrict_is_transpose ? P_.transposed() :
+                   (kind == TimeMGType::k) ? get_time_projection_matrix(type, r, r - 1, nts) :
+                                             get_time_restriction_matrix(type, r, nts);
+    STFEM_REQUIRE(R.m == P_.n && R.n == P_.m, "time transfer: restriction has the wrong shape");
     std::vector<T> p(P_.a.begin(), P_.a.end()), rr(R.a.begin(), R.a.end());
     dP_.upload(p.data(), p.size(), ctx.stream);
     dR_.upload(rr.data(), rr.size(), ctx.stream);
@@ -1269,6 +1399,23 @@ namespace stfem
     for (int l = 0; l < nl; ++l)
       lv_[l].omega = opt_.estimate_relaxation ? estimate_omega(l) : 1.0;
     setup_dense_coarse();
+    // timed phases record events around every level call: no graph then
+    use_graph_ = opt_.use_graph == 1 && !ctx_.timers;
+    if (opt_.use_graph == -1 && !ctx_.timers)
+      {
+        // rocprofv3 loads its tool library through ROCP_TOOL_LIBRARIES; with
+        // a tool present the V-cycle is launched eagerly (same kernels)
+        const char *tool = std::getenv("ROCP_TOOL_LIBRARIES");
+        use_graph_       = !(tool && *tool);
+        if (!use_graph_)
+          {
+            static std::once_flag once;
+            std::call_once(once, [] {
+              std::fprintf(stderr, "[stfem] profiler tool loaded: the V-cycle is launched "
+                                   "kernel by kernel instead of as a HIP graph\n");
+            });
+          }
+      }
   }
 
   namespace
@@ -1610,6 +1757,7 @@ namespace stfem
   template <typename T>
   void GMG<T>::vmult(double *dst, const double *src) const
   {
+    const PhaseScope scope(ctx_, PhaseTimers::gmg);
     const int  L  = int(lv_.size()) - 1;
     const long sz = lv_[L].A->m();
     if constexpr (std::is_same_v<T, double>)
@@ -1617,7 +1765,7 @@ namespace stfem
                                ctx_.stream));
     else
       launch_convert<T, double>(sz, src, def_[L].get(), ctx_.stream);
-    if (opt_.use_graph)
+    if (use_graph_)
       {
         // the V-cycle launches a fixed kernel sequence on fixed buffers:
         // record it once, replay it afterwards
@@ -1804,127 +1952,6 @@ namespace stfem
     return state == 1 ? STATUS_OK : STATUS_ERR_NO_CONVERGE;
   }
 
-  // ============================================================ ProblemSpec
-  ProblemSpec ProblemSpec::parse(const std::string &s)
-  {
-    ProblemSpec                        ps;
-    std::map<std::string, std::string> kv;
-    std::stringstream                  ss(s);
-    std::string                        tok;
-    while (std::getline(ss, tok, ';'))
-      {
-        const auto e = tok.find('=');
-        if (e == std::string::npos)
-          continue;
-        kv[tok.substr(0, e)] = tok.substr(e + 1);
-      }
-    auto geti = [&](const char *k, int &v) {
-      if (kv.count(k))
-        v = std::stoi(kv[k]);
-    };
-    auto getd = [&](const char *k, double &v) {
-      if (kv.count(k))
-        v = std::stod(kv[k]);
-    };
-    auto getb = [&](const char *k, bool &v) {
-      if (kv.count(k))
-        v = std::stoi(kv[k]) != 0;
-    };
-    geti("dim", ps.dim);
-    if (kv.count("subdiv"))
-      {
-        std::stringstream s2(kv["subdiv"]);
-        for (int d = 0; d < 3 && std::getline(s2, tok, ','); ++d)
-          ps.subdiv[d] = std::stoi(tok);
-      }
-    if (kv.count("lo"))
-      {
-        std::stringstream s2(kv["lo"]);
-        for (int d = 0; d < 3 && std::getline(s2, tok, ','); ++d)
-          ps.lo[d] = std::stod(tok);
-      }
-    if (kv.count("hi"))
-      {
-        std::stringstream s2(kv["hi"]);
-        for (int d = 0; d < 3 && std::getline(s2, tok, ','); ++d)
-          ps.hi[d] = std::stod(tok);
-      }
-    geti("refine", ps.refine);
-    geti("p", ps.p);
-    if (kv.count("type"))
-      ps.type = (kv["type"] == "CGP" || kv["type"] == "1") ? TimeStepType::CGP : TimeStepType::DG;
-    geti("r", ps.r);
-    geti("nts", ps.nts);
-    getd("tau", ps.tau);
-    geti("space_time_mg", ps.space_time_mg);
-    geti("nts_min", ps.nts_min);
-    geti("r_min", ps.r_min);
-    geti("time_before_space", ps.time_before_space);
-    geti("mg_space", ps.mg_space);
-    geti("mg_float", ps.mg_float);
-    geti("dirichlet", ps.dirichlet);
-    geti("atomic_scatter", ps.atomic_scatter);
-    geti("coarse_coef", ps.coarse_coef);
-    if (kv.count("problem"))
-      ps.problem = (kv["problem"] == "wave" || kv["problem"] == "1") ? 1 : 0;
-    geti("vanka_device_setup", ps.vanka_device_setup);
-    geti("vanka_kernel", ps.vanka_kernel);
-    geti("op_jb", ps.op_jb);
-    geti("op_jb64", ps.op_jb64);
-    geti("op_split", ps.op_split);
-    geti("smoothing_steps", ps.mg.smoothing_steps);
-    getb("variable", ps.mg.variable);
-    getb("estimate_relaxation", ps.mg.estimate_relaxation);
-    geti("eig_iters", ps.mg.eig_iterations);
-    getd("eig_safety", ps.mg.eig_safety);
-    getb("overwrite_transfer", ps.mg.overwrite_time_transfer);
-    getb("vanka_factored", ps.mg.vanka_factored);
-    getb("exact_single_cell", ps.mg.exact_single_cell);
-    getb("use_graph", ps.mg.use_graph);
-    if (kv.count("fuse_residual"))
-      ps.mg.fuse_residual = std::stol(kv["fuse_residual"]);
-    if (kv.count("dense_coarse_dofs"))
-      ps.mg.dense_coarse_dofs = std::stol(kv["dense_coarse_dofs"]);
-    geti("maxiter", ps.solver.maxiter);
-    getd("abstol", ps.solver.abstol);
-    getd("reltol", ps.solver.reltol);
-    geti("basis", ps.solver.basis);
-    geti("orthogonalization", ps.solver.orthogonalization);
-    geti("extrapolate", ps.extrapolate);
-    getd("end_time", ps.end_time);
-    getd("frequency", ps.frequency);
-    geti("source", ps.source);
-    geti("n_bumps", ps.n_bumps);
-    if (kv.count("seed"))
-      ps.seed = std::stol(kv["seed"]);
-    // Parameters::parse derivations (gmg.h:637-648)
-    if (ps.nts_min == -1)
-      ps.nts_min = ps.nts / 2;
-    ps.nts_min = std::clamp(ps.nts_min, 1, std::max(1, ps.nts));
-    const int lowest = ps.type == TimeStepType::DG ? 0 : 1;
-    if (ps.r_min == -1)
-      ps.r_min = ps.r - 1;
-    ps.r_min = std::clamp(ps.r_min, lowest, std::max(lowest, ps.r));
-    if (ps.p < 0)
-      ps.p = ps.r + 1;
-    STFEM_REQUIRE(ps.dim == 2 || ps.dim == 3, "spec: dim must be 2 or 3");
-    STFEM_REQUIRE(ps.p >= 1 && ps.p < MAX_N, "spec: space degree must be in [1, 5]");
-    STFEM_REQUIRE(ps.r >= (ps.type == TimeStepType::DG ? 0 : 1), "spec: bad time degree");
-    STFEM_REQUIRE(ps.nts >= 1 && (ps.nts & (ps.nts - 1)) == 0,
-                  "spec: n_timesteps_at_once must be a power of two");
-    STFEM_REQUIRE(ps.vanka_device_setup >= 0 && ps.vanka_device_setup <= 5,
-                  "spec: vanka_device_setup must be in [0, 5]");
-    STFEM_REQUIRE(ps.solver.orthogonalization == 0 || ps.solver.orthogonalization == 1,
-                  "spec: orthogonalization must be 0 or 1");
-    STFEM_REQUIRE(ps.op_jb >= 1 && ps.op_jb <= 4 && ps.op_jb64 >= 0 && ps.op_jb64 <= 4,
-                  "spec: op_jb must be in [1, 4], op_jb64 in [0, 4]");
-    STFEM_REQUIRE(ps.vanka_kernel == 0 || (ps.vanka_kernel >= 2 && ps.vanka_kernel <= 4),
-                  "spec: vanka_kernel must be 0, 2, 3 or 4");
-    STFEM_REQUIRE(ps.coarse_coef >= 0 && ps.coarse_coef <= 3, "spec: coarse_coef must be in [0, 3]");
-    STFEM_REQUIRE(ps.solver.basis >= 2 && ps.solver.basis <= 60, "spec: basis must be in [2, 60]");
-    return ps;
-  }
-
   // ================================================================ Problem
   namespace
   {
@@ -1937,6 +1964,8 @@ namespace stfem
         , sh_(s.p)
       {
         spec = s;
+        if (spec.print_timing)
+          ctx_.timers = std::make_unique<PhaseTimers>();
         PhaseLog plog;
         mesh_ = build_mesh_levels(spec.dim, spec.subdiv, spec.refine, spec.p, vertices, spec.lo,
                                   spec.hi, kappa, spec.dirichlet, spec.coarse_coef);
@@ -2016,29 +2045,10 @@ namespace stfem
           });
         else if (spec.source == 3)
           {
-            // seeded Gaussian bumps: centres uniform in [0.2, 0.8]^dim,
-            // amplitudes uniform in [0.5, 1.5], width 0.1
-            std::mt19937_64                  rng(uint64_t(spec.seed));
-            std::uniform_real_distribution<> Uc(0.2, 0.8), Ua(0.5, 1.5);
-            std::vector<std::array<double, 4>> bumps(std::max(1, spec.n_bumps));
-            for (auto &b : bumps)
-              {
-                for (int d = 0; d < 3; ++d)
-                  b[d] = Uc(rng);
-                b[3] = Ua(rng);
-              }
-            const double s2 = 2.0 * 0.1 * 0.1;
-            build_load([&](const double *x) {
-              double v = 0.0;
-              for (const auto &b : bumps)
-                {
-                  double r2 = 0.0;
-                  for (int d = 0; d < spec.dim; ++d)
-                    r2 += (x[d] - b[d]) * (x[d] - b[d]);
-                  v += b[3] * std::exp(-r2 / s2);
-                }
-              return v;
-            });
+            // seeded Gaussian bumps (spec.h: gaussian_bumps)
+            const std::vector<double> bumps = gaussian_bumps(spec.n_bumps, spec.seed);
+            build_load(
+              [&](const double *x) { return gaussian_bumps_value(bumps, spec.dim, x); });
           }
         ctx_.sync();
       }
@@ -2116,7 +2126,7 @@ namespace stfem
             else
               tr[l].time = std::make_unique<MGTwoLevelTransferST<T>>(
                 ctx_, spec.type, rl[l], nl[l], t, mesh_[sref[l]].g.n_nodes,
-                spec.mg.overwrite_time_transfer);
+                spec.mg.overwrite_time_transfer, spec.restrict_is_transpose != 0);
           }
         plog("transfers");
         return std::make_unique<GMG<T>>(ctx_, std::move(levels), std::move(tr), spec.mg);
@@ -2180,6 +2190,7 @@ namespace stfem
       // TimeIntegratorHeat::solve (time_integrators.h:157-180)
       int solve_slab(double *x, const double *prev, double time, int *its) override
       {
+        const PhaseScope scope(ctx_, PhaseTimers::step);
         STFEM_REQUIRE(spec.problem == 0, "solve_slab: the wave problem needs solve_slab_wave");
         return solve_displacement(x, prev, nullptr, time, its);
       }
@@ -2190,6 +2201,7 @@ namespace stfem
       int solve_slab_wave(double *x, double *v, const double *prev_u, const double *prev_v,
                           double time, int *its) override
       {
+        const PhaseScope scope(ctx_, PhaseTimers::step);
         STFEM_REQUIRE(spec.problem == 1, "solve_slab_wave: the problem is not the wave problem");
         const int rc = solve_displacement(x, prev_u, prev_v, time, its);
         launch_wave_velocity(fine_->n_nodes(), spec.nts, nt / spec.nts,
@@ -2229,8 +2241,12 @@ namespace stfem
             launch_time_transfer<double>(N, coefs_.get(), nt, 1, load_.get(), rh, true,
                                          ctx_.stream);
           }
-        // extrapolate: every block starts from prev
-        launch_time_transfer<double>(N, ones_.get(), nt, 1, prev, x, false, ctx_.stream);
+        // start vector (time_integrators.h:96-104): with extrapolation every
+        // block starts from prev, otherwise from zero
+        if (spec.extrapolate)
+          launch_time_transfer<double>(N, ones_.get(), nt, 1, prev, x, false, ctx_.stream);
+        else
+          STFEM_HIP(hipMemsetAsync(x, 0, size_t(N) * nt * sizeof(double), ctx_.stream));
         const int rc = solver_->solve(*fine_, *gmg_, x, rh);
         launch_zero_dirichlet<double>(fine_->grid(), nt, x, ctx_.stream);
         if (its)

@@ -19,6 +19,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "spec.h"
 #include "time_fe.h"
 
 namespace stfem
@@ -87,6 +88,47 @@ namespace stfem
     size_t n_ = 0;
   };
 
+  // ---------------------------------------------------------- phase timers
+  // Device time per phase, the TimerOutput scopes of the reference: "vmult"
+  // (SystemMatrix::vmult, operators.h:97; every operator application of a
+  // level or of the fine system), "vanka" (PreconditionVanka::vmult,
+  // gmg.h:479; here every relaxation sweep, which applies the smoother),
+  // "gmg" (GMG::vmult, gmg.h:850) and "step" (one slab solve, tp_01.cc:632).
+  // Scopes may nest.  Measured with HIP events on the library stream and
+  // summed when read.
+  class PhaseTimers
+  {
+  public:
+    enum Phase
+    {
+      vmult    = 0,
+      vanka    = 1,
+      gmg      = 2,
+      step     = 3,
+      n_phases = 4
+    };
+    ~PhaseTimers();
+    hipEvent_t start(hipStream_t s);
+    void       stop(int phase, hipEvent_t a, hipStream_t s);
+    // totals (ms) and call counts so far
+    void       read(double *ms, long *calls);
+    void       reset();
+    static const char *name(int phase);
+
+  private:
+    struct Pending
+    {
+      int        phase;
+      hipEvent_t a, b;
+    };
+    hipEvent_t           get();
+    void                 resolve();
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    double               ms_[n_phases]    = {};
+    long                 calls_[n_phases] = {};
+  };
+
   // --------------------------------------------------------------- context
   struct Context
   {
@@ -94,9 +136,38 @@ namespace stfem
     hipStream_t      stream = nullptr;
     DBuf<double>     partial;  // reduction scratch
     DBuf<double>     scalars;  // device scalars (Hessenberg column, norms)
+    std::unique_ptr<PhaseTimers> timers; // set: phases are timed (print_timing)
     explicit Context(int dev);
     ~Context();
     void sync();
+  };
+
+  // times the enclosing block as one call of `phase` when the context's
+  // timers are on
+  class PhaseScope
+  {
+  public:
+    PhaseScope(const Context &ctx, int phase)
+      : t_(ctx.timers.get())
+      , s_(ctx.stream)
+      , phase_(phase)
+    {
+      if (t_)
+        a_ = t_->start(s_);
+    }
+    ~PhaseScope()
+    {
+      if (t_)
+        t_->stop(phase_, a_, s_);
+    }
+    PhaseScope(const PhaseScope &)            = delete;
+    PhaseScope &operator=(const PhaseScope &) = delete;
+
+  private:
+    PhaseTimers *t_;
+    hipStream_t  s_;
+    int          phase_;
+    hipEvent_t   a_ = nullptr;
   };
 
   // ------------------------------------------------------------ geometry
@@ -249,10 +320,11 @@ namespace stfem
   class MGTwoLevelTransferST
   {
   public:
-    // gmg.h:86-115 with restrict_is_transpose_prolongate; `overwrite`
+    // gmg.h:86-115 (restrict_is_transpose: R = P^T, else the L2
+    // restriction / projection matrices); `overwrite`
     // reproduces the assigning transfer of tests/transfer_01.cc:42-83.
     MGTwoLevelTransferST(Context &ctx, TimeStepType type, int r, int nts, TimeMGType kind,
-                         long n_nodes, bool overwrite = false);
+                         long n_nodes, bool overwrite = false, bool restrict_is_transpose = true);
     void prolongate_and_add(T *dst, const T *src) const;
     void restrict_and_add(T *dst, const T *src) const;
     int  nt_fine() const { return P_.m; }
@@ -311,36 +383,6 @@ namespace stfem
   };
 
   // ------------------------------------------------------------ GMG
-  struct GMGOptions
-  {
-    int    smoothing_steps         = 1;    // PreconditionRelaxation n_iterations
-    bool   variable                = true; // MGSmootherPrecondition steps 2^(L-l)
-    bool   estimate_relaxation     = true;
-    int    eig_iterations          = 20;
-    double eig_safety              = 1.15;
-    bool   overwrite_time_transfer = false;
-    bool   vanka_factored          = false;
-    // On a one-cell level the Vanka patch is the whole level, so the
-    // smoother is the exact inverse and s relaxation sweeps with omega equal
-    // one sweep with 1 - (1 - omega)^s.
-    bool   exact_single_cell       = true;
-    // replay the V-cycle from a captured HIP graph
-    bool   use_graph               = true;
-    // The V-cycle restricted to the coarse levels is a fixed linear map of
-    // the restricted defect.  Below the finest level, the largest level with
-    // at most this many unknowns (and everything under it) is replaced by
-    // that map as one dense matrix, built at setup from the level kernels
-    // and applied as one matrix-vector product per V-cycle; 0 disables.
-    long   dense_coarse_dofs       = 8192;
-    // relaxation sweeps on levels of at most this many cells take A u at
-    // the patch nodes from the operator's cell buffer (one node gather
-    // launch fewer per sweep, at the price of summing the node values in
-    // the sweep kernel) where that kernel supports it; 0 disables.  The
-    // default 512 covers levels of up to 8^3 cells, where the saved launch
-    // latency outweighs the extra loads (on larger levels it does not)
-    long   fuse_residual           = 512;
-  };
-
   // Level data of the hierarchy (precision T).
   template <typename T>
   struct GMGLevel
@@ -393,24 +435,13 @@ namespace stfem
     mutable std::vector<DBuf<T>>     sol_, def_, t1_, t2_;
     mutable hipGraph_t               graph_ = nullptr;
     mutable hipGraphExec_t           exec_  = nullptr;
+    bool                             use_graph_   = false;
     int                              dense_level_ = -1;
     long                             dense_ld_    = 0;
     DBuf<T>                          dense_rows_; // row-major, rows padded to dense_ld_
   };
 
   // ------------------------------------------------------------ FGMRES
-  struct SolverControlData
-  {
-    int    maxiter = 200;
-    double abstol  = 1e-12;
-    double reltol  = 1e-12;
-    int    basis   = 30;
-    // 0: modified Gram-Schmidt as a chain of axpy+dot passes; 1: the same
-    // coefficients from one pass of dots against the whole basis corrected
-    // with the basis' Gram matrix (two passes over the basis per step)
-    int    orthogonalization = 1;
-  };
-
   // Right-preconditioned flexible GMRES (deal.II SolverFGMRES semantics:
   // modified Gram-Schmidt, projected least-squares solve once two basis
   // vectors exist, ReductionControl on the projected residual).
@@ -435,47 +466,6 @@ namespace stfem
   };
 
   // ------------------------------------------------------------ problem
-  // Everything tp_01.cc:56-359 sets up for one (refinement, degree) pair:
-  // mesh hierarchy, FP64 system and rhs matrices, MG levels, transfers, GMG,
-  // FGMRES and the heat time integrator.
-  struct ProblemSpec
-  {
-    int          dim = 2;
-    int          subdiv[3] = {1, 1, 1};
-    double       lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
-    int          refine = 2;
-    int          p = -1;
-    TimeStepType type = TimeStepType::DG;
-    int          r = 1;
-    int          nts = 1;
-    double       tau = -1;
-    int          space_time_mg = 1;
-    int          nts_min = -1, r_min = -1;
-    int          time_before_space = 0;
-    int          mg_space = 1;
-    int          mg_float = 0;
-    int          dirichlet = 63;
-    int          atomic_scatter = 0; // 1: one-pass operator with atomic adds
-    int          problem = 0; // 0 heat, 1 wave (first order in time, tp_01.cc:167-197)
-    int          coarse_coef = 0; // coarse-level coefficient: 0 sampled, 1 geometric, 2 harmonic, 3 arithmetic mean
-    int          vanka_device_setup = 3; // factored Vanka on the GPU: 1 Jacobi, 2 divide and conquer, 3 by patch size
-    int          vanka_kernel = 0; // factored sweep kernel: 0 automatic, 2 LDS-staged, 3 register-tiled
-    // output components per operator cell pass (FP32 levels, FP64 operators;
-    // 0: 2 for FP64 from Q4 on, where four do not fit two waves per SIMD, else 4)
-    int          op_jb = 4, op_jb64 = 0;
-    int          op_split = 128;         // StOpArgs::split_below of every operator
-    GMGOptions   mg;
-    SolverControlData solver;
-    int          extrapolate = 1;
-    double       end_time = 1.0;
-    double       frequency = 1.0;
-    int          source = 0; // 0 manufactured heat source, 1 none, 2 load vector given,
-                             // 3 seeded Gaussian bumps
-    int          n_bumps = 4;
-    long         seed    = 1;
-    static ProblemSpec parse(const std::string &s);
-  };
-
   class ProblemBase
   {
   public:

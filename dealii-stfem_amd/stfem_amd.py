@@ -69,6 +69,8 @@ _SIGS = {
     "stfem_time_kernel": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp],
                           ctypes.c_int),
     "stfem_time_slabs": ([_vp, ctypes.c_int, _dp, _ip], ctypes.c_int),
+    "stfem_spec_normalize": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
+    "stfem_phase_timings": ([_vp, ctypes.c_int, _dp, _lp], ctypes.c_int),
 }
 
 
@@ -189,6 +191,16 @@ def spec_string(spec):
     return ";".join(f"{k}={fmt(v)}" for k, v in spec.items())
 
 
+def spec_normalize(spec):
+    """Parameters::parse without building anything: the spec (a dict, a
+    "key=value;..." string or the text of a JSON parameter file) with every
+    key resolved, as a dict of strings."""
+    text = spec if isinstance(spec, str) else spec_string(spec)
+    buf = ctypes.create_string_buffer(8192)
+    _check(load().stfem_spec_normalize(text.encode(), buf, len(buf)))
+    return dict(kv.split("=", 1) for kv in buf.value.decode().split(";"))
+
+
 class Problem:
     """Hierarchy + solver built from a spec dict (keys listed in include/stfem.h)."""
 
@@ -196,11 +208,12 @@ class Problem:
         self._v = None if vertices is None else np.ascontiguousarray(vertices, np.float64)
         self._k = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
         h = ctypes.c_void_p()
-        _check(load().stfem_problem_create(spec_string(spec).encode(), _p(self._v), _p(self._k),
+        text = spec if isinstance(spec, str) else spec_string(spec)
+        _check(load().stfem_problem_create(text.encode(), _p(self._v), _p(self._k),
                                            device, ctypes.byref(h)))
         self.h = h
         self.device = device
-        self.dim = int(spec.get("dim", 2))
+        self.dim = int(spec_normalize(text)["dim"])
         info = np.zeros(6, np.int64)
         _check(load().stfem_problem_info(self.h, info.ctypes.data_as(_lp)))
         self.n_levels, self.nt, self.n_nodes, self.n_cells, self.level_size, self.nts = \
@@ -332,6 +345,14 @@ class Problem:
         ms = ctypes.c_double()
         _check(load().stfem_time_kernel(self.h, what, level, warmup, reps, ctypes.byref(ms)))
         return ms.value
+
+    def phase_timings(self, reset=False):
+        """{phase: (ms, calls)} for vmult, vanka, gmg, step (spec print_timing=1)."""
+        ms = np.zeros(4)
+        calls = np.zeros(4, np.int64)
+        _check(load().stfem_phase_timings(self.h, int(reset), _p(ms), calls.ctypes.data_as(_lp)))
+        return {k: (float(ms[i]), int(calls[i]))
+                for i, k in enumerate(("vmult", "vanka", "gmg", "step"))}
 
     def time_slabs(self, n_slabs, allow_unconverged=False):
         """(ms, iterations) of n_slabs heat slabs; with allow_unconverged,

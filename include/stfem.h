@@ -77,12 +77,19 @@ int stfem_time_mg_sequence(int n_sp_lvl, int k_max, int k_min, int nts, int nts_
  * and rhs matrices (SystemMatrix, operators.h:71-256), MG levels with
  * PreconditionVanka (gmg.h:402-540), transfers (gmg.h:35-400), GMG
  * (gmg.h:652-906) and FGMRES (time_integrators.h:44, :113).
- * spec: "key=value;..." with keys dim, subdiv (a,b,c), lo, hi, refine, p,
+ * spec: "key=value;..." (or a JSON parameter file, see stfem_spec_normalize)
+ * with keys dim, subdiv (a,b,c), lo, hi, refine, p,
  * type (DG|CGP), r, nts, tau, nts_min, r_min, space_time_mg,
  * time_before_space, mg_space, mg_float, smoothing_steps, variable,
  * estimate_relaxation, eig_iters, eig_safety, overwrite_transfer,
- * vanka_factored, exact_single_cell, use_graph, maxiter, abstol, reltol,
- * basis, extrapolate, end_time, frequency, source (0 manufactured heat
+ * vanka_factored, exact_single_cell, use_graph (1 replay the V-cycle as a
+ * HIP graph, 0 launch it kernel by kernel, -1 (default) the graph unless a
+ * rocprofiler-sdk tool is loaded), maxiter, abstol, reltol,
+ * basis, extrapolate (1: every block of the start vector is the previous
+ * slab's last value, 0: zero; time_integrators.h:96-104), end_time,
+ * frequency, restrict_is_transpose (1 default: time restriction = P^T; 0:
+ * the L2 restriction / projection matrices, gmg.h:101-113), print_timing,
+ * source (0 manufactured heat
  * source, 1 none, 2 load vector set by stfem_set_load_vector, 3 seeded
  * Gaussian bumps with n_bumps, seed), dirichlet (face bit mask), problem
  * (heat, or wave: the first-order-in-time wave system of tp_01.cc:167-197,
@@ -104,11 +111,30 @@ int stfem_time_mg_sequence(int n_sp_lvl, int k_max, int k_min, int nts, int nts_
  * operators; op_jb64=0, the default, picks 2 from Q4 on and 4 below),
  * op_split (operators with fewer cell workgroups than this, 128 by
  * default, compute one output time component per workgroup).
+ * Unknown keys are an error (STFEM_ERR_ARG).
  * vertices: finest-level vertex coordinates ((nc+1)^dim points, dim each,
  * x fastest) or NULL for the uniform box; cell_coef: finest-level per-cell
  * coefficient or NULL for 1. */
 int stfem_problem_create(const char *spec, const double *vertices, const double *cell_coef,
                          int device, stfem_problem *out);
+/* Parameters::parse (tests/include/gmg.h:591-649) without building anything:
+ * the spec of stfem_problem_create with every key resolved, as
+ * "key=value;...", into out (cap bytes including the terminating zero; out
+ * NULL only checks the spec).  Accepted besides this library's keys: the
+ * camelCase keys of the reference's parameter files (spaceTimeMg,
+ * mgTimeBeforeSpace, timeType, problemType, nTimestepsAtOnce[Min],
+ * feDegree[Min] (the time degree; the space degree defaults to feDegree+1 as
+ * in tp_01.cc:77), refinement, extrapolate, hyperRectLowerLeft/UpperRight,
+ * subdivisions, endTime, smoothingSteps, estimateRelaxation,
+ * restrictIsTransposeProlongate, variable, printTiming), tp_01's
+ * precondition_float, and a whole JSON parameter file (a spec starting with
+ * '{').  Driver-only keys (doOutput, functionalFile, nDegCycles, nRefCycles,
+ * sourcePoint, smoothingDegree, coarseGrid{Maxiter,Abstol,Reltol}) are
+ * checked and have no effect.  Unknown keys, malformed values and keys
+ * given twice return STFEM_ERR_ARG; coarseGridSmootherType other than
+ * "Smoother", spaceTimeConvergenceTest=false, nonzero distortGrid /
+ * distortCoeff and timeType GCC return STFEM_ERR_UNSUPPORTED. */
+int stfem_spec_normalize(const char *spec, char *out, int cap);
 int stfem_problem_destroy(stfem_problem p);
 /* info[0..5] = n_levels, nt, n_nodes(fine), n_cells(fine), sizeof(level
  * number), n_timesteps_at_once */
@@ -213,10 +239,22 @@ int stfem_nodes_to_blocks(long n_nodes, int nt, const double *nodes, double *blo
  * what 0 fine FP64 vmult, 1 level vmult, 2 level Vanka, 3 GMG V-cycle,
  * 4 prolongate_and_add into `level`, 5 restrict_and_add from `level` */
 int stfem_time_kernel(stfem_problem p, int what, int level, int warmup, int reps, double *ms);
-/* runs `n_slabs` heat slabs from a zero initial value; returns the summed
- * slab-solve time (ms, HIP events) and the summed FGMRES iterations.  Slabs
- * that reach the iteration limit are timed and counted, and the run goes on
- * (status STFEM_ERR_NO_CONVERGENCE at the end); any other failure stops it. */
+/* TimerOutput scopes of the reference (printTiming): device time (ms) and
+ * call counts of the phases {vmult, vanka, gmg, step} since creation or the
+ * last reset: vmult = every SystemMatrix application (operators.h:97),
+ * vanka = every smoother application (gmg.h:479), gmg = GMG::vmult
+ * (gmg.h:850), step = one slab solve (tp_01.cc:632).  Needs the spec key
+ * print_timing=1 (the V-cycle then launches eagerly; the totals are also
+ * printed to stderr when the problem is destroyed).  reset != 0 zeroes the
+ * totals after reading. */
+int stfem_phase_timings(stfem_problem p, int reset, double *ms, long *calls);
+/* runs `n_slabs` heat slabs of the configured run (from t = 0 and a zero
+ * initial value to end_time in end_time / (nts * tau) slabs; after the last
+ * slab of a run the next one starts again from t = 0 and zero); returns the
+ * summed slab-solve time (ms, HIP events) and the summed FGMRES iterations.
+ * Slabs that reach the iteration limit are timed and counted, and the run
+ * goes on (status STFEM_ERR_NO_CONVERGENCE at the end); any other failure
+ * stops it. */
 int stfem_time_slabs(stfem_problem p, int n_slabs, double *ms, int *its);
 
 #ifdef __cplusplus

@@ -46,6 +46,11 @@ def load():
         L.oracle_gmg_vmult.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.oracle_fine_matrix.restype = ctypes.c_long
         L.oracle_fine_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _lp, _dp]
+        L.oracle_level_matrix.restype = ctypes.c_long
+        L.oracle_level_matrix.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _lp, _lp,
+                                          _dp]
+        L.oracle_level_lambda.restype = ctypes.c_double
+        L.oracle_level_lambda.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.oracle_fgmres.argtypes = [ctypes.c_void_p, _dp, _dp, _ip, _dp, ctypes.c_int]
         L.oracle_run_heat.argtypes = [ctypes.c_void_p, _dp, _ip, ctypes.c_int]
         L.oracle_fe_time_weights.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
@@ -230,6 +235,36 @@ class Problem:
         v = np.zeros(nnz)
         L.oracle_fine_matrix(self.h, which, rp.ctypes.data_as(_lp), ci.ctypes.data_as(_lp), _p(v))
         return rp, ci, v
+
+    def level_matrix(self, l, which):
+        """Assembled K (which=0) or M (1) of MG level l as a dense array
+        (deal.II constraint convention: constrained rows/columns reduced to
+        the diagonal)."""
+        L = load()
+        nnz = L.oracle_level_matrix(self.h, l, which, None, None, None)
+        n = self.level_info(l)["n_nodes"]
+        rp = np.zeros(n + 1, np.int64)
+        ci = np.zeros(nnz, np.int64)
+        v = np.zeros(nnz)
+        L.oracle_level_matrix(self.h, l, which, rp.ctypes.data_as(_lp), ci.ctypes.data_as(_lp),
+                              _p(v))
+        A = np.zeros((n, n))
+        for i in range(n):
+            A[i, ci[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+        return A
+
+    def load_vector(self, time=0.0):
+        """int f(time, x) phi_i over the fine level (zero at Dirichlet nodes)."""
+        L = load()
+        L.oracle_load_vector.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
+        out = np.zeros(self.level_info(self.n_levels - 1)["n_nodes"])
+        L.oracle_load_vector(self.h, float(time), _p(out))
+        return out
+
+    def level_lambda(self, l):
+        """Raw power-iteration estimate of the largest eigenvalue of the
+        Vanka-preconditioned level operator (0 where none was made)."""
+        return load().oracle_level_lambda(self.h, l)
 
     def fgmres(self, b, x0):
         x = np.ascontiguousarray(x0, np.float64).copy()

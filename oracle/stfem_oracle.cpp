@@ -43,6 +43,7 @@
 // of tests/transfer_01.output and tests/tp_01.output.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -51,6 +52,7 @@
 #include <map>
 #include <memory>
 #include <numeric>
+#include <random>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -1441,7 +1443,10 @@ namespace orc
     int                 extrapolate = 1;
     double              end_time = 1.0;
     double              frequency = 1.0;
-    int                 rhs = 0; // 0: manufactured solution's source, 1: zero
+    int                 rhs = 0; // 0: manufactured solution's source, 1: zero, 3: Gaussian bumps
+    int                 n_bumps = 4;  // rhs=3: number of bumps
+    long                seed = 1;     // rhs=3: generator seed
+    int                 r_is_pT = 1;  // time restriction = P^T (gmg.h:557)
     int                 problem = 0; // 0: heat, 1: wave (tp_01.cc:167-197, time_integrators.h:183-280)
     int                 coarse_coef = 0; // coarse-level coefficient: 0 sampled, 1/2/3 geometric/harmonic/arithmetic mean
     int                 dirichlet = 63;  // zero-Dirichlet faces (bit 2d: low side, 2d+1: high side)
@@ -1511,6 +1516,10 @@ namespace orc
     getd("end_time", sp.end_time);
     getd("frequency", sp.frequency);
     geti("rhs", sp.rhs);
+    geti("n_bumps", sp.n_bumps);
+    if (kv.count("seed"))
+      sp.seed = std::atol(kv["seed"].c_str());
+    geti("r_is_pT", sp.r_is_pT);
     if (kv.count("problem"))
       sp.problem = (kv["problem"] == "wave" || kv["problem"] == "1") ? 1 : 0;
     geti("coarse_coef", sp.coarse_coef);
@@ -1554,6 +1563,7 @@ namespace orc
     int                        nt = 1;
     double                     tau = 0.0;
     std::vector<double>        omega_log;
+    std::vector<double>        lambda_log; // raw power-iteration estimates per level
     // FGMRES statistics of the last solve
     int                        last_its = 0;
     std::vector<double>        res_hist;
@@ -1612,6 +1622,8 @@ namespace orc
     void init(const Spec &s, const double *coords, const double *kappa)
     {
       spec = s;
+      if (spec.rhs == 3)
+        make_bumps();
       sh.init(spec.p, spec.p + 1);
       // fine grid
       fine_grid.dim       = spec.dim;
@@ -1705,10 +1717,15 @@ namespace orc
             }
           else
             {
-              // gmg.h:86-115 with restrict_is_transpose_prolongate = true
+              // gmg.h:86-115: R = P^T (restrict_is_transpose_prolongate, the
+              // default of gmg.h:557) or the restriction / projection matrix
               t.Pt = (t.kind == MG_K) ? time_projection(spec.type, rl[l] - 1, rl[l], nl[l]) :
                                         time_prolongation(spec.type, rl[l], nl[l]);
-              t.Rt = transpose(t.Pt);
+              if (spec.r_is_pT)
+                t.Rt = transpose(t.Pt);
+              else
+                t.Rt = (t.kind == MG_K) ? time_projection(spec.type, rl[l], rl[l] - 1, nl[l]) :
+                                          time_restriction(spec.type, rl[l], nl[l]);
               t.Ptt.assign(t.Pt.a.begin(), t.Pt.a.end());
               t.Rtt.assign(t.Rt.a.begin(), t.Rt.a.end());
             }
@@ -1761,6 +1778,7 @@ namespace orc
       }
       // relaxation parameters (gmg.h:708-844)
       omega_log.assign(nlev, 1.0);
+      lambda_log.assign(nlev, 0.0);
       for (int l = 0; l < nlev; ++l)
         {
           lv[l].omega  = spec.estimate_relaxation ? estimate_omega(l) : 1.0;
@@ -1828,6 +1846,7 @@ namespace orc
       // nothing to estimate
       if (!std::isfinite(est) || est == 0.0)
         return 1.0;
+      lambda_log[l] = est;
       double lmax  = std::abs(est) * spec.eig_safety;
       double alpha = spec.omega_rule == 0 ? 0.9 * lmax : 0.0;
       return spec.omega_scale * 2.0 / (alpha + lmax);
@@ -2030,11 +2049,47 @@ namespace orc
       return state == 1 ? 0 : -1;
     }
 
+    // Gaussian-bump source h(x) (rhs=3, time profile 1): bump b has centre
+    // c_b and amplitude a_b from the 64-bit Mersenne twister seeded with
+    // `seed`, each variate u = (top 53 bits) / 2^53, c_b = 0.2 + 0.6 u per
+    // coordinate (x, y, z drawn in that order even in 2D), a_b = 0.5 + u;
+    // h(x) = sum_b a_b exp(-|x - c_b|^2 / 0.02).  The benchmark's synthetic
+    // forcing: not a reference source (the reference has the manufactured
+    // source of exact_solution.h:53-75 only).
+    std::vector<std::array<double, 4>> bumps;
+    void make_bumps()
+    {
+      std::mt19937_64 gen{static_cast<unsigned long long>(spec.seed)};
+      bumps.assign(std::max(1, spec.n_bumps), {0, 0, 0, 0});
+      for (auto &b : bumps)
+        {
+          for (int d = 0; d < 4; ++d)
+            {
+              const double u = std::ldexp(double(gen() >> 11), -53);
+              b[d] = d < 3 ? 0.2 + 0.6 * u : 0.5 + u;
+            }
+        }
+    }
+    double bump_value(const double *x) const
+    {
+      double v = 0.0;
+      for (const auto &b : bumps)
+        {
+          double d2 = 0.0;
+          for (int d = 0; d < spec.dim; ++d)
+            d2 += (x[d] - b[d]) * (x[d] - b[d]);
+          v += b[3] * std::exp(-50.0 * d2);
+        }
+      return v;
+    }
+
     // manufactured heat source, exact_solution.h:53-75 (frequency f)
     double rhs_value(double t, const double *x) const
     {
       if (spec.rhs == 1)
         return 0.0;
+      if (spec.rhs == 3)
+        return bump_value(x);
       if (spec.problem == 1)
         {
           // wave::RHSFunction (exact_solution.h:166-180)
@@ -2596,6 +2651,39 @@ extern "C"
     ORC_DISPATCH(h, P.gmg_vmult(dst, src));
   }
   // assembled fine K or M (CSR) export: returns nnz; arrays may be null
+  // the load vector int f(t, x) phi_i dx of the fine level (Dirichlet
+  // entries zero), the spatial part of assemble_force (time_integrators.h:60-87)
+  void oracle_load_vector(void *h, double time, double *out)
+  {
+    ORC_DISPATCH(h, P.integrate_rhs(time, out));
+  }
+  // raw power-iteration estimate of lambda_max(P^-1 A) on level l (0 where
+  // no estimate was made)
+  double oracle_level_lambda(void *h, int l)
+  {
+    double out = 0.0;
+    ORC_DISPATCH(h, out = P.lambda_log.at(l));
+    return out;
+  }
+  // assembled K (which=0) or M (1) of level l in CSR, FP64 values
+  long oracle_level_matrix(void *h, int l, int which, long *rp, long *ci, double *v)
+  {
+    long nnz = 0;
+    ORC_DISPATCH(h, {
+      const CSR &A = which == 0 ? P.lv.at(l).K : P.lv.at(l).M;
+      nnz          = (long)A.ci.size();
+      if (rp)
+        for (size_t i = 0; i < A.rp.size(); ++i)
+          rp[i] = A.rp[i];
+      if (ci)
+        for (size_t i = 0; i < A.ci.size(); ++i)
+          ci[i] = A.ci[i];
+      if (v)
+        for (size_t i = 0; i < A.v.size(); ++i)
+          v[i] = A.v[i];
+    });
+    return nnz;
+  }
   long oracle_fine_matrix(void *h, int which, long *rp, long *ci, double *v)
   {
     long nnz = 0;

@@ -154,8 +154,9 @@ def test_wave_layered_pulse_matches_oracle():
     sys.path.insert(0, os.path.join(os.path.dirname(__file__), ".."))
     import bench
     spec, X, kappa, centre = bench.workload_wave(2)
-    spec = dict(spec, rhs=1, mg_float=1)
-    o, g = orc.Problem(spec, coords=X, kappa=kappa), st.Problem(spec, vertices=X, kappa=kappa)
+    spec = dict(spec, mg_float=1)
+    o = orc.Problem(dict(spec, rhs=1), coords=X, kappa=kappa)
+    g = st.Problem(spec, vertices=X, kappa=kappa)
     n, nt = g.n_nodes, g.nt
     u0 = bench.wave_initial(g, centre)
     assert np.abs(u0).max() > 0.1

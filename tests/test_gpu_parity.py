@@ -25,9 +25,8 @@ CASES = {
     "2d_dg1_nts2": dict(spec=dict(dim=2, refine=2, type=2, r=1, nts=2, r_min=0), perturb=0.0,
                         kappa=False),
     # tp_01 tf02 shape: 2D CGP(2), 2 steps, FP32 multigrid
-    "2d_cg2_nts2_fp32": dict(spec=dict(dim=2, refine=2, type=1, r=2, nts=2, r_min=1, mg_float=1,
-                                       vanka_invert_T=0),
-                             perturb=0.0, kappa=False),
+    "2d_cg2_nts2_fp32": dict(spec=dict(dim=2, refine=2, type=1, r=2, nts=2, r_min=1, mg_float=1),
+                             perturb=0.0, kappa=False, oracle=dict(vanka_invert_T=0)),
     # distorted 2D mesh with a cell coefficient, DG(2), Q3
     "2d_dg2_distorted": dict(spec=dict(dim=2, refine=3, type=2, r=2, nts=1, p=3, r_min=0),
                              perturb=0.2, kappa=True),
@@ -38,16 +37,17 @@ CASES = {
     "3d_q3_dg2_factored": dict(spec=dict(dim=3, refine=2, type=2, r=2, nts=1, p=3, r_min=0,
                                          vanka_factored=1), perturb=0.2, kappa=True),
     # the 64^3 benchmark's discretisation (Q4, dG(3)), 2^3 cells, FP32 multigrid
-    "3d_q4_dg3_fp32": dict(spec=dict(dim=3, refine=1, type=2, r=3, nts=1, p=4, r_min=2, vanka_invert_T=0,
-                                     mg_float=1), perturb=0.1, kappa=True),
+    "3d_q4_dg3_fp32": dict(spec=dict(dim=3, refine=1, type=2, r=3, nts=1, p=4, r_min=2,
+                                     mg_float=1), perturb=0.1, kappa=True,
+                           oracle=dict(vanka_invert_T=0)),
     # the same with the factored Vanka set up on the device
     "3d_q4_dg3_fp32_factored": dict(spec=dict(dim=3, refine=1, type=2, r=3, nts=1, p=4, r_min=2,
                                               mg_float=1, vanka_factored=1),
                                     perturb=0.1, kappa=True),
     # FP32 multigrid with the factored Vanka (the benchmark configuration)
     "3d_q3_dg2_fp32_factored": dict(spec=dict(dim=3, refine=2, type=2, r=2, nts=1, p=3, r_min=0,
-                                              mg_float=1, vanka_factored=1, vanka_invert_T=0),
-                                    perturb=0.2, kappa=True),
+                                              mg_float=1, vanka_factored=1),
+                                    perturb=0.2, kappa=True, oracle=dict(vanka_invert_T=0)),
     # the wave benchmark's time discretisation cG(2) with Q2 (heat operator), 4^3 cells
     "3d_q2_cg2": dict(spec=dict(dim=3, refine=2, type=1, r=2, nts=2, p=2, r_min=1), perturb=0.0,
                       kappa=False),
@@ -79,7 +79,9 @@ def _build(name):
     nc = [1 << spec["refine"]] * dim
     X = box_vertices(dim, nc, perturb=c["perturb"], seed=7) if c["perturb"] > 0 else None
     kap = log_uniform(int(np.prod(nc)), 1e-2, 1e2, seed=11) if c["kappa"] else None
-    o = orc.Problem(spec, coords=X, kappa=kap)
+    # oracle-only keys (the explicit FP32 inverses formed in FP64 and then
+    # rounded, as the library does) go to the oracle alone
+    o = orc.Problem(dict(spec, **c.get("oracle", {})), coords=X, kappa=kap)
     g = st.Problem(spec, vertices=X, kappa=kap)
     return spec, o, g
 

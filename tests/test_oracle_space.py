@@ -34,8 +34,15 @@ def csr_dense(rp, ci, v):
     return A
 
 
-def fine_KM(spec, X=None, kappa=None):
+def fine_KM(spec, X=None, kappa=None, sparse=False):
     P = orc.Problem(dict(spec, dirichlet=0), coords=X, kappa=kappa)
+    if sparse:
+        import scipy.sparse as sp
+        mats = []
+        for w in (0, 1):
+            rp, ci, v = P.fine_matrix(w)
+            mats.append(sp.csr_matrix((v, ci, rp), shape=(len(rp) - 1, len(rp) - 1)))
+        return P, mats[0], mats[1]
     K = csr_dense(*P.fine_matrix(0))
     M = csr_dense(*P.fine_matrix(1))
     return P, K, M
@@ -81,11 +88,32 @@ def cell_volumes(dim, nc, X):
             vol.append(0.5 * abs(sum(q[k][0] * q[(k + 1) % 4][1] - q[(k + 1) % 4][0] * q[k][1]
                                      for k in range(4))))
         else:
-            raise NotImplementedError
+            # trilinear hexahedron: det J has degree <= 2 per reference
+            # variable, so 2-point Gauss per direction integrates it exactly
+            g = 0.5 + np.array([-1.0, 1.0]) / (2.0 * np.sqrt(3.0))
+            V = np.array([X[(c[0] + (v & 1)) + nv[0] * ((c[1] + ((v >> 1) & 1)) +
+                                                        nv[1] * (c[2] + ((v >> 2) & 1)))]
+                          for v in range(8)])
+            tot = 0.0
+            for xi in np.array(np.meshgrid(g, g, g)).reshape(3, -1).T:
+                J = np.zeros((3, 3))
+                for v in range(8):
+                    b = [(v >> e) & 1 for e in range(3)]
+                    for e in range(3):
+                        gr = 1.0
+                        for f in range(3):
+                            if f == e:
+                                gr *= 1.0 if b[f] else -1.0
+                            else:
+                                gr *= xi[f] if b[f] else 1 - xi[f]
+                        J[:, e] += gr * V[v]
+                tot += 0.125 * abs(np.linalg.det(J))
+            vol.append(tot)
     return np.array(vol)
 
 
-CASES = [(2, 1, 0.0), (2, 2, 0.25), (2, 3, 0.25), (2, 4, 0.2), (3, 1, 0.0), (3, 2, 0.2)]
+CASES = [(2, 1, 0.0), (2, 2, 0.25), (2, 3, 0.25), (2, 4, 0.2), (3, 1, 0.0), (3, 2, 0.2),
+         (3, 3, 0.2), (3, 4, 0.15), (3, 5, 0.1)]
 
 
 @pytest.mark.parametrize("dim,p,perturb", CASES)
@@ -101,19 +129,24 @@ def test_mass_and_stiffness_invariants(dim, p, perturb):
     assert np.allclose(K, K.T, atol=1e-14 * np.abs(K).max())
     assert np.allclose(M, M.T, atol=1e-15)
     if p >= 2:
+        # x*y (2D; degree 2 per reference variable on Q1-mapped cells) and
+        # x*y*z (3D, degree 3: in the space from p = 3 on) are interpolated
+        # exactly and integrated exactly by Gauss(p+1)
         Xv = X if X is not None else box_vertices(dim, nc, 0.0)
         xs = node_coords(dim, nc, p, Xv)
-        u = np.prod(xs, axis=1)
-        assert abs(u @ M @ one - 0.5 ** dim) < 1e-13
+        k = dim if (dim == 2 or p >= 3) else 2
+        u = np.prod(xs[:, :k], axis=1)
+        assert abs(u @ M @ one - 0.5 ** k) < 1e-13
 
 
-@pytest.mark.parametrize("p,perturb", [(1, 0.0), (2, 0.25), (3, 0.2)])
-def test_stiffness_energy_with_cell_coefficient(p, perturb):
-    dim, refine = 2, 2
+@pytest.mark.parametrize("dim,p,perturb", [(2, 1, 0.0), (2, 2, 0.25), (2, 3, 0.2), (3, 1, 0.2),
+                                            (3, 3, 0.2), (3, 4, 0.15), (3, 5, 0.1)])
+def test_stiffness_energy_with_cell_coefficient(dim, p, perturb):
+    refine = 2
     nc = [1 << refine] * dim
     X = box_vertices(dim, nc, perturb=perturb, seed=5)
     kap = log_uniform(int(np.prod(nc)), 1e-2, 1e2, seed=9)
-    _, K, _ = fine_KM(dict(dim=dim, refine=refine, p=p, type=2, r=0, nts=1), X, kap)
+    _, K, _ = fine_KM(dict(dim=dim, refine=refine, p=p, type=2, r=0, nts=1), X, kap, sparse=True)
     u = node_coords(dim, nc, p, X)[:, 0]
     ref = float(np.sum(kap * cell_volumes(dim, nc, X)))
     assert abs(u @ K @ u - ref) < 1e-12 * ref
