@@ -1,28 +1,39 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X space-time multigrid slab solve.
 
-Metric: space-time DoF/s per slab solve to 1e-8 relative residual with the
-FP32 multigrid preconditioner inside FP64 FGMRES (value), next to the plain
-FP64 solve, plus the FP64 space-time vmult in HBM GB/s.
+Metric: space-time DoF/s per slab solve to a 1e-8 relative FP64 residual
+with the FP32 multigrid preconditioner inside FP64 FGMRES (value), next to
+the plain FP64 solve, plus the FP64 space-time vmult in HBM GB/s.
 
-Workload (default): 3D heat on the unit cube, 32^3 cells with vertices
-perturbed by up to 0.2h (seeded), Q3 space, dG(2) time (3 time dofs per
-slab, 2.74M space-time dofs per slab), T = 1 in 8 slabs, log-uniform
-cell coefficient in [1e-2, 1e2] (seeded), seeded Gaussian-bump source.
-A step is one slab solve (rhs assembly, extrapolation, FGMRES + GMG).
+Workload (default, --workload headline): 3D heat on the unit cube, 64^3
+cells with vertices perturbed by up to 0.1h (seeded), Q4 space, dG(3) time
+(4 time dofs per slab, 67.9M space-time dofs per slab), T = 1 in 4 slabs,
+cellwise log-uniform coefficient in [1e-3, 1e3] (seeded), seeded
+Gaussian-bump source.  A step is one slab solve of that run (rhs assembly,
+extrapolation, FGMRES + GMG); the run restarts from t = 0 and a zero state
+after its 4th slab, so K steps are K slabs of T = 1 runs.
+
+Other workloads: --workload q3 (32^3 cells, Q3/dG(2), 0.2h, [1e-2, 1e2],
+T = 1 in 8 slabs) and --workload wave (3D wave, Q2/cG(2)).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dealii-stfem_amd"))
 
-HBM_PEAK_GBS = 8000.0
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def box_vertices(dim, nc, perturb, seed):
@@ -41,11 +52,11 @@ def box_vertices(dim, nc, perturb, seed):
     return np.ascontiguousarray(X)
 
 
-def workload(refine):
-    """Spec, vertices and coefficient of the 3D heat workload on 2^refine cells per side
-    (Q3 space, dG(2) time, vertices perturbed by up to 0.2h, cellwise log-uniform
-    coefficient in [1e-2, 1e2], seeded Gaussian bumps).  Two pre- and two
-    post-smoothing steps with the factored Vanka smoother."""
+def workload_q3(refine=5):
+    """3D heat on 2^refine cells per side: Q3 space, dG(2) time, vertices
+    perturbed by up to 0.2h, cellwise log-uniform coefficient in [1e-2, 1e2],
+    seeded Gaussian bumps, T = 1 in 8 slabs; two pre- and two post-smoothing
+    steps with the factored Vanka smoother."""
     import numpy as np
     dim = 3
     nc = [1 << refine] * dim
@@ -55,23 +66,24 @@ def workload(refine):
     spec = dict(dim=3, refine=refine, p=3, type=2, r=2, nts=1, r_min=0, tau=0.125,
                 end_time=1.0, source=3, n_bumps=4, seed=7, reltol=1e-8, abstol=1e-300,
                 maxiter=200, basis=30, vanka_factored=1, smoothing_steps=2)
-    return spec, X, kappa
+    desc = (f"3D heat, {1 << refine}^3 cells perturbed 0.2h, Q3 space, dG(2) time, "
+            f"log-uniform kappa [1e-2,1e2], Gaussian bumps, T=1 in 8 slabs, tol 1e-8")
+    return spec, X, kappa, desc
 
 
-def workload_large():
-    """Spec, vertices and coefficient of the large 3D heat workload: 64^3
-    cells perturbed by up to 0.1h, Q4 space, dG(3) time (4 time dofs per
-    slab, 67.9M space-time dofs per slab), T = 1 in 4 slabs, cellwise
-    log-uniform coefficient in [1e-3, 1e3], seeded Gaussian bumps.  Three pre-
-    and three post-smoothing steps (35 iterations per slab instead of 51 with
-    two, 6 % less time); the coarse levels take the arithmetic mean of the
-    finest-level coefficient over each quadrature point's share of the cell
-    (with the point-sampled coefficient the multigrid does not converge for
-    this contrast); the V-cycle from the 4^3-cell level down (19,652 DoFs) is
-    one dense operator; an FGMRES basis of 60 vectors (no restart within a
-    slab: 34 instead of 35.5 iterations, 3 % less time)."""
+def workload_headline(refine=6):
+    """The large 3D heat workload: 2^refine (64) cells per side perturbed by
+    up to 0.1h, Q4 space, dG(3) time (4 time dofs per slab, 67.9M space-time
+    dofs per slab at 64^3), T = 1 in 4 slabs, cellwise log-uniform
+    coefficient in [1e-3, 1e3], seeded Gaussian bumps.  Three pre- and three
+    post-smoothing steps (35 iterations per slab instead of 51 with two, 6 %
+    less time); the coarse levels take the arithmetic mean of the finest-level
+    coefficient over each quadrature point's share of the cell (with the
+    point-sampled coefficient the multigrid does not converge for this
+    contrast); the V-cycle from the 4^3-cell level down (19,652 DoFs) is one
+    dense operator; an FGMRES basis of 60 vectors (no restart within a slab)."""
     import numpy as np
-    dim, refine = 3, 6
+    dim = 3
     nc = [1 << refine] * dim
     X = box_vertices(dim, nc, 0.1, seed=2026)
     rng = np.random.default_rng(2027)
@@ -80,7 +92,9 @@ def workload_large():
                 source=3, n_bumps=4, seed=7, reltol=1e-8, abstol=1e-300, maxiter=200,
                 basis=60, vanka_factored=1, smoothing_steps=3, coarse_coef=3,
                 dense_coarse_dofs=20000)
-    return spec, X, kappa
+    desc = (f"3D heat, {1 << refine}^3 cells perturbed 0.1h, Q4 space, dG(3) time, "
+            f"log-uniform kappa [1e-3,1e3], Gaussian bumps, T=1 in 4 slabs, tol 1e-8")
+    return spec, X, kappa, desc
 
 
 def workload_wave(refine=5):
@@ -154,7 +168,7 @@ def run_wave(spec, X, kappa, centre, steps, warmup=1):
 def main_wave(a):
     """The 3D wave workload: FP32 multigrid against FP64, solution difference."""
     import torch
-    spec, X, kappa, centre = workload_wave(a.refine)
+    spec, X, kappa, centre = workload_wave(a.refine or 5)
     for kv in a.set:
         k, v = kv.split("=", 1)
         spec[k] = int(v) if v.lstrip("-").isdigit() else float(v)
@@ -171,9 +185,9 @@ def main_wave(a):
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms32 / a.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 (FGMRES) + f32 (multigrid)", "data": "synthetic",
-        "config": {"workload": (f"3D wave, {1 << a.refine}^3 cells, Q2 space, cG(2) time, "
-                                f"c^2 = 1 | 4 split at z = 0.5, Gaussian pulse width 0.05, "
-                                f"v0 = 0, tol 1e-8"),
+        "config": {"workload": (f"3D wave, {1 << (a.refine or 5)}^3 cells, Q2 space, cG(2) "
+                                f"time, c^2 = 1 | 4 split at z = 0.5, Gaussian pulse width "
+                                f"0.05, v0 = 0, tol 1e-8"),
                    "space_time_dofs_per_slab": nst},
         "fp64_dofs_per_s": a.steps * nst / (ms64 * 1e-3),
         "speedup_fp32mg_vs_fp64": ms64 / ms32,
@@ -183,10 +197,6 @@ def main_wave(a):
         "fp32mg_vs_fp64_rel_l2": {"u": du, "v": dv},
     }
     print(json.dumps(out))
-
-
-def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def run_gpu(spec, X, kappa, steps, warmup):
@@ -213,50 +223,76 @@ def final_state(g, n_slabs=2):
     prev = torch.zeros(n, dtype=torch.float64, device="cuda:0")
     for k in range(n_slabs):
         prev.copy_(x.view(n, nt)[:, nt - 1])
-        g.solve_slab(x, prev, k * g.tau)
+        g.solve_slab(x, prev, k * g.spec_nts * g.tau)
     torch.cuda.synchronize()
     return x
 
 
-def cpu_baseline(budget_s=25.0):
-    """Oracle (FP64 CPU restatement, single thread) on a bounded sample: the
-    same discretisation and solver on 8^3 cells, slab solves until the
-    budget is used."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(workload, budget_s=30.0, min_s=10.0):
+    """The oracle (FP64 CPU restatement of the same algorithm: assembled
+    matrices, explicit Vanka inverses, OpenMP over rows and cells on every
+    host thread it is given) on a bounded sample of the same discretisation
+    and source: the workload's spec on 8^3 cells, the
+    slabs of one T = 1 run (slab solves until the budget is used)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as orc
-    spec, X, kappa = workload(3)
-    ospec = dict(dim=3, refine=3, p=3, type=2, r=2, nts=1, r_min=0, tau=0.125, reltol=1e-8,
-                 abstol=1e-300, rhs=1)
+    refine = 3
+    spec, X, kappa, _ = (workload_headline if workload == "headline" else workload_q3)(refine)
+    ospec = {k: v for k, v in spec.items()
+             if k not in ("source", "vanka_factored", "dense_coarse_dofs")}
+    ospec["rhs"] = spec["source"]
+    t0 = time.perf_counter()
     o = orc.Problem(ospec, coords=X, kappa=kappa)
+    setup = time.perf_counter() - t0
     info = o.level_info(o.n_levels - 1)
     n, nt = info["n_nodes"], info["nt"]
-    # start from a nonzero state so every slab does the full solve
-    rng = np.random.default_rng(1)
-    x = rng.standard_normal(n * nt)
+    period = int(round(spec["end_time"] / (spec["nts"] * spec["tau"])))
+    x = np.zeros(n * nt)
     t0 = time.perf_counter()
     slabs, its = 0, 0
-    tm = 0.0
-    while time.perf_counter() - t0 < budget_s and slabs < 8:
+    # whole T = 1 runs (restarting from zero) until min_s has passed, within
+    # the budget
+    while slabs % period or time.perf_counter() - t0 < min_s:
+        if time.perf_counter() - t0 > budget_s:
+            break
+        if slabs % period == 0:
+            x = np.zeros(n * nt)
         prev = x[(nt - 1) * n:].copy()
-        rc, x, k = orc.solve_slab(o, x, prev, tm)
+        rc, x, k = orc.solve_slab(o, x, prev, (slabs % period) * spec["nts"] * spec["tau"])
         its += k
         slabs += 1
-        tm += 0.125
     dt = time.perf_counter() - t0
     # the space-time vmult of the same sample (assembled CSR matrices)
+    rng = np.random.default_rng(1)
     xb = rng.standard_normal(n * nt)
     reps, tv0 = 0, time.perf_counter()
     while time.perf_counter() - tv0 < 2.0:
         o.fine_vmult(xb)
         reps += 1
     tv = (time.perf_counter() - tv0) / reps
-    return dict(value=slabs * n * nt / dt, unit="space-time DoF/s", cores=1, kind="port",
-                sample=f"oracle FP64 slab solves of the same Q3/dG(2) perturbed-mesh heat problem "
-                       f"on 8^3 cells ({n * nt} space-time dofs/slab), {slabs} slabs, {its} "
-                       f"FGMRES its, {dt:.1f} s",
+    thr = orc.num_threads()
+    return dict(value=slabs * n * nt / dt, unit="space-time DoF/s", cores=thr, kind="port",
+                cpu_model=cpu_model(), host_cpus=os.cpu_count(),
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                sample=(f"oracle FP64 slab solves of the same discretisation and source "
+                        f"(spec of the workload on {1 << refine}^3 cells, {n * nt} space-time "
+                        f"dofs/slab): {slabs} slabs of T=1 runs ({period} slabs each), {its} "
+                        f"FGMRES its, "
+                        f"{dt:.1f} s on {thr} threads (setup {setup:.1f} s untimed)"),
                 vmult={"value": n * nt / tv, "unit": "space-time DoF/s", "ms": tv * 1e3,
-                       "sample": f"oracle FP64 space-time vmult on the same 8^3 problem, {reps} "
+                       "sample": f"oracle FP64 space-time vmult on the same sample, {reps} "
                                  f"applications"})
 
 
@@ -271,11 +307,12 @@ def _counter_csv(d):
     return None
 
 
-def pmc_traffic(kernel, name, refine, workload="default", timeout_s=150):
+def pmc_traffic(kernel, name, workload, refine, timeout_s):
     """HBM bytes per launch of one kernel from two rocprofv3 counter passes
     (FETCH_SIZE, then WRITE_SIZE) over `bench.py --kernel-only`.  Each counter
-    is scaled by what it reads for a device copy of COPY_BYTES in the same
-    process; the finest-level launches (largest grid) are the ones reported."""
+    is scaled by what it reads for the calibration copy of COPY_BYTES (the
+    first three copy dispatches of the process, before any setup); the
+    finest-level launches (largest grid) are the ones reported."""
     import csv
     import re
     import shutil
@@ -293,7 +330,7 @@ def pmc_traffic(kernel, name, refine, workload="default", timeout_s=150):
         cmd = [exe, "--kernel-include-regex", f"copyBuffer|{name}", "--pmc", counter,
                "--output-format", "csv", "-d", d, "-o", "run", "--",
                sys.executable, os.path.abspath(__file__), "--kernel-only", kernel,
-               "--refine", str(refine), "--workload", workload]
+               "--workload", workload] + (["--refine", str(refine)] if refine else [])
         log(f"counter pass {counter}: {' '.join(cmd)}")
         try:
             r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
@@ -305,13 +342,15 @@ def pmc_traffic(kernel, name, refine, workload="default", timeout_s=150):
             return None, f"{counter} pass wrote no counters (rc {r.returncode})"
         rows = [x for x in csv.DictReader(open(path)) if x["Counter_Name"] == counter]
         shutil.rmtree(d, ignore_errors=True)
-        copy = [float(x["Counter_Value"]) for x in rows if "copyBuffer" in x["Kernel_Name"]]
+        copies = sorted((x for x in rows if "copyBuffer" in x["Kernel_Name"]),
+                        key=lambda x: int(x["Dispatch_Id"]))[:3]
         mine = [x for x in rows if re.search(name, x["Kernel_Name"])]
-        if not copy or not mine:
+        if len(copies) < 3 or not mine:
             return None, f"{counter}: calibration copy or kernel missing from the counters"
         gmax = max(int(x["Grid_Size"]) for x in mine)
         vals = [float(x["Counter_Value"]) for x in mine if int(x["Grid_Size"]) == gmax]
-        scale = COPY_BYTES / (max(copy) * 1024.0)
+        scale = COPY_BYTES / (statistics.median(float(x["Counter_Value"]) for x in copies)
+                              * 1024.0)
         res[counter] = (statistics.median(vals) * 1024.0 * scale, scale, len(vals))
     fetch, fs, nf = res["FETCH_SIZE"]
     write, ws, nw = res["WRITE_SIZE"]
@@ -330,7 +369,8 @@ def kernel_only(a, spec, X, kappa):
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    log(f"calibration copy: {n_copy * 8} bytes read, {n_copy * 8} bytes written per launch")
+    del src, dst
+    log(f"calibration copy: {COPY_BYTES} bytes read, {COPY_BYTES} bytes written per launch")
     g = st.Problem(dict(spec, mg_float=1), vertices=X, kappa=kappa)
     L = g.n_levels - 1
     what = {"vanka": (2, L), "vmult32": (1, L), "vmult64": (0, L)}[a.kernel_only]
@@ -340,22 +380,32 @@ def kernel_only(a, spec, X, kappa):
     g.close()
 
 
+def vmult_flops(dim, p, nt, n_cells):
+    """Flop model of one space-time vmult: per cell and time component
+    8 dim (p+1)^(dim+1) for the sum-factorised interpolation and gradients
+    (in and out) plus (p+1)^dim (2 dim^2 + 2 + 2 nt (dim+1)) at the quadrature
+    points (metric, coefficient, time mixing)."""
+    n = p + 1
+    per = 8 * dim * n ** (dim + 1) + n ** dim * (2 * dim * dim + 2 + 2 * nt * (dim + 1))
+    return float(per) * nt * n_cells
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--refine", type=int, default=5)
-    ap.add_argument("--workload", choices=["default", "large", "wave"], default="default",
-                    help="large: 64^3 cells, Q4 space, dG(3) time (67.9M dofs per slab); "
-                         "wave: 3D wave, Q2/cG(2), two-layer c^2, Gaussian pulse (--refine 5: "
-                         "32^3 cells)")
+    ap.add_argument("--workload", choices=["headline", "q3", "wave"], default="headline",
+                    help="headline: 64^3 cells, Q4 space, dG(3) time (67.9M dofs per slab); "
+                         "q3: 32^3 cells, Q3/dG(2); wave: 3D wave, Q2/cG(2)")
+    ap.add_argument("--refine", type=int, default=None,
+                    help="cells per side 2^refine (headline 6, q3 and wave 5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-pmc", action="store_true",
-                    help="skip the two rocprofv3 counter passes behind roofline.traffic")
-    ap.add_argument("--dense-coarse-dofs", type=int, default=None,
-                    help="largest coarse level collapsed into one dense matrix (0: off)")
+    ap.add_argument("--pmc", choices=["auto", "on", "off"], default="auto",
+                    help="rocprofv3 counter passes behind roofline.traffic (auto: on for "
+                         "workloads whose setup under the profiler takes seconds, off for "
+                         "the 64^3 headline)")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="override a solver spec key (e.g. atomic_scatter=1)")
     ap.add_argument("--kernel-only", choices=["vanka", "vmult32", "vmult64"], default=None)
@@ -369,18 +419,19 @@ def main():
     torch.cuda.synchronize()
     if a.workload == "wave":
         return main_wave(a)
-    spec, X, kappa = workload(a.refine) if a.workload == "default" else workload_large()
-    if a.dense_coarse_dofs is not None:
-        spec["dense_coarse_dofs"] = a.dense_coarse_dofs
+    make = workload_headline if a.workload == "headline" else workload_q3
+    spec, X, kappa, desc = make(a.refine) if a.refine else make()
     for kv in a.set:
         k, v = kv.split("=", 1)
         spec[k] = int(v) if v.lstrip("-").isdigit() else float(v)
     if a.kernel_only:
         return kernel_only(a, spec, X, kappa)
+    period = int(round(spec["end_time"] / (spec["nts"] * spec["tau"])))
 
     # FP32 multigrid inside FP64 FGMRES (headline)
     s32 = dict(spec, mg_float=1)
     g32, setup32, ms32, its32, ok32 = run_gpu(s32, X, kappa, a.steps, a.warmup)
+    g32.spec_nts = spec["nts"]
     nst = g32.n_nodes * g32.nt
     dofs32 = a.steps * nst / (ms32 * 1e-3)
 
@@ -388,6 +439,7 @@ def main():
     L = g32.n_levels - 1
     vm64_ms = g32.time_kernel(0, warmup=3, reps=20)
     vm64_bytes = g32.bytes_model(0)
+    vm64_flops = vmult_flops(3, spec["p"], g32.nt, g32.n_cells)
     vm32_ms = g32.time_kernel(1, L, warmup=3, reps=20)
     nv = (g32.level_info(L)["nc0"] + 1) ** 3
     vm32_bytes = 4.0 * 2 * nst + 4.0 * g32.n_cells + 8.0 * 3 * nv
@@ -397,13 +449,17 @@ def main():
     tp_ms, tp_bytes = g32.time_kernel(4, L, warmup=2, reps=10), g32.bytes_model(2, L)
     tr_ms, tr_bytes = g32.time_kernel(5, L, warmup=2, reps=10), g32.bytes_model(3, L)
     u32 = None if a.no_fp64 else final_state(g32)
+    n_levels = g32.n_levels
     g32.close()
 
-    dofs64, its64, ms64, setup64, ok64, diff = None, None, None, None, None, None
+    # plain FP64: one T = 1 run (at most `period` slabs) after one warm-up slab
+    dofs64, its64, ms64, setup64, ok64, diff, steps64 = None, None, None, None, None, None, 0
     if not a.no_fp64:
+        steps64 = min(a.steps, period)
         s64 = dict(spec, mg_float=0)
-        g64, setup64, ms64, its64, ok64 = run_gpu(s64, X, kappa, a.steps, a.warmup)
-        dofs64 = a.steps * nst / (ms64 * 1e-3)
+        g64, setup64, ms64, its64, ok64 = run_gpu(s64, X, kappa, steps64, min(a.warmup, 1))
+        g64.spec_nts = spec["nts"]
+        dofs64 = steps64 * nst / (ms64 * 1e-3)
         u64 = final_state(g64)
         g64.close()
         diff = ((u32 - u64).norm() / u64.norm()).item()
@@ -421,16 +477,17 @@ def main():
                    symbol=r"k_st_apply<3, \d, \d, double")
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
 
-    traffic, traffic_note = None, "counter passes skipped (--no-pmc)"
-    if a.workload != "default":
-        # a counter pass serialises every dispatch of the setup, which takes
-        # minutes at 68M DoFs
-        traffic_note = "counter passes run for the default workload only"
-    elif not a.no_pmc:
-        traffic, traffic_note = pmc_traffic(dom["only"], dom["symbol"], a.refine, a.workload)
+    pmc = a.pmc == "on" or (a.pmc == "auto" and a.workload != "headline")
+    traffic, traffic_note = None, "counter passes off (--pmc)"
+    if a.pmc == "auto" and a.workload == "headline":
+        traffic_note = ("counter passes not run by default at 64^3: the setup under the "
+                        "profiler takes minutes (bench.py --pmc on runs them)")
+    if pmc:
+        traffic, traffic_note = pmc_traffic(dom["only"], dom["symbol"], a.workload, a.refine,
+                                            timeout_s=300 if a.workload == "headline" else 150)
         log(f"traffic: {traffic if traffic else traffic_note}")
 
-    cpu = None if a.no_cpu else cpu_baseline()
+    cpu = None if a.no_cpu else cpu_baseline(a.workload)
     out = {
         "metric": "space-time DoF/s per solve to 1e-8 (FP32-MG vs FP64) + vmult HBM GB/s",
         "value": dofs32,
@@ -444,14 +501,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 (FGMRES) + f32 (multigrid)",
         "data": "synthetic",
-        "config": {"workload": (f"3D heat, {1 << a.refine}^3 cells perturbed 0.2h, Q3 space, "
-                                f"dG(2) time, log-uniform kappa [1e-2,1e2], Gaussian bumps, "
-                                f"tol 1e-8") if a.workload == "default" else
-                               ("3D heat, 64^3 cells perturbed 0.1h, Q4 space, dG(3) time, "
-                                "log-uniform kappa [1e-3,1e3], Gaussian bumps, tol 1e-8"),
-                   "space_time_dofs_per_slab": nst, "n_levels": g32.n_levels,
-                   "vanka": "factored"},
+        "config": {"workload": desc, "space_time_dofs_per_slab": nst, "n_levels": n_levels,
+                   "slabs_per_run": period, "vanka": "factored",
+                   "secondary_workloads": "bench.py --workload q3 (32^3 Q3/dG(2)), "
+                                          "--workload wave (32^3 Q2/cG(2) wave)"},
         "fp64_dofs_per_s": dofs64,
+        "fp64_steps": steps64,
         "speedup_fp32mg_vs_fp64": (dofs32 / dofs64) if dofs64 else None,
         "fgmres_its_fp32mg": its32,
         "fgmres_its_fp64": its64,
@@ -459,11 +514,14 @@ def main():
         "ms_per_fgmres_iteration": {"fp32mg": ms32 / its32 if its32 else None,
                                     "fp64": (ms64 / its64) if its64 else None},
         "setup_s": {"fp32mg": setup32, "fp64": setup64},
-        # both solves stop at a 1e-8 relative FGMRES residual; the FP64 GPU
-        # path itself matches the oracle to 1e-9 at sizes the oracle solves
+        # both solves stop at a 1e-8 relative FGMRES residual; at sizes the
+        # oracle solves, the GPU slab solves match its FP64 solutions to 1e-12
         "fp32mg_vs_fp64_rel_l2_2slabs": diff,
         "vmult_fp64": {"ms": vm64_ms, "GBps": vm64_bytes / (vm64_ms * 1e-3) / 1e9,
                        "hbm_frac": vm64_bytes / (vm64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "model_flops": vm64_flops,
+                       "TFLOPs": vm64_flops / (vm64_ms * 1e-3) / 1e12,
+                       "fp64_frac": vm64_flops / (vm64_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                        "dofs_per_s": nst / (vm64_ms * 1e-3)},
         "vmult_fp32": {"ms": vm32_ms, "GBps": vm32_bytes / (vm32_ms * 1e-3) / 1e9},
         "vanka_fp32": {"ms": vk_ms, "GBps": vk_bytes / (vk_ms * 1e-3) / 1e9},

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "capi_common.h"
 #include "stfem_host.h"
 
 using namespace stfem;
@@ -19,32 +20,6 @@ struct stfem_problem_s
 
 namespace
 {
-  thread_local std::string g_last_error;
-
-  template <typename F>
-  int guarded(F &&f)
-  {
-    try
-      {
-        f();
-        return STFEM_OK;
-      }
-    catch (const stfem::Error &e)
-      {
-        g_last_error = e.what();
-        return e.code;
-      }
-    catch (const std::exception &e)
-      {
-        g_last_error = e.what();
-        return STFEM_ERR_INTERNAL;
-      }
-    catch (...)
-      {
-        g_last_error = "unknown error";
-        return STFEM_ERR_INTERNAL;
-      }
-  }
   void copy_mat(const DMat &m, double *out)
   {
     if (out)
@@ -105,7 +80,7 @@ namespace
 extern "C"
 {
   int stfem_version(void) { return 100; }
-  const char *stfem_last_error(void) { return g_last_error.c_str(); }
+  const char *stfem_last_error(void) { return last_error_message().c_str(); }
 
   int stfem_fe_time_weights(int type, int r, double tau, int nts, double *alpha, double *beta,
                             double *gamma, double *zeta)
@@ -349,9 +324,9 @@ extern "C"
         for (int i = 0; i < hist_cap && i < int(h.size()); ++i)
           hist[i] = h[i];
       if (rc == STFEM_ERR_NO_CONVERGENCE)
-        g_last_error = "FGMRES: iteration limit reached";
+        last_error_message() = "FGMRES: iteration limit reached";
       else if (rc == STFEM_ERR_NOT_FINITE)
-        g_last_error = "FGMRES: residual is not finite";
+        last_error_message() = "FGMRES: residual is not finite";
     });
     return g != STFEM_OK ? g : rc;
   }
@@ -384,9 +359,9 @@ extern "C"
       rc = pb.solve_slab(x, prev, time, its);
       pb.context().sync();
       if (rc == STFEM_ERR_NO_CONVERGENCE)
-        g_last_error = "FGMRES: iteration limit reached";
+        last_error_message() = "FGMRES: iteration limit reached";
       else if (rc == STFEM_ERR_NOT_FINITE)
-        g_last_error = "FGMRES: residual is not finite";
+        last_error_message() = "FGMRES: residual is not finite";
     });
     return g != STFEM_OK ? g : rc;
   }
@@ -400,9 +375,9 @@ extern "C"
       rc = pb.solve_slab_wave(x, v, prev_u, prev_v, time, its);
       pb.context().sync();
       if (rc == STFEM_ERR_NO_CONVERGENCE)
-        g_last_error = "FGMRES: iteration limit reached";
+        last_error_message() = "FGMRES: iteration limit reached";
       else if (rc == STFEM_ERR_NOT_FINITE)
-        g_last_error = "FGMRES: residual is not finite";
+        last_error_message() = "FGMRES: residual is not finite";
     });
     return g != STFEM_OK ? g : rc;
   }

@@ -1377,7 +1377,8 @@ namespace stfem
   // =================================================================== GMG
   template <typename T>
   GMG<T>::GMG(Context &ctx, std::vector<GMGLevel<T>> &&levels,
-              std::vector<TwoLevelTransfer<T>> &&transfers, const GMGOptions &opt)
+              std::vector<TwoLevelTransfer<T>> &&transfers, const GMGOptions &opt,
+              const std::vector<double> *omega)
     : ctx_(ctx)
     , lv_(std::move(levels))
     , tr_(std::move(transfers))
@@ -1396,8 +1397,9 @@ namespace stfem
         t1_[l].alloc(sz);
         t2_[l].alloc(sz);
       }
+    STFEM_REQUIRE(!omega || int(omega->size()) == nl, "GMG: one relaxation parameter per level");
     for (int l = 0; l < nl; ++l)
-      lv_[l].omega = opt_.estimate_relaxation ? estimate_omega(l) : 1.0;
+      lv_[l].omega = omega ? (*omega)[l] : opt_.estimate_relaxation ? estimate_omega(l) : 1.0;
     setup_dense_coarse();
     // timed phases record events around every level call: no graph then
     use_graph_ = opt_.use_graph == 1 && !ctx_.timers;
@@ -2005,6 +2007,8 @@ namespace stfem
             rhs_  = std::make_unique<SystemMatrix<double>>(ctx_, F, ww.rhs_uK, ww.rhs_uM, tp);
             rhsv_ = std::make_unique<SystemMatrix<double>>(ctx_, F, DMat(ww.rhs_vM.m, 1),
                                                            ww.rhs_vM, tp);
+            rhsv_->set_max_jb(spec.op_jb64);
+            rhsv_->set_split_below(spec.op_split);
             // TimeIntegratorWave velocity matrices (time_integrators.h:219-233)
             const DMat Ai = w1.Alpha.inverse();
             DMat       AixB = Ai * w1.Beta, AixG = Ai * w1.Gamma, AixZ = Ai * w1.Zeta;
@@ -2019,11 +2023,15 @@ namespace stfem
         else
           {
             fine_ = std::make_unique<SystemMatrix<double>>(ctx_, F, w.Alpha, w.Beta, tp);
-            fine_->set_max_jb(spec.op_jb64);
-            fine_->set_split_below(spec.op_split);
             const DMat rK = (spec.type == TimeStepType::CGP) ? w.Gamma : zero;
             const DMat rM = (spec.type == TimeStepType::CGP) ? w.Zeta : w.Gamma;
             rhs_ = std::make_unique<SystemMatrix<double>>(ctx_, F, rK, rM, tp);
+          }
+        // every FP64 operator with the same output chunking
+        for (auto *op : {fine_.get(), rhs_.get()})
+          {
+            op->set_max_jb(spec.op_jb64);
+            op->set_split_below(spec.op_split);
           }
         alpha1_ = w1.Alpha;
         gamma1_ = w1.Gamma;
@@ -2744,4 +2752,11 @@ namespace stfem
   template class MGTwoLevelBlockTransfer<float>;
   template class GMG<double>;
   template class GMG<float>;
+  // the solver loop for every operator/preconditioner pair of the C ABI
+  template int SolverFGMRES::solve(const SystemMatrix<double> &, const GMG<double> &, double *,
+                                   const double *);
+  template int SolverFGMRES::solve(const SystemMatrix<double> &, const GMG<float> &, double *,
+                                   const double *);
+  template int SolverFGMRES::solve(const SystemMatrix<double> &, const IdentityPreconditioner &,
+                                   double *, const double *);
 } // namespace stfem

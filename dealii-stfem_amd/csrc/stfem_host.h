@@ -296,6 +296,8 @@ namespace stfem
     bool   fused_residual_ok() const;
     void   relax_opc(T *u, const T *rhs, const T *opc, T omega) const;
     long   n_cells() const { return grid_.n_cells; }
+    long   n_nodes() const { return grid_.n_nodes; }
+    int    n_blocks() const { return nt_; }
     Form   form() const { return form_; }
     // factored sweep kernel (0 automatic, 2 LDS-staged, 3 register-tiled)
     void   set_variant(int v) { variant_ = v; }
@@ -368,8 +370,8 @@ namespace stfem
   template <typename T>
   struct TwoLevelTransfer
   {
-    std::unique_ptr<MGTwoLevelTransferST<T>>    time;
-    std::unique_ptr<MGTwoLevelBlockTransfer<T>> space;
+    std::shared_ptr<MGTwoLevelTransferST<T>>    time;
+    std::shared_ptr<MGTwoLevelBlockTransfer<T>> space;
     void prolongate_and_add(T *dst, const T *src) const
     {
       time ? time->prolongate_and_add(dst, src) : space->prolongate_and_add(dst, src);
@@ -387,8 +389,8 @@ namespace stfem
   template <typename T>
   struct GMGLevel
   {
-    std::unique_ptr<SystemMatrix<T>>      A;
-    std::unique_ptr<PreconditionVanka<T>> V;
+    std::shared_ptr<SystemMatrix<T>>      A;
+    std::shared_ptr<PreconditionVanka<T>> V;
     double                                omega = 1.0;
     int                                   r = 0, nts = 1;
     TimeMGType                            type_to_here = TimeMGType::none;
@@ -402,8 +404,11 @@ namespace stfem
   class GMG
   {
   public:
+    // omega: relaxation parameter per level (nullptr: estimated as
+    // GMG::estimate_relaxation does, or 1 without estimate_relaxation)
     GMG(Context &ctx, std::vector<GMGLevel<T>> &&levels,
-        std::vector<TwoLevelTransfer<T>> &&transfers, const GMGOptions &opt);
+        std::vector<TwoLevelTransfer<T>> &&transfers, const GMGOptions &opt,
+        const std::vector<double> *omega = nullptr);
     ~GMG();
     void   vmult(double *dst, const double *src) const;
     // one V-cycle on level vectors (def[L] given, sol[L] returned)
@@ -442,6 +447,18 @@ namespace stfem
   };
 
   // ------------------------------------------------------------ FGMRES
+  // The identity as FGMRES preconditioner (the solver without multigrid).
+  struct IdentityPreconditioner
+  {
+    Context *ctx;
+    long     n;
+    void     vmult(double *dst, const double *src) const
+    {
+      STFEM_HIP(hipMemcpyAsync(dst, src, size_t(n) * sizeof(double), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    }
+  };
+
   // Right-preconditioned flexible GMRES (deal.II SolverFGMRES semantics:
   // modified Gram-Schmidt, projected least-squares solve once two basis
   // vectors exist, ReductionControl on the projected residual).

@@ -257,6 +257,119 @@ int stfem_phase_timings(stfem_problem p, int reset, double *ms, long *calls);
  * stops it. */
 int stfem_time_slabs(stfem_problem p, int n_slabs, double *ms, int *its);
 
+/* ==== Object-level boundary =========================================
+ * The reference's objects as separate handles, composed by the caller the
+ * way tests/tp_01.cc:200-359 composes them: spaces (one DoFHandler +
+ * coefficient per level), space-time operators (SystemMatrix), smoothers
+ * (PreconditionVanka), two-level transfers, the GMG preconditioner and the
+ * FGMRES solver.  Handles are owned by the caller (each has a _destroy) and
+ * keep what they use alive (a GMG holds its levels and transfers; every
+ * handle holds its context), so they may be destroyed in any order.  Device
+ * vectors are node-major (x[node * nt + t]) in the object's precision
+ * (precision 4: float, 8: double); calls are synchronous like the problem
+ * API above; errors are reported through stfem_last_error(). */
+typedef struct stfem_context_s  *stfem_context;
+typedef struct stfem_space_s    *stfem_space;
+typedef struct stfem_stop_s     *stfem_stop;
+typedef struct stfem_vanka_s    *stfem_vanka;
+typedef struct stfem_transfer_s *stfem_transfer;
+typedef struct stfem_gmg_s      *stfem_gmg;
+typedef struct stfem_fgmres_s   *stfem_fgmres;
+
+/* device and stream (one MPI rank of the reference, tp_01.cc:740-750) */
+int stfem_context_create(int device, stfem_context *out);
+int stfem_context_destroy(stfem_context c);
+
+/* The level spaces of tp_01.cc:83-90 and :262-275 (subdivided box refined
+ * globally, FE_Q(p) on Gauss-Lobatto nodes, zero Dirichlet faces): one
+ * space per refinement level 0..refine (coarsest first), built from the
+ * mesh keys of a spec (dim, subdiv, lo, hi, refine, p, dirichlet,
+ * coarse_coef) with the finest-level vertices and per-cell coefficient as
+ * in stfem_problem_create; coarser levels evaluate the coefficient at their
+ * quadrature points (MatrixFreeOperator::evaluate_coefficient,
+ * operators.h:435-462).  cap: length of `spaces`; *n: spaces written. */
+int stfem_space_hierarchy_create(stfem_context c, const char *spec, const double *vertices,
+                                 const double *cell_coef, int cap, stfem_space *spaces, int *n);
+int stfem_space_destroy(stfem_space s);
+/* info[0..7] = dim, p, cells x, y, z, n_nodes, n_cells, Dirichlet face mask */
+int stfem_space_info(stfem_space s, long *info);
+/* host mask (n_nodes bytes): 1 for nodes constrained to zero (the
+ * AffineConstraints of DoFTools::make_zero_boundary_constraints, tp_01.cc:92) */
+int stfem_space_constrained(stfem_space s, unsigned char *mask);
+
+/* SystemMatrix(timer, K, M, Alpha, Beta), operators.h:78-93: the space-time
+ * operator Alpha (x) K + Beta (x) M with K the space's stiffness (with its
+ * coefficient) and M its mass matrix, applied matrix-free
+ * (MatrixFreeOperator, operators.h:355-562).  Alpha, Beta: row-major
+ * nt_rows x nt_cols; nt_cols = 1 gives the rhs operator of
+ * operators.h:154-187. */
+int stfem_stop_create(stfem_space s, int nt_rows, int nt_cols, const double *alpha,
+                      const double *beta, int precision, stfem_stop *out);
+int stfem_stop_destroy(stfem_stop op);
+/* info[0..3] = n_nodes, nt_rows, nt_cols, precision */
+int stfem_stop_info(stfem_stop op, long *info);
+/* SystemMatrix::vmult / Tvmult, operators.h:94-152 (square time matrices
+ * for Tvmult) */
+int stfem_stop_vmult(stfem_stop op, void *dst, const void *src);
+int stfem_stop_tvmult(stfem_stop op, void *dst, const void *src);
+/* SystemMatrix::vmult_add(BlockVector&, const Vector&), operators.h:154-187:
+ * dst (nt_rows values per node) += (Alpha(:,0) (x) K + Beta(:,0) (x) M) src
+ * (one value per node); needs nt_cols = 1 */
+int stfem_stop_vmult_add(stfem_stop op, void *dst, const void *src);
+
+/* PreconditionVanka(timer, K, M, SP, Alpha, Beta, dof_handler),
+ * gmg.h:409-474: cell patches of the space, patch blocks of the assembled K
+ * and M; form 0: explicit inverses (gauss_jordan, gmg.h:470), 1: the same
+ * inverse factored through the generalized eigenproblem of (K_c, M_c);
+ * device_setup as the spec key vanka_device_setup (3 default). */
+int stfem_vanka_create(stfem_space s, int nt, const double *alpha, const double *beta,
+                       int precision, int form, int device_setup, stfem_vanka *out);
+int stfem_vanka_destroy(stfem_vanka v);
+/* PreconditionVanka::vmult, gmg.h:476-514 */
+int stfem_vanka_apply(stfem_vanka v, void *dst, const void *src);
+
+/* MGTwoLevelTransferST(type, r, n_timesteps_at_once, restrict_is_transpose,
+ * mg_type), gmg.h:86-115: kind 1 tau (between nts and nts/2 steps of
+ * degree r), 2 k (between degree r and r - 1); on the nodes of space s. */
+int stfem_time_transfer_create(stfem_space s, int type, int r, int nts, int kind,
+                               int restrict_is_transpose, int precision, stfem_transfer *out);
+/* MGTwoLevelBlockTransfer (gmg.h:35-75) over deal.II's MGTwoLevelTransfer
+ * between two spaces one refinement apart (gmg.h:371-380), nt blocks */
+int stfem_space_transfer_create(stfem_space fine, stfem_space coarse, int nt, int precision,
+                                stfem_transfer *out);
+int stfem_transfer_destroy(stfem_transfer t);
+/* prolongate_and_add / restrict_and_add, gmg.h:117-135 and :56-74 */
+int stfem_transfer_prolongate_and_add(stfem_transfer t, void *dst, const void *src);
+int stfem_transfer_restrict_and_add(stfem_transfer t, void *dst, const void *src);
+
+/* GMG(..., mg_operators, mg_smoother (Vanka), transfers, ...),
+ * gmg.h:666-790: levels 0 (coarsest) .. n_levels-1, transfers[l] between
+ * l-1 and l (transfers[0] unused), all in one precision.  options: spec keys
+ * smoothing_steps, variable, estimate_relaxation, eig_iters, eig_safety,
+ * exact_single_cell, use_graph, dense_coarse_dofs, fuse_residual; omega:
+ * one relaxation parameter per level or NULL (estimated as
+ * GMG::estimate_relaxation does, gmg.h:792-844). */
+int stfem_gmg_create(int n_levels, const stfem_stop *ops, const stfem_vanka *smoothers,
+                     const stfem_transfer *transfers, const char *options, const double *omega,
+                     stfem_gmg *out);
+int stfem_gmg_destroy(stfem_gmg g);
+int stfem_gmg_omega(stfem_gmg g, int level, double *omega);
+/* GMG::vmult, gmg.h:846-859: FP64 in and out, the levels in their precision */
+int stfem_gmg_apply(stfem_gmg g, double *dst, const double *src);
+
+/* SolverFGMRES with ReductionControl(maxiter, abstol, reltol) and a basis
+ * of `basis` vectors (time_integrators.h:44, :113) for systems of n FP64
+ * unknowns */
+int stfem_fgmres_create(stfem_context c, long n, int maxiter, double abstol, double reltol,
+                        int basis, int orthogonalization, stfem_fgmres *out);
+int stfem_fgmres_destroy(stfem_fgmres s);
+/* SolverFGMRES::solve(A, x, b, P), time_integrators.h:174: A an FP64
+ * operator, P a GMG or NULL (no preconditioner); x holds the start vector.
+ * Returns 0, STFEM_ERR_NO_CONVERGENCE or STFEM_ERR_NOT_FINITE; its and hist
+ * (residual history, hist_cap entries max) as stfem_fgmres_solve. */
+int stfem_fgmres_run(stfem_fgmres s, stfem_stop A, stfem_gmg P, double *x, const double *b,
+                     int *its, double *hist, int hist_cap);
+
 #ifdef __cplusplus
 }
 #endif

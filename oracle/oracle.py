@@ -75,6 +75,13 @@ def _p(a):
     return a.ctypes.data_as(_dp)
 
 
+def num_threads():
+    """Threads of the oracle's OpenMP loops (spmv, Vanka setup and apply)."""
+    L = load()
+    L.oracle_num_threads.restype = ctypes.c_int
+    return L.oracle_num_threads()
+
+
 def nt_intvl(type_, r):
     return r if type_ == CGP else r + 1
 

@@ -57,6 +57,10 @@
 #include <string>
 #include <vector>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 namespace orc
 {
   enum TimeType
@@ -1072,6 +1076,8 @@ namespace orc
   static void spmv_free(const CSR &A, const std::vector<T> &vals, const std::vector<char> &cons,
                         const T *x, T *y)
   {
+    // rows are independent: threads change nothing in the arithmetic
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < A.n; ++i)
       {
         if (cons[i])
@@ -1196,14 +1202,24 @@ namespace orc
       const int  nl = sp.nc_local;
       const long nc = sp.g.n_cells();
       std::vector<double> val(N(), 0.0);
-      std::vector<long>   idx;
       Binv.assign(nc, {});
       const int nb = nt * nl;
+      {
+        std::vector<long> idx;
+        for (long c = 0; c < nc; ++c)
+          {
+            sp.cell_dofs(c, idx);
+            for (long g : idx)
+              val[g] += 1.0;
+          }
+      }
+      // the cells' blocks are independent (one thread per cell)
+      bool singular = false;
+#pragma omp parallel for schedule(dynamic)
       for (long c = 0; c < nc; ++c)
         {
+          std::vector<long> idx;
           sp.cell_dofs(c, idx);
-          for (long g : idx)
-            val[g] += 1.0;
           std::vector<double> Kc(size_t(nl) * nl, 0.0), Mc(size_t(nl) * nl, 0.0);
           for (int i = 0; i < nl; ++i)
             {
@@ -1235,8 +1251,8 @@ namespace orc
                           At[i * nt + j] * T(Kc[size_t(k) * nl + l]);
               if (!invert_inplace(B, nb))
                 {
-                  fprintf(stderr, "oracle: singular Vanka block\n");
-                  abort();
+#pragma omp atomic write
+                  singular = true;
                 }
               Binv[c] = B;
             }
@@ -1252,11 +1268,16 @@ namespace orc
                           Beta(i, j) * Mc[size_t(k) * nl + l] + Alpha(i, j) * Kc[size_t(k) * nl + l];
               if (!invert_inplace(B, nb))
                 {
-                  fprintf(stderr, "oracle: singular Vanka block\n");
-                  abort();
+#pragma omp atomic write
+                  singular = true;
                 }
               Binv[c].assign(B.begin(), B.end());
             }
+        }
+      if (singular)
+        {
+          fprintf(stderr, "oracle: singular Vanka block\n");
+          abort();
         }
       inv_valence.resize(N());
       for (long i = 0; i < N(); ++i)
@@ -1268,23 +1289,38 @@ namespace orc
     {
       const int  nl = sp.nc_local, nb = nt * nl;
       const long n  = N();
+      const long nc = sp.g.n_cells();
       std::fill(dst, dst + n * nt, T(0));
+      // local solves in parallel, scatter in cell order (the same sums as a
+      // serial loop whatever the thread count)
+      std::vector<T> Y(size_t(nc) * nb);
+#pragma omp parallel
+      {
+        std::vector<long> idx;
+        std::vector<T>    xl(nb);
+#pragma omp for schedule(static)
+        for (long c = 0; c < nc; ++c)
+          {
+            sp.cell_dofs(c, idx);
+            for (int b = 0, k = 0; b < nt; ++b)
+              for (int j = 0; j < nl; ++j, ++k)
+                xl[k] = src[b * n + idx[j]];
+            const auto &B  = Binv[c];
+            T          *yl = Y.data() + size_t(c) * nb;
+            for (int i = 0; i < nb; ++i)
+              {
+                T s = T(0);
+                for (int j = 0; j < nb; ++j)
+                  s += B[size_t(i) * nb + j] * xl[j];
+                yl[i] = s;
+              }
+          }
+      }
       std::vector<long> idx;
-      std::vector<T>    xl(nb), yl(nb);
-      for (long c = 0; c < sp.g.n_cells(); ++c)
+      for (long c = 0; c < nc; ++c)
         {
           sp.cell_dofs(c, idx);
-          for (int b = 0, k = 0; b < nt; ++b)
-            for (int j = 0; j < nl; ++j, ++k)
-              xl[k] = src[b * n + idx[j]];
-          const auto &B = Binv[c];
-          for (int i = 0; i < nb; ++i)
-            {
-              T s = T(0);
-              for (int j = 0; j < nb; ++j)
-                s += B[size_t(i) * nb + j] * xl[j];
-              yl[i] = s;
-            }
+          const T *yl = Y.data() + size_t(c) * nb;
           for (int b = 0, k = 0; b < nt; ++b)
             for (int j = 0; j < nl; ++j, ++k)
               dst[b * n + idx[j]] += inv_valence[idx[j]] * yl[k];
@@ -2651,6 +2687,15 @@ extern "C"
     ORC_DISPATCH(h, P.gmg_vmult(dst, src));
   }
   // assembled fine K or M (CSR) export: returns nnz; arrays may be null
+  // threads the oracle's parallel loops use (OpenMP)
+  int oracle_num_threads(void)
+  {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+  }
   // the load vector int f(t, x) phi_i dx of the fine level (Dirichlet
   // entries zero), the spatial part of assemble_force (time_integrators.h:60-87)
   void oracle_load_vector(void *h, double time, double *out)

@@ -124,10 +124,25 @@ def test_bump_source_q3_dg2(mg_float):
 def test_heat_without_extrapolation():
     spec = dict(dim=2, refine=3, p=2, type=2, r=1, nts=2, r_min=0, extrapolate=0)
     check(run_slabs(spec, 3), fp64=True)
-    # and the start vector does matter: with extrapolation the iterations differ
-    a = run_slabs(dict(spec, extrapolate=1), 3)
-    b = run_slabs(spec, 3)
-    assert [r[2] for r in a] != [r[2] for r in b]
+    # the start vector matters once the solve is cut short: two iterations
+    # from a nonzero previous slab, with and without extrapolation, differ,
+    # and each matches the oracle's (which also stops at the limit)
+    rng = np.random.default_rng(6)
+    outs = []
+    for ex in (0, 1):
+        sp2 = dict(spec, extrapolate=ex, maxiter=2)
+        o, g = orc.Problem(sp2), st.Problem(sp2)
+        n, nt = g.n_nodes, g.nt
+        prev = rng.standard_normal(n) if ex == 0 else prev
+        rc_o, xo, its_o = orc.solve_slab(o, np.zeros(n * nt), prev, 0.0)
+        xg = torch.zeros(n * nt, dtype=torch.float64, device=DEV)
+        print(f"[extrapolate={ex}, maxiter=2]", flush=True)
+        rc_g, its_g = g.solve_slab(xg, torch.tensor(prev, device=DEV), 0.0)
+        assert rc_o != 0 and rc_g == -3 and its_g == its_o == 2, (rc_o, rc_g, its_g, its_o)
+        assert rel_err(nodes_to_blocks(xg.cpu().numpy(), nt), xo) < 1e-12
+        outs.append(xo)
+        g.close()
+    assert rel_err(outs[0], outs[1]) > 1e-6
 
 
 def test_wave_without_extrapolation():
@@ -239,12 +254,16 @@ def test_reference_parameter_file_drives_the_heat_run():
     text = open(os.path.join(HERE, "golden", "json", "tf01.json")).read()
     ref = json.loads(text)
     a = st.Problem(text)
-    b = st.Problem(dict(type=ref["timeType"], r=int(ref["feDegree"]),
-                        nts=int(ref["nTimestepsAtOnce"]), refine=int(ref["refinement"]),
-                        problem=ref["problemType"], dim=2))
+    b_spec = dict(type=ref["timeType"], r=int(ref["feDegree"]),
+                  nts=int(ref["nTimestepsAtOnce"]), refine=int(ref["refinement"]),
+                  problem=ref["problemType"], dim=2, frequency=ref["frequency"])
+    b = st.Problem(b_spec)
     print("[tf01.json heat run]", flush=True)
+    assert st.spec_normalize(text) == st.spec_normalize(b_spec)
     ra, rb = a.run_heat(), b.run_heat()
-    assert ra["rc"] == 0 and ra == rb
+    assert ra["rc"] == 0 and ra["its"] == rb["its"] and ra["slabs"] == rb["slabs"]
+    for key in ("L2", "Linf", "H1"):
+        assert abs(ra[key] - rb[key]) <= 1e-12 * abs(rb[key]), key
     a.close()
     b.close()
 
@@ -260,7 +279,9 @@ def test_phase_timers():
     rc, its = g.solve_slab(x, prev, 0.0)
     assert rc == 0
     t = g.phase_timings(reset=True)
-    assert t["step"][1] == 1 and t["gmg"][1] == its
+    # one preconditioner application per FGMRES step (the last one is not
+    # counted as an iteration when it meets the tolerance)
+    assert t["step"][1] == 1 and its <= t["gmg"][1] <= its + 1, (t, its)
     assert t["vmult"][1] > its and t["vanka"][1] > its
     assert 0 < t["gmg"][0] <= t["step"][0]
     assert g.phase_timings()["step"] == (0.0, 0)
