@@ -454,7 +454,7 @@ extern "C"
       const std::string o = options ? options : "";
       const ProblemSpec ps = ProblemSpec::parse(o);
       {
-        const std::string def = ProblemSpec().to_string(), got = ps.to_string();
+        const std::string def = ProblemSpec::parse("").to_string(), got = ps.to_string();
         // only the multigrid keys may differ from the defaults
         const char *mg_keys[] = {"smoothing_steps", "variable",          "estimate_relaxation",
                                  "eig_iters",       "eig_safety",        "exact_single_cell",

@@ -71,6 +71,42 @@ _SIGS = {
     "stfem_time_slabs": ([_vp, ctypes.c_int, _dp, _ip], ctypes.c_int),
     "stfem_spec_normalize": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int], ctypes.c_int),
     "stfem_phase_timings": ([_vp, ctypes.c_int, _dp, _lp], ctypes.c_int),
+    # object-level boundary
+    "stfem_context_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "stfem_context_destroy": ([_vp], ctypes.c_int),
+    "stfem_space_hierarchy_create": ([_vp, ctypes.c_char_p, _dp, _dp, ctypes.c_int,
+                                      ctypes.POINTER(_vp), _ip], ctypes.c_int),
+    "stfem_space_destroy": ([_vp], ctypes.c_int),
+    "stfem_space_info": ([_vp, _lp], ctypes.c_int),
+    "stfem_space_constrained": ([_vp, ctypes.c_void_p], ctypes.c_int),
+    "stfem_stop_create": ([_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
+                           ctypes.POINTER(_vp)], ctypes.c_int),
+    "stfem_stop_destroy": ([_vp], ctypes.c_int),
+    "stfem_stop_info": ([_vp, _lp], ctypes.c_int),
+    "stfem_stop_vmult": ([_vp, _vp, _vp], ctypes.c_int),
+    "stfem_stop_tvmult": ([_vp, _vp, _vp], ctypes.c_int),
+    "stfem_stop_vmult_add": ([_vp, _vp, _vp], ctypes.c_int),
+    "stfem_vanka_create": ([_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "stfem_vanka_destroy": ([_vp], ctypes.c_int),
+    "stfem_vanka_apply": ([_vp, _vp, _vp], ctypes.c_int),
+    "stfem_time_transfer_create": ([_vp] + [ctypes.c_int] * 6 + [ctypes.POINTER(_vp)],
+                                   ctypes.c_int),
+    "stfem_space_transfer_create": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)],
+                                    ctypes.c_int),
+    "stfem_transfer_destroy": ([_vp], ctypes.c_int),
+    "stfem_transfer_prolongate_and_add": ([_vp, _vp, _vp], ctypes.c_int),
+    "stfem_transfer_restrict_and_add": ([_vp, _vp, _vp], ctypes.c_int),
+    "stfem_gmg_create": ([ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                          ctypes.POINTER(_vp), ctypes.c_char_p, _dp, ctypes.POINTER(_vp)],
+                         ctypes.c_int),
+    "stfem_gmg_destroy": ([_vp], ctypes.c_int),
+    "stfem_gmg_omega": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "stfem_gmg_apply": ([_vp, _vp, _vp], ctypes.c_int),
+    "stfem_fgmres_create": ([_vp, ctypes.c_long, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "stfem_fgmres_destroy": ([_vp], ctypes.c_int),
+    "stfem_fgmres_run": ([_vp, _vp, _vp, _vp, _vp, _ip, _dp, ctypes.c_int], ctypes.c_int),
 }
 
 
@@ -365,3 +401,164 @@ class Problem:
             return ms.value, its.value, False
         _check(rc)
         return (ms.value, its.value, True) if allow_unconverged else (ms.value, its.value)
+
+
+# ------------------------------------------------------------ object boundary
+class _Handle:
+    """Owner of one library handle; destroyed with the Python object."""
+    _destroy = None
+
+    def __init__(self, h, keep=()):
+        self.h = h
+        self._keep = keep  # Python-side references only; the library keeps its own
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            getattr(load(), self._destroy)(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _new(fn, *args):
+    h = ctypes.c_void_p()
+    _check(getattr(load(), fn)(*args, ctypes.byref(h)))
+    return h
+
+
+def _mat(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+class Context(_Handle):
+    """Device and stream shared by the objects built on it."""
+    _destroy = "stfem_context_destroy"
+
+    def __init__(self, device=0):
+        super().__init__(_new("stfem_context_create", device))
+
+
+class Space(_Handle):
+    """One level's DoFHandler: structured mesh, FE_Q(p), coefficient."""
+    _destroy = "stfem_space_destroy"
+
+    @staticmethod
+    def hierarchy(ctx, spec, vertices=None, kappa=None):
+        text = spec if isinstance(spec, str) else spec_string(spec)
+        v = None if vertices is None else np.ascontiguousarray(vertices, np.float64)
+        k = None if kappa is None else np.ascontiguousarray(kappa, np.float64)
+        arr = (ctypes.c_void_p * 16)()
+        n = ctypes.c_int()
+        _check(load().stfem_space_hierarchy_create(ctx.h, text.encode(), _p(v), _p(k), 16, arr,
+                                                   ctypes.byref(n)))
+        return [Space(ctypes.c_void_p(arr[i])) for i in range(n.value)]
+
+    def info(self):
+        i = np.zeros(8, np.int64)
+        _check(load().stfem_space_info(self.h, i.ctypes.data_as(_lp)))
+        keys = ["dim", "p", "nc0", "nc1", "nc2", "n_nodes", "n_cells", "dirichlet"]
+        return dict(zip(keys, [int(x) for x in i]))
+
+    def constrained(self):
+        m = np.zeros(self.info()["n_nodes"], np.uint8)
+        _check(load().stfem_space_constrained(self.h, m.ctypes.data_as(ctypes.c_void_p)))
+        return m.astype(bool)
+
+
+class SystemMatrix(_Handle):
+    """Alpha (x) K + Beta (x) M on a space (operators.h:71-256)."""
+    _destroy = "stfem_stop_destroy"
+
+    def __init__(self, space, alpha, beta, precision=8):
+        a, b = _mat(alpha), _mat(beta)
+        super().__init__(_new("stfem_stop_create", space.h, a.shape[0], a.shape[1], _p(a), _p(b),
+                              precision), keep=(space,))
+
+    def vmult(self, dst, src):
+        _check(load().stfem_stop_vmult(self.h, _ptr(dst), _ptr(src)))
+
+    def Tvmult(self, dst, src):
+        _check(load().stfem_stop_tvmult(self.h, _ptr(dst), _ptr(src)))
+
+    def vmult_add(self, dst, src):
+        _check(load().stfem_stop_vmult_add(self.h, _ptr(dst), _ptr(src)))
+
+
+class PreconditionVanka(_Handle):
+    """Cell-patch Vanka smoother (gmg.h:402-540)."""
+    _destroy = "stfem_vanka_destroy"
+
+    def __init__(self, space, alpha, beta, precision=8, factored=True, device_setup=3):
+        a, b = _mat(alpha), _mat(beta)
+        super().__init__(_new("stfem_vanka_create", space.h, a.shape[0], _p(a), _p(b), precision,
+                              int(factored), device_setup), keep=(space,))
+
+    def vmult(self, dst, src):
+        _check(load().stfem_vanka_apply(self.h, _ptr(dst), _ptr(src)))
+
+
+class Transfer(_Handle):
+    """Two-level transfer in time (MGTwoLevelTransferST) or space
+    (MGTwoLevelBlockTransfer)."""
+    _destroy = "stfem_transfer_destroy"
+
+    @staticmethod
+    def time(space, type_, r, nts, kind, restrict_is_transpose=True, precision=8):
+        return Transfer(_new("stfem_time_transfer_create", space.h, type_, r, nts, kind,
+                             int(restrict_is_transpose), precision), keep=(space,))
+
+    @staticmethod
+    def space(fine, coarse, nt, precision=8):
+        return Transfer(_new("stfem_space_transfer_create", fine.h, coarse.h, nt, precision),
+                        keep=(fine, coarse))
+
+    def prolongate_and_add(self, dst, src):
+        _check(load().stfem_transfer_prolongate_and_add(self.h, _ptr(dst), _ptr(src)))
+
+    def restrict_and_add(self, dst, src):
+        _check(load().stfem_transfer_restrict_and_add(self.h, _ptr(dst), _ptr(src)))
+
+
+class GMG(_Handle):
+    """Space-time multigrid preconditioner (gmg.h:652-906)."""
+    _destroy = "stfem_gmg_destroy"
+
+    def __init__(self, ops, smoothers, transfers, options="", omega=None):
+        n = len(ops)
+        arr = lambda objs: (ctypes.c_void_p * n)(*[o.h.value if o is not None else None
+                                                   for o in objs])
+        tr = [None] + list(transfers[1:]) if len(transfers) == n else [None] + list(transfers)
+        om = None if omega is None else np.ascontiguousarray(omega, np.float64)
+        text = options if isinstance(options, str) else spec_string(options)
+        super().__init__(_new("stfem_gmg_create", n, arr(ops), arr(smoothers), arr(tr),
+                              text.encode(), _p(om)), keep=(ops, smoothers, transfers))
+        self.n_levels = n
+
+    def omega(self, l):
+        w = ctypes.c_double()
+        _check(load().stfem_gmg_omega(self.h, l, ctypes.byref(w)))
+        return w.value
+
+    def vmult(self, dst, src):
+        _check(load().stfem_gmg_apply(self.h, _ptr(dst), _ptr(src)))
+
+
+class SolverFGMRES(_Handle):
+    """SolverFGMRES with ReductionControl (time_integrators.h:44, :113)."""
+    _destroy = "stfem_fgmres_destroy"
+
+    def __init__(self, ctx, n, maxiter=200, abstol=1e-12, reltol=1e-12, basis=30,
+                 orthogonalization=1):
+        super().__init__(_new("stfem_fgmres_create", ctx.h, int(n), maxiter, abstol, reltol,
+                              basis, orthogonalization), keep=(ctx,))
+
+    def solve(self, A, x, b, P=None, hist_cap=256):
+        """(status, iterations, residual history); x holds the start vector."""
+        its = ctypes.c_int()
+        hist = np.zeros(hist_cap)
+        rc = load().stfem_fgmres_run(self.h, A.h, P.h if P is not None else None, _ptr(x),
+                                     _ptr(b), ctypes.byref(its), _p(hist), hist_cap)
+        if rc not in (0, -3, -5):  # argument errors raise; solver outcomes are returned
+            _check(rc)
+        return rc, its.value, hist[: its.value + 1]
