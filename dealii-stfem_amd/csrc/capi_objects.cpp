@@ -276,6 +276,19 @@ extern "C"
       ctx.sync();
     });
   }
+  int stfem_stop_vmult_noalloc(stfem_stop op, void *dst, const void *src)
+  {
+    return guarded([&] {
+      STFEM_REQUIRE(op && dst && src, "null argument");
+      Context &ctx = C(op->ctx);
+      barrier(ctx);
+      if (op->precision == 4)
+        vmult_noalloc(*op->f, ctx.device, (float *)dst, (const float *)src);
+      else
+        vmult_noalloc(*op->d, ctx.device, (double *)dst, (const double *)src);
+      ctx.sync();
+    });
+  }
   int stfem_stop_tvmult(stfem_stop op, void *dst, const void *src)
   {
     return guarded([&] {

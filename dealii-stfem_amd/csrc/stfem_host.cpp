@@ -598,7 +598,7 @@ namespace stfem
 
   template <typename T>
   void SystemMatrix<T>::apply(T *dst, const T *src, const DBuf<T> &a, const DBuf<T> &b, int mode,
-                              const T *rhs) const
+                              const T *rhs, T *cells) const
   {
     const PhaseScope scope(ctx_, PhaseTimers::vmult);
     StOpArgs<T> args;
@@ -613,7 +613,13 @@ namespace stfem
     args.nt_out     = nt_out_;
     args.max_jb     = max_jb_ > 0 ? max_jb_ : (sizeof(T) == 8 && grid_.p >= 4 ? 2 : 4);
     args.split_below = split_below_;
-    launch_st_apply<T>(args, src, dst, cells_.get(), mode, rhs, ctx_.stream);
+    launch_st_apply<T>(args, src, dst, cells ? cells : cells_.get(), mode, rhs, ctx_.stream);
+  }
+  template <typename T>
+  void SystemMatrix<T>::vmult_with_cells(T *dst, const T *src, T *cells) const
+  {
+    STFEM_REQUIRE(cells_.get() && cells, "vmult_with_cells: needs the two-pass operator");
+    apply(dst, src, dA_, dB_, 0, nullptr, cells);
   }
   template <typename T>
   void SystemMatrix<T>::residual(T *dst, const T *rhs, const T *src) const

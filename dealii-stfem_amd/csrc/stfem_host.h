@@ -225,6 +225,10 @@ namespace stfem
                  bool two_pass = true);
     // dst = (Alpha x K + Beta x M) src   (operators.h:94-122)
     void vmult(T *dst, const T *src) const;
+    // the same with the cell-major results in `cells` (at least
+    // cell_buffer_bytes()) instead of the operator's own buffer
+    void vmult_with_cells(T *dst, const T *src, T *cells) const;
+    size_t cell_buffer_bytes() const { return st_cells_size(grid_, nt_out_) * sizeof(T); }
     // the cell pass of vmult alone, into cell_buffer() (two-pass form)
     void vmult_cells(const T *src) const;
     const T *cell_buffer() const { return cells_.get(); }
@@ -255,7 +259,7 @@ namespace stfem
 
   private:
     void apply(T *dst, const T *src, const DBuf<T> &a, const DBuf<T> &b, int mode,
-               const T *rhs = nullptr) const;
+               const T *rhs = nullptr, T *cells = nullptr) const;
     Context     &ctx_;
     GridDesc     grid_;
     DBuf<double> verts_;
@@ -531,6 +535,12 @@ namespace stfem
     virtual double fine_vmult_bytes() const = 0;
     virtual double vanka_sweep_bytes(int l) const = 0;
   };
+
+  // dst = op src with the operator's cell-major results in a fixed device
+  // array of the library (no allocation); throws if they do not fit
+  template <typename T>
+  void   vmult_noalloc(const SystemMatrix<T> &op, int device, T *dst, const T *src);
+  size_t noalloc_scratch_bytes();
 
   std::unique_ptr<ProblemBase> make_problem(const std::string &spec, const double *vertices,
                                             const double *kappa, int device);

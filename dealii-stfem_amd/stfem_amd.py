@@ -85,6 +85,7 @@ _SIGS = {
     "stfem_stop_info": ([_vp, _lp], ctypes.c_int),
     "stfem_stop_vmult": ([_vp, _vp, _vp], ctypes.c_int),
     "stfem_stop_tvmult": ([_vp, _vp, _vp], ctypes.c_int),
+    "stfem_stop_vmult_noalloc": ([_vp, _vp, _vp], ctypes.c_int),
     "stfem_stop_vmult_add": ([_vp, _vp, _vp], ctypes.c_int),
     "stfem_vanka_create": ([_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_int,
                             ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
@@ -480,6 +481,10 @@ class SystemMatrix(_Handle):
 
     def Tvmult(self, dst, src):
         _check(load().stfem_stop_tvmult(self.h, _ptr(dst), _ptr(src)))
+
+    def vmult_noalloc(self, dst, src):
+        """vmult with the library's fixed scratch array (no allocation)."""
+        _check(load().stfem_stop_vmult_noalloc(self.h, _ptr(dst), _ptr(src)))
 
     def vmult_add(self, dst, src):
         _check(load().stfem_stop_vmult_add(self.h, _ptr(dst), _ptr(src)))

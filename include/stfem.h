@@ -312,6 +312,16 @@ int stfem_stop_info(stfem_stop op, long *info);
  * for Tvmult) */
 int stfem_stop_vmult(stfem_stop op, void *dst, const void *src);
 int stfem_stop_tvmult(stfem_stop op, void *dst, const void *src);
+/* No counterpart in the reference.  stfem_stop_vmult for callers that cannot
+ * allocate device memory during the call (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation): the same result
+ * with the operator's cell-major intermediate results in a fixed 64 MiB
+ * device array of the library instead of the operator's own buffer.  Calls no
+ * allocator.  Operators whose intermediate results exceed 64 MiB
+ * (n_cells * (p+1)^dim * nt_rows * precision bytes) return STFEM_ERR_ARG.
+ * The array is one per GPU and shared by every call on that GPU: calls that
+ * overlap on one GPU (several threads or processes' streams) race. */
+int stfem_stop_vmult_noalloc(stfem_stop op, void *dst, const void *src);
 /* SystemMatrix::vmult_add(BlockVector&, const Vector&), operators.h:154-187:
  * dst (nt_rows values per node) += (Alpha(:,0) (x) K + Beta(:,0) (x) M) src
  * (one value per node); needs nt_cols = 1 */
