@@ -208,14 +208,7 @@ extern "C"
       for (long k = 0; k < g.nn[2]; ++k)
         for (long j = 0; j < g.nn[1]; ++j)
           for (long i = 0; i < g.nn[0]; ++i)
-            {
-              const long idx[3] = {i, j, k};
-              bool       c      = false;
-              for (int d = 0; d < g.dim; ++d)
-                c = c || ((g.dirichlet >> (2 * d)) & 1 && idx[d] == 0) ||
-                    ((g.dirichlet >> (2 * d + 1)) & 1 && idx[d] == g.nn[d] - 1);
-              mask[i + g.nn[0] * (j + g.nn[1] * k)] = c ? 1 : 0;
-            }
+            mask[i + g.nn[0] * (j + g.nn[1] * k)] = node_constrained(g, i, j, k) ? 1 : 0;
     });
   }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "grid.h"
 #include "status.h"
 
 #include <cstdint>
@@ -22,20 +23,4 @@ namespace stfem
     }                                                                                             \
   while (0)
 
-  // Largest 1D node count (space degree + 1) with compiled kernels.
-  constexpr int MAX_N = 6;
-
-  // Structured level description passed to kernels.  Nodes of the
-  // Gauss-Lobatto grid are numbered lexicographically (x fastest); a vector
-  // stores the nt time components of a node contiguously ([node][t]).
-  struct GridDesc
-  {
-    int  dim;
-    int  p;            // space degree
-    int  nc[3];        // cells per direction
-    int  nn[3];        // nodes per direction (nc*p + 1)
-    long n_cells;
-    long n_nodes;
-    int  dirichlet;    // bit f set: face f (x0,x1,y0,y1,z0,z1) carries zero Dirichlet values
-  };
 } // namespace stfem
