@@ -373,10 +373,11 @@ def kernel_only(a, spec, X, kappa):
     log(f"calibration copy: {COPY_BYTES} bytes read, {COPY_BYTES} bytes written per launch")
     g = st.Problem(dict(spec, mg_float=1), vertices=X, kappa=kappa)
     L = g.n_levels - 1
-    what = {"vanka": (2, L), "vmult32": (1, L), "vmult64": (0, L)}[a.kernel_only]
-    ms = g.time_kernel(what[0], what[1], warmup=1, reps=5)
-    byts = g.bytes_model(1, L) if a.kernel_only == "vanka" else g.bytes_model(0)
-    log(f"{a.kernel_only}: {ms * 1e3:.1f} us per launch, model bytes {byts:.4e}")
+    for k in a.kernel_only.split(","):
+        what = {"vanka": (2, L), "vmult32": (1, L), "vmult64": (0, L)}[k]
+        ms = g.time_kernel(what[0], what[1], warmup=1, reps=5)
+        byts = g.bytes_model(1, L) if k == "vanka" else g.bytes_model(0)
+        log(f"{k}: {ms * 1e3:.1f} us per launch, model bytes {byts:.4e}")
     g.close()
 
 
@@ -408,7 +409,9 @@ def main():
                          "the 64^3 headline)")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="override a solver spec key (e.g. atomic_scatter=1)")
-    ap.add_argument("--kernel-only", choices=["vanka", "vmult32", "vmult64"], default=None)
+    ap.add_argument("--kernel-only", default=None, metavar="KERNELS",
+                    help="comma-separated subset of vanka, vmult32, vmult64: set up the FP32 "
+                         "multigrid problem and launch only these finest-level kernels")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "

@@ -11,6 +11,9 @@
 // S^T M_c S = I, S^T K_c S = diag(lambda), W_k = (lambda_k Alpha + Beta)^{-1}
 // applied by solving the nt x nt time block from lambda_k.  The per-cell device code is in
 // dev_vanka.h.
+#include <mutex>
+#include <vector>
+
 #include "dev_vanka.h"
 
 #include <algorithm>
@@ -190,6 +193,8 @@ namespace stfem
             {
               STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v3<T, NL, NT, false>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+              STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v3<T, NL, NT, true>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
               STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v3_opc<T, NL, NT>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
             }
@@ -198,16 +203,26 @@ namespace stfem
               M, g, S, lam, ta, tb, rhs, opc, cells, omega);
           else if (variant == 4)
             {
-              static int resident = 0;
-              if (!resident)
-                {
-                  int dev = 0, cus = 0, per = 0;
-                  STFEM_HIP(hipGetDevice(&dev));
-                  STFEM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-                  STFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per, (const void *)k_vanka_factored_v3<T, NL, NT, true>, 256, lds));
-                  resident = std::max(1, cus * std::max(1, per));
-                }
+              // resident workgroups of the persistent form, per device
+              static std::mutex       mtx;
+              static std::vector<int> per_device;
+              int                     dev = 0, resident = 0;
+              STFEM_HIP(hipGetDevice(&dev));
+              {
+                std::lock_guard<std::mutex> lk(mtx);
+                if (dev >= int(per_device.size()))
+                  per_device.resize(dev + 1, 0);
+                if (!per_device[dev])
+                  {
+                    int cus = 0, per = 0;
+                    STFEM_HIP(
+                      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+                    STFEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                      &per, (const void *)k_vanka_factored_v3<T, NL, NT, true>, 256, lds));
+                    per_device[dev] = std::max(1, cus * std::max(1, per));
+                  }
+                resident = per_device[dev];
+              }
               const long grid = std::min<long>(g.n_cells, resident);
               k_vanka_factored_v3<T, NL, NT, true><<<dim3((unsigned)grid), dim3(256), lds, s>>>(
                 M, g.n_cells, S, lam, ta, tb, rhs, ax, cells, omega);

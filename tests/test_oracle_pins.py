@@ -263,3 +263,92 @@ def test_relaxation_estimate_against_dense_eigenvalues(name):
         checked += 1
     assert checked >= 1
     assert P.omega(0) == 1.0  # gmg.h:795-796
+
+
+# ------------------------------------------------- composition and Krylov
+def test_space_time_operator_is_the_kronecker_sum():
+    """SystemMatrix::vmult (operators.h:94-122) on block vectors: dst_j =
+    sum_i Alpha(j,i) K src_i + Beta(j,i) M src_i, with K and M the assembled
+    spatial matrices (constrained rows and columns read as zero) and the
+    time weights of fe_time.h; Tvmult with the transposed weights."""
+    spec = dict(dim=2, refine=2, p=3, type=2, r=2, nts=2, r_min=0)
+    P = _problem(spec, 0.2, True)
+    l = P.n_levels - 1
+    K, M = P.level_matrix(l, 0), P.level_matrix(l, 1)
+    info = P.level_info(l)
+    n = info["n_nodes"]
+    free = np.ones(n)
+    xs = logical_coords(info, spec["p"], 2)
+    on_b = np.any((np.abs(xs) < 1e-14) | (np.abs(xs - 1) < 1e-14), axis=1)
+    free[on_b] = 0.0
+    D = np.diag(free)
+    Kf, Mf = D @ K @ D, D @ M @ D
+    A, B, _, _ = orc.fe_time_weights(orc.DG, 2, P.tau, 2)
+    x = np.random.default_rng(2).standard_normal(n * A.shape[0])
+    ref = (np.kron(A, Kf) + np.kron(B, Mf)) @ x
+    assert np.linalg.norm(P.fine_vmult(x) - ref) < 1e-13 * np.linalg.norm(ref)
+    reft = (np.kron(A.T, Kf) + np.kron(B.T, Mf)) @ x
+    assert np.linalg.norm(P.fine_vmult(x, transpose=True) - reft) < 1e-13 * np.linalg.norm(reft)
+
+
+def test_time_transfers_are_kronecker_products():
+    """tensorproduct_add (operators.h:19-69) with the fe_time.h matrices:
+    prolongation dst += (P (x) I) src, restriction dst += (P^T (x) I) src,
+    for tau (P = time_prolongation) and k levels (P = time_projection)."""
+    spec = dict(dim=2, refine=1, p=2, type=2, r=2, nts=2, r_min=0)
+    P = _problem(spec)
+    rng = np.random.default_rng(12)
+    seen = set()
+    for l in range(1, P.n_levels):
+        fi, ci = P.level_info(l), P.level_info(l - 1)
+        kind = fi["mg_type"]
+        if kind == 3:
+            continue
+        Pt = (orc.time_prolongation(orc.DG, fi["r"], fi["nts"]) if kind == 1 else
+              orc.time_projection(orc.DG, fi["r"] - 1, fi["r"], fi["nts"]))
+        n = fi["n_nodes"]
+        I = np.eye(n)
+        xc = rng.standard_normal(ci["n_nodes"] * ci["nt"])
+        y0 = rng.standard_normal(n * fi["nt"])
+        got = P.level_apply(l, 2, xc, dst=y0.copy())
+        assert np.allclose(got, y0 + np.kron(Pt, I) @ xc, rtol=0, atol=1e-13)
+        xf = rng.standard_normal(n * fi["nt"])
+        c0 = rng.standard_normal(ci["n_nodes"] * ci["nt"])
+        got = P.level_apply(l, 3, xf, dst=c0.copy())
+        assert np.allclose(got, c0 + np.kron(Pt.T, I) @ xf, rtol=0, atol=1e-13)
+        seen.add(kind)
+    assert seen == {1, 2}
+
+
+def test_fgmres_residuals_are_minimal_over_the_krylov_space():
+    """SolverFGMRES (time_integrators.h:174) with a fixed (linear)
+    preconditioner P is right-preconditioned GMRES: its j-th residual is
+    min_y |r0 - A P V_j y| over the Krylov space V_j = span{r0, (AP) r0, ...},
+    computed here by dense least squares, independently of the Arnoldi
+    process and of how the solver updates its residual estimate."""
+    spec = dict(dim=2, refine=1, p=2, type=2, r=1, nts=1, r_min=0, reltol=1e-10,
+                abstol=1e-300)
+    P = _problem(spec)
+    info = P.level_info(P.n_levels - 1)
+    n = info["n_nodes"] * info["nt"]
+    A = np.stack([P.fine_vmult(np.eye(n)[j]) for j in range(n)], 1)
+    Pm = np.stack([P.gmg_vmult(np.eye(n)[j]) for j in range(n)], 1)
+    xs = logical_coords(info, 2, 2)
+    on_b = np.any((np.abs(xs) < 1e-14) | (np.abs(xs - 1) < 1e-14), axis=1)
+    b = A @ np.random.default_rng(5).standard_normal(n)
+    b[np.tile(on_b, info["nt"])] = 0.0
+    rc, x, its, hist = P.fgmres(b, np.zeros(n))
+    assert rc == 0 and its >= 2
+    AP = A @ Pm
+    r0 = b.copy()
+    K = [r0 / np.linalg.norm(r0)]
+    for j in range(1, its + 1):
+        # residual after j steps: best combination of AP K_j
+        Q, _ = np.linalg.qr(np.stack(K, 1))
+        W = AP @ Q
+        y, *_ = np.linalg.lstsq(W, r0, rcond=None)
+        rj = np.linalg.norm(r0 - W @ y)
+        assert abs(rj - hist[j]) <= 1e-8 * hist[0] + 1e-6 * rj, (j, rj, hist[j])
+        K.append(AP @ K[-1])
+        K[-1] /= np.linalg.norm(K[-1])
+    assert np.linalg.norm(b - A @ x) <= 1.01e-10 * np.linalg.norm(b)
