@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "dev_vanka.h"
+#include "dev_vanka_dma.h"
 
 #include <algorithm>
 #include <type_traits>
@@ -159,6 +160,59 @@ namespace stfem
     return ok;
   }
 
+  namespace
+  {
+    // the FP32 sweep with LDS-prefetched S (dev_vanka_dma.h): two persistent
+    // workgroups per CU; false if the shape has no such kernel
+    bool launch_vanka_v5(const GridDesc &g, int nt, const float *S, const float *lam,
+                         const float *ta, const float *tb, const float *rhs, const float *ax,
+                         float *cells, float omega, hipStream_t s)
+    {
+      const int nloc = patch_size(g);
+      if (g.dim != 3 || nt < 1 || nt > 4 || (nloc != 64 && nloc != 125))
+        return false;
+      static std::mutex       mtx;
+      static std::vector<int> cus;
+      int                     dev = 0, ncu = 0;
+      STFEM_HIP(hipGetDevice(&dev));
+      {
+        std::lock_guard<std::mutex> lk(mtx);
+        if (dev >= int(cus.size()))
+          cus.resize(dev + 1, 0);
+        if (!cus[dev])
+          STFEM_HIP(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+        ncu = cus[dev];
+      }
+      const PatchMap M    = make_map(g);
+      const long     grid = std::min<long>(g.n_cells, 2L * std::max(1, ncu));
+      bool           done = false;
+      auto           go   = [&](auto nl_c, auto nt_c) {
+        constexpr int NL = decltype(nl_c)::value, NT = decltype(nt_c)::value;
+        if (done || nloc != NL || nt != NT)
+          return;
+        constexpr size_t lds = V5<NL, NT>::lds;
+        if (lds > max_lds_per_block())
+          return;
+        if (lds > 64 * 1024)
+          STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v5<NL, NT>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        k_vanka_factored_v5<NL, NT><<<dim3((unsigned)grid), dim3(256), lds, s>>>(
+          M, g.n_cells, S, lam, ta, tb, rhs, ax, cells, omega);
+        STFEM_HIP(hipGetLastError());
+        done = true;
+      };
+      auto for_nt = [&](auto nl_c) {
+        go(nl_c, std::integral_constant<int, 1>{});
+        go(nl_c, std::integral_constant<int, 2>{});
+        go(nl_c, std::integral_constant<int, 3>{});
+        go(nl_c, std::integral_constant<int, 4>{});
+      };
+      for_nt(std::integral_constant<int, 64>{});
+      for_nt(std::integral_constant<int, 125>{});
+      return done;
+    }
+  } // namespace
+
   template <typename T>
   void launch_vanka_factored(const GridDesc &g, int nt, const T *S, const T *lam, const T *ta,
                              const T *tb, const T *rhs, const T *ax, T *u, T omega, T *cells,
@@ -174,6 +228,13 @@ namespace stfem
     const size_t base = 2 * size_t(nt) * nloc * sizeof(T);
     const size_t full = base + size_t(nloc) * nloc * sizeof(T);
     const PatchMap M  = make_map(g);
+    if constexpr (std::is_same_v<T, float>)
+      if (variant == 5 && cells && !opc &&
+          launch_vanka_v5(g, nt, S, lam, ta, tb, rhs, ax, cells, omega, s))
+        {
+          launch_gather_cells<T>(g, nt, cells, u, nullptr, mode, false, s);
+          return;
+        }
     // compile-time patch shapes first
     bool done = false;
     auto try_v3 = [&](auto nl_c, auto nt_c) {

@@ -443,8 +443,8 @@ namespace stfem
                   "spec: orthogonalization must be 0 or 1");
     STFEM_REQUIRE(ps.op_jb >= 1 && ps.op_jb <= 4 && ps.op_jb64 >= 0 && ps.op_jb64 <= 4,
                   "spec: op_jb must be in [1, 4], op_jb64 in [0, 4]");
-    STFEM_REQUIRE(ps.vanka_kernel == 0 || (ps.vanka_kernel >= 2 && ps.vanka_kernel <= 4),
-                  "spec: vanka_kernel must be 0, 2, 3 or 4");
+    STFEM_REQUIRE(ps.vanka_kernel == 0 || (ps.vanka_kernel >= 2 && ps.vanka_kernel <= 5),
+                  "spec: vanka_kernel must be 0, 2, 3, 4 or 5");
     STFEM_REQUIRE(ps.coarse_coef >= 0 && ps.coarse_coef <= 3, "spec: coarse_coef must be in [0, 3]");
     STFEM_REQUIRE(ps.solver.basis >= 2 && ps.solver.basis <= 60, "spec: basis must be in [2, 60]");
     STFEM_REQUIRE(ps.solver.maxiter >= 1, "spec: maxiter must be positive");

@@ -475,10 +475,11 @@ def test_single_cell_levels_shortcut():
 
 @pytest.mark.parametrize("name", ["3d_q3_dg2_fp32_factored", "3d_q4_dg3_fp32_factored",
                                   "3d_q3_dg2_factored"])
-@pytest.mark.parametrize("variant", [2, 3, 4])
+@pytest.mark.parametrize("variant", [2, 3, 4, 5])
 def test_vanka_kernel_variants(name, variant):
     """The factored sweep kernels (2 LDS-staged S, 3 register-tiled S, 4 its
-    software-pipelined persistent form in FP32) on every level against the
+    software-pipelined persistent form in FP32, 5 the FP32 form streaming S
+    through LDS with direct global-to-LDS loads) on every level against the
     oracle's PreconditionVanka::vmult."""
     spec, o, _ = problems(name)
     c = CASES[name]
@@ -500,3 +501,62 @@ def test_vanka_kernel_variants(name, variant):
         ref = o.level_apply(l, 1, x)
         assert rel_err(nodes_to_blocks(y.cpu().double().numpy(), nt), ref) < tol_of(g) * 100
     g.close()
+
+
+def test_streamed_sweep_over_many_cells_per_workgroup():
+    """The LDS-streamed FP32 sweep (vanka_kernel=5) runs one persistent
+    workgroup per CU over a range of cells, loading the next cell's S while
+    it works on the current one.  On 8^3 cells (two cells per workgroup on
+    256 CUs, Q3/dG(2)) it matches the oracle's PreconditionVanka::vmult on
+    the finest level; on 16^3 cells (16 cells per workgroup, Q4/dG(3)) it
+    matches the register-tiled sweep (pinned to the oracle above) on every
+    level, including the two-cell rounding at the ends of the ranges."""
+    rng = np.random.default_rng(44)
+    # Q3/dG(2), 8^3 cells, against the oracle (finest level)
+    spec = dict(dim=3, refine=3, type=2, r=2, nts=1, p=3, r_min=0, mg_float=1, vanka_factored=1,
+                mg_space=0, space_time_mg=0)
+    nc = [8] * 3
+    X = box_vertices(3, nc, perturb=0.2, seed=7)
+    kap = log_uniform(512, 1e-2, 1e2, seed=11)
+    o = orc.Problem(dict(spec, vanka_invert_T=0), coords=X, kappa=kap)
+    g = st.Problem(dict(spec, vanka_kernel=5), vertices=X, kappa=kap)
+    L = g.n_levels - 1
+    info = g.level_info(L)
+    n, nt = info["n_nodes"], info["nt"]
+    x = rng.standard_normal(n * nt)
+    xg = torch.tensor(blocks_to_nodes(x, nt), dtype=torch.float32, device=DEV)
+    y = torch.zeros_like(xg)
+    print("[streamed sweep] 8^3 Q3 dG(2) vs oracle", flush=True)
+    g.vanka(L, y, xg)
+    ref = o.level_apply(L, 1, x)
+    assert rel_err(nodes_to_blocks(y.cpu().double().numpy(), nt), ref) < 2e-4
+    g.close()
+    # Q4/dG(3), 16^3 cells, against the register-tiled sweep
+    spec = dict(dim=3, refine=4, type=2, r=3, nts=1, p=4, r_min=2, mg_float=1, vanka_factored=1,
+                coarse_coef=3, dense_coarse_dofs=0)
+    nc = [16] * 3
+    X = box_vertices(3, nc, perturb=0.1, seed=2026)
+    kap = log_uniform(4096, 1e-3, 1e3, seed=2027)
+    a = st.Problem(dict(spec, vanka_kernel=5), vertices=X, kappa=kap)
+    b = st.Problem(dict(spec, vanka_kernel=3), vertices=X, kappa=kap)
+    for l in range(a.n_levels):
+        info = a.level_info(l)
+        xl = torch.tensor(rng.standard_normal(info["n_nodes"] * info["nt"]), dtype=torch.float32,
+                          device=DEV)
+        ya, yb = torch.zeros_like(xl), torch.zeros_like(xl)
+        print(f"[streamed sweep] 16^3 Q4 dG(3) level {l}", flush=True)
+        a.vanka(l, ya, xl)
+        b.vanka(l, yb, xl)
+        assert rel_err(ya.cpu().double().numpy(), yb.cpu().double().numpy()) < 1e-5, l
+    # and the whole slab solve takes the same iterations
+    n, nt = a.n_nodes, a.nt
+    xa = torch.zeros(n * nt, dtype=torch.float64, device=DEV)
+    xb = torch.zeros_like(xa)
+    prev = torch.zeros(n, dtype=torch.float64, device=DEV)
+    print("[streamed sweep] 16^3 slab solve", flush=True)
+    ra, ia = a.solve_slab(xa, prev, 0.0)
+    rb, ib = b.solve_slab(xb, prev, 0.0)
+    assert ra == 0 and rb == 0 and abs(ia - ib) <= 1, (ia, ib)
+    assert rel_err(xa.cpu().numpy(), xb.cpu().numpy()) < 1e-10
+    a.close()
+    b.close()
