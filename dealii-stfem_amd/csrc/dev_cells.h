@@ -10,6 +10,19 @@ namespace stfem
 {
   namespace
   {
+    // gfx950 hands workgroup b of a launch to XCD b % 8 (round robin over
+    // the 8 XCDs, each with its own L2).  Renumber the workgroups so that
+    // XCD x works through one contiguous eighth of [0, G): neighbouring
+    // cells, which share nodes of the input vectors and cache lines of the
+    // cell-major buffer, then meet in one L2 instead of in eight.  A
+    // bijection of [0, G) for every G.
+    __device__ __forceinline__ long xcd_block(long b, long G)
+    {
+      constexpr long X = 8;
+      const long     q = G / X, r = G % X, x = b % X;
+      return x * q + (x < r ? x : r) + b / X;
+    }
+
     // Position of the cell-local value (cell (cx,cy,cz), local node
     // (a,b,c), output component j) in the cell-major result buffer: for each
     // row of cells along x, local (c,b) and component, the values of all

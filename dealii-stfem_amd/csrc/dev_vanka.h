@@ -750,10 +750,11 @@ namespace stfem
 
       if constexpr (!PIPE)
         {
-          // one cell per workgroup
-          Buf A;
-          load(A, (long)blockIdx.x);
-          process(A, (long)blockIdx.x);
+          // one cell per workgroup, consecutive cells on one XCD
+          const long cell = xcd_block((long)blockIdx.x, (long)gridDim.x);
+          Buf        A;
+          load(A, cell);
+          process(A, cell);
         }
       else
         {

@@ -28,7 +28,8 @@ namespace stfem
       // output components [j0, jn) of this grid row
       const int j0 = (int)blockIdx.y * jb;
       const int jn = (j0 + jb < A.nt_out) ? j0 + jb : A.nt_out;
-      st_apply_cells<DIM, N, JB, T, TO_CELLS>(A, src, dst, j0, jn, (long)blockIdx.x,
+      st_apply_cells<DIM, N, JB, T, TO_CELLS>(A, src, dst, j0, jn,
+                                              xcd_block((long)blockIdx.x, (long)gridDim.x),
                                               (int)threadIdx.x, lds0, lds1, vlds);
     }
 
@@ -37,7 +38,7 @@ namespace stfem
       k_gather_nodes(const GridDesc g, int nto, const T *__restrict__ cells, T *__restrict__ dst,
                      const T *__restrict__ rhs, int mode)
     {
-      const long node = (long)blockIdx.x * 256 + threadIdx.x;
+      const long node = xcd_block((long)blockIdx.x, (long)gridDim.x) * 256 + threadIdx.x;
       if (node < g.n_nodes)
         gather_node<DIM, N, NT, T, SKIP_DIR>(g, nto, cells, dst, rhs, mode, node);
     }

@@ -31,7 +31,8 @@ namespace stfem
                        T omega)
     {
       extern __shared__ unsigned char smem_raw[];
-      vanka_explicit_cell<T>(M, nt, nloc, binv, rhs, ax, u, omega, (long)blockIdx.x, true,
+      vanka_explicit_cell<T>(M, nt, nloc, binv, rhs, ax, u, omega,
+                             xcd_block((long)blockIdx.x, (long)gridDim.x), true,
                              (int)threadIdx.x, (int)blockDim.x,
                              reinterpret_cast<T *>(smem_raw));
     }
@@ -45,7 +46,7 @@ namespace stfem
     {
       extern __shared__ unsigned char smem_raw[];
       vanka_factored_cell<T, STAGE>(M, nt, nloc, ld, S, lam, ta, tb, rhs, ax, u, omega,
-                                    (long)blockIdx.x,
+                                    xcd_block((long)blockIdx.x, (long)gridDim.x),
                                     (int)threadIdx.x, (int)blockDim.x,
                                     reinterpret_cast<T *>(smem_raw));
     }
@@ -59,7 +60,7 @@ namespace stfem
     {
       extern __shared__ __align__(16) unsigned char smem_raw[];
       vanka_v2_cell<T, NL, NT, TWO_PASS>(M, S, lam, ta, tb, rhs, ax, u, cells, omega,
-                                         (long)blockIdx.x,
+                                         xcd_block((long)blockIdx.x, (long)gridDim.x),
                                          (int)threadIdx.x, reinterpret_cast<T *>(smem_raw));
     }
 
