@@ -583,6 +583,7 @@ namespace stfem
         T     lk; // eigenvalue of mode k = tid
       };
       auto load = [&](Buf &b, long cell) {
+        // S is read once per sweep: streaming (nontemporal) loads
         const T *Sg = S + (size_t)cell * NL * LDG;
 #pragma unroll
         for (int m = 0; m < R; ++m)
@@ -591,7 +592,8 @@ namespace stfem
             {
               const int a = ta + 16 * m, g = tk + 16 * h;
               if (a < NL && VW * g < LDG)
-                b.s[m][h] = *reinterpret_cast<const vec_t *>(Sg + a * LDG + VW * g);
+                b.s[m][h] = __builtin_nontemporal_load(
+                  reinterpret_cast<const vec_t *>(Sg + a * LDG + VW * g));
               else
                 b.s[m][h] = vec_t(T(0));
             }
