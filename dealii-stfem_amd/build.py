@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libstfem_amd.so")
 BUILD = os.path.join(HERE, "_build")
 
-SOURCES = ["k_operator.hip", "k_noalloc.hip", "k_error.hip", "k_setup.hip", "k_adjoint.hip", "k_vanka.hip", "k_transfer.hip", "k_blas.hip",
+SOURCES = ["k_operator.hip", "k_noalloc.hip", "k_error.hip", "k_setup.hip", "k_eigen.hip", "k_adjoint.hip", "k_vanka.hip", "k_transfer.hip", "k_blas.hip",
            "time_fe.cpp", "spec.cpp", "mesh.cpp", "stfem_host.cpp", "capi.cpp", "capi_objects.cpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -172,6 +172,22 @@ namespace stfem
   // [e0, e0 + ne), which must contain every neighbour of the patches.
   void launch_patch_matrices(const GridDesc &g, long p0, long np, long e0, const double *Me,
                              const double *Ke, double *Mp, double *Kp, hipStream_t s);
+  // Batched generalized eigensolver of np patches of order n <= 128 (FP64):
+  // K_c z = lambda M_c z, z^T M_c z = 1.  Mp, Kp: column-major n x n per
+  // patch, symmetric (Mp is overwritten with the Cholesky factor, Kp with
+  // the eigenvectors, eigenvector k in column k); lam: the eigenvalues (n per
+  // patch, unsorted); ee, hh: n per patch of work space; rot: the rotation
+  // log, patch_eigen_rotation_cap(n) entries per patch; chains:
+  // patch_eigen_chain_cap(n) per patch; counts: two per patch.  info[c] = 0
+  // on success, 1 for a mass matrix that is not positive definite, 2 for an
+  // eigenvalue iteration that did not converge, 3 for a log that would
+  // overflow.
+  long   patch_eigen_rotation_cap(int n);
+  int    patch_eigen_chain_cap(int n);
+  size_t patch_eigen_lds(int n);
+  void   launch_patch_eigen(int n, long np, double *Mp, double *Kp, double *lam, double *ee,
+                            double *hh, double2 *rot, int2 *chains, int *counts, int *info,
+                            hipStream_t s);
   // S_c (row-major with the row stride vanka_s_ld, eigenvector k = column k)
   // from the column-major eigenvectors Z and the eigenvalues in the level
   // precision (lam_out, n_c per cell); sets *singular when a time block

@@ -437,8 +437,8 @@ namespace stfem
     STFEM_REQUIRE(ps.dirichlet >= 0 && ps.dirichlet <= 63,
                   "spec: dirichlet must be a bit mask of the box faces (0-63)");
     ps.dirichlet &= (1 << (2 * ps.dim)) - 1; // faces of the box in dim dimensions
-    STFEM_REQUIRE(ps.vanka_device_setup >= 0 && ps.vanka_device_setup <= 5,
-                  "spec: vanka_device_setup must be in [0, 5]");
+    STFEM_REQUIRE(ps.vanka_device_setup >= 0 && ps.vanka_device_setup <= 6,
+                  "spec: vanka_device_setup must be in [0, 6]");
     STFEM_REQUIRE(ps.solver.orthogonalization == 0 || ps.solver.orthogonalization == 1,
                   "spec: orthogonalization must be 0 or 1");
     STFEM_REQUIRE(ps.op_jb >= 1 && ps.op_jb <= 4 && ps.op_jb64 >= 0 && ps.op_jb64 <= 4,

@@ -88,7 +88,7 @@ namespace stfem
     int          atomic_scatter = 0; // 1: one-pass operator with atomic adds
     int          problem = 0; // 0 heat, 1 wave (first order in time, tp_01.cc:167-197)
     int          coarse_coef = 0; // coarse-level coefficient: 0 sampled, 1 geometric, 2 harmonic, 3 arithmetic mean
-    int          vanka_device_setup = 3; // factored Vanka on the GPU: 1 Jacobi, 2 divide and conquer, 3 by patch size
+    int          vanka_device_setup = 3; // factored Vanka on the GPU: 1 Jacobi, 2 divide and conquer, 4/5 Cholesky + divide and conquer/Jacobi, 6 own batched eigensolver, 3 by patch size
     int          vanka_kernel = 0; // factored sweep kernel: 0 automatic, 2 LDS-staged, 3 register-tiled
     // output components per operator cell pass (FP32 levels, FP64 operators;
     // 0: 2 for FP64 from Q4 on, where four do not fit two waves per SIMD, else 4)

@@ -508,12 +508,19 @@ namespace stfem
       }
     if (form_ == Form::factored && device_setup != 0 && nt_ <= 8)
       {
-        // batched generalized eigensolver: reduction to a standard problem
-        // with batched BLAS-3, then Jacobi for patches of up to 64 nodes and
-        // divide and conquer above (the faster of the two on each side)
+        // batched generalized eigensolver: the hand-written one (Cholesky,
+        // Householder and implicit QL per patch in LDS) up to 128 nodes per
+        // patch, above that the reduction to a standard problem with batched
+        // BLAS-3 and rocSOLVER's divide and conquer
         const int nl = (m.g.dim == 3) ? sh.n * sh.n * sh.n : sh.n * sh.n;
-        setup_factored_on_device(m, sh, Alpha, Beta,
-                                 device_setup == 3 ? (nl > 64 ? 4 : 5) : device_setup);
+        int       dev = 0, lds = 0;
+        STFEM_HIP(hipGetDevice(&dev));
+        STFEM_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const bool own_ok = nl <= 128 && patch_eigen_lds(nl) + 64 <= size_t(lds);
+        int        mth    = device_setup == 3 ? (own_ok ? 6 : 4) : device_setup;
+        if (mth == 6 && !own_ok)
+          mth = 4;
+        setup_factored_on_device(m, sh, Alpha, Beta, mth);
         return;
       }
     std::vector<double> Kp, Mp;
@@ -711,14 +718,22 @@ namespace stfem
     launch_fill_zero<int>(flag.get(), 1, s);
     const long layer = (g.dim == 3) ? long(g.nc[0]) * g.nc[1] : long(g.nc[0]);
     const int  nlay  = (g.dim == 3) ? g.nc[2] : g.nc[1];
-    // about 8 GB of FP64 work arrays per chunk, at most 16384 patches
-    const long per_cell = long(bs) * 8 * 5;
-    const long target   = std::min<long>(16384, std::max<long>(1, (8L << 30) / per_cell));
+    // about 8 GB of FP64 work arrays per chunk (12 GB with the rotation log
+    // of the hand-written eigensolver), at most 16384 patches
+    const bool own      = method == 6;
+    const long cap      = own ? patch_eigen_rotation_cap(nl) : 0;
+    const long per_cell = long(bs) * 8 * 5 + cap * long(sizeof(double2));
+    const long target =
+      std::min<long>(16384, std::max<long>(1, ((own ? 12L : 8L) << 30) / per_cell));
     const int  lper     = int(std::max<long>(1, std::min<long>(nlay, target / layer)));
     const long emax     = std::min<long>(nlay, lper + 2) * layer, pmax = long(lper) * layer;
     DBuf<double> Me{size_t(emax) * bs}, Ke{size_t(emax) * bs}, Mp{size_t(pmax) * bs},
       Kp{size_t(pmax) * bs}, lam{size_t(pmax) * nl}, res{size_t(pmax) * nl};
     DBuf<int>     sweeps{size_t(pmax)}, info{size_t(pmax)};
+    DBuf<double>  hh{own ? size_t(pmax) * nl : 0};
+    DBuf<double2> rot{size_t(pmax) * cap};
+    DBuf<int2>    chains{own ? size_t(pmax) * patch_eigen_chain_cap(nl) : 0};
+    DBuf<int>     counts{own ? 2 * size_t(pmax) : 0};
     rocblas_handle h = nullptr;
     if (rocblas_create_handle(&h) != rocblas_status_success)
       throw Error(STATUS_ERR_INTERNAL, "rocBLAS: handle creation failed");
@@ -737,8 +752,36 @@ namespace stfem
                                 Ke.get(), s);
         launch_patch_matrices(g, p0, np, e0, Me.get(), Ke.get(), Mp.get(), Kp.get(), s);
         // K_c z = lambda M_c z with z^T M_c z = 1: eigenvectors overwrite Kp
-        rocblas_status st;
-        if (method >= 4)
+        rocblas_status st = rocblas_status_success;
+        int            mth = method;
+        if (own)
+          {
+            launch_patch_eigen(nl, np, Mp.get(), Kp.get(), lam.get(), res.get(), hh.get(),
+                               rot.get(), chains.get(), counts.get(), info.get(), s);
+            STFEM_HIP(hipMemcpyAsync(hinfo.data(), info.get(), size_t(np) * sizeof(int),
+                                     hipMemcpyDeviceToHost, s));
+            ctx_.sync();
+            bool redo = false;
+            for (long k = 0; k < np; ++k)
+              {
+                if (hinfo[k] == 1)
+                  throw Error(STATUS_ERR_INTERNAL,
+                              "Vanka factored: patch mass not positive definite");
+                redo = redo || hinfo[k] != 0;
+              }
+            if (redo)
+              {
+                // an eigenvalue iteration that did not settle (or a full
+                // rotation log): this chunk again by divide and conquer
+                launch_patch_matrices(g, p0, np, e0, Me.get(), Ke.get(), Mp.get(), Kp.get(), s);
+                mth = 4;
+              }
+          }
+        if (own && mth == 6)
+          {
+            // eigenvectors in Kp, eigenvalues in lam
+          }
+        else if (mth >= 4)
           {
             // reduction to a standard problem with batched BLAS-3:
             // M = L L^T, C = L^{-1} K L^{-T}, C Q = Q Lambda, Z = L^{-T} Q
@@ -766,7 +809,7 @@ namespace stfem
                 for (long k = 0; k < np; ++k)
                   if (hinfo[k] != 0)
                     throw Error(STATUS_ERR_INTERNAL, "Vanka factored: patch mass not positive definite");
-                st = method == 4 ?
+                st = mth == 4 ?
                        rocsolver_dsyevd_strided_batched(h, rocblas_evect_original,
                                                         rocblas_fill_lower, nl, Kp.get(), nl, sb,
                                                         lam.get(), nl, res.get(), nl, info.get(),
@@ -1674,7 +1717,9 @@ namespace stfem
           }
         alpha1_ = w1.Alpha;
         gamma1_ = w1.Gamma;
+        plog("FP64 system and rhs operators");
         solver_ = std::make_unique<SolverFGMRES>(ctx_, fine_->m(), spec.solver);
+        plog("FGMRES basis");
         rhs_vec_.alloc(size_t(fine_->m()));
         coefs_.alloc(64);
         ones_.alloc(64);
@@ -1698,6 +1743,7 @@ namespace stfem
               [&](const double *x) { return gaussian_bumps_value(bumps, spec.dim, x); });
           }
         ctx_.sync();
+        plog("load vector");
       }
 
       // The space-time multigrid of tp_01.cc:200-359 / GMG::reinit
@@ -2284,75 +2330,119 @@ namespace stfem
                2 * PI * f * std::cos(2 * PI * f * t);
       }
       // load vector h_i = int h(x) phi_i with Gauss(p+1) quadrature on the
-      // mapped cells (VectorTools::create_right_hand_side), Dirichlet rows 0
+      // mapped cells (VectorTools::create_right_hand_side), Dirichlet rows 0.
+      // Per cell the quadrature values w_q h(x_q) are contracted with the 1D
+      // basis one direction at a time (sum factorization); cells run in
+      // parallel within each of the 2^dim parity classes of (cx, cy, cz),
+      // whose cells share no node, so every node sums its cells in the
+      // same order for any thread count.
       template <typename F>
       void build_load(F hfun)
       {
         const MeshLevel &m   = mesh_.back();
         const GridDesc  &g   = m.g;
-        const int        dim = g.dim, n = sh_.n, nq = n;
+        const int        dim = g.dim, n = sh_.n;
+        const int        nz  = dim == 3 ? n : 1;
         std::vector<double> h(g.n_nodes, 0.0);
         const long          nv0 = g.nc[0] + 1, nv1 = g.nc[1] + 1;
-        for (long c = 0; c < g.n_cells; ++c)
+        const long          hc[3] = {(g.nc[0] + 1) / 2, (g.nc[1] + 1) / 2,
+                                     dim == 3 ? (g.nc[2] + 1) / 2 : 1};
+        for (int color = 0; color < (1 << dim); ++color)
           {
-            int cx, cy, cz;
-            cell_xyz(g, c, cx, cy, cz);
-            double V[8][3] = {};
-            for (int v = 0; v < (1 << dim); ++v)
-              {
-                const long id =
-                  (cx + (v & 1)) + nv0 * ((cy + ((v >> 1) & 1)) + nv1 * (cz + ((v >> 2) & 1)));
-                for (int d = 0; d < dim; ++d)
-                  V[v][d] = m.X[id * dim + d];
-              }
-            for (int qz = 0; qz < ((dim == 3) ? nq : 1); ++qz)
-              for (int qy = 0; qy < nq; ++qy)
-                for (int qx = 0; qx < nq; ++qx)
-                  {
-                    const double xi[3] = {sh_.qx[qx], sh_.qx[qy], dim == 3 ? sh_.qx[qz] : 0.0};
-                    double       x[3] = {0, 0, 0}, J[3][3] = {};
-                    for (int v = 0; v < (1 << dim); ++v)
-                      {
-                        double N0 = 1;
-                        for (int e = 0; e < dim; ++e)
-                          N0 *= ((v >> e) & 1) ? xi[e] : 1 - xi[e];
-                        for (int e = 0; e < dim; ++e)
-                          {
-                            double gg = 1;
-                            for (int ff = 0; ff < dim; ++ff)
-                              {
-                                const int bit = (v >> ff) & 1;
-                                gg *= (ff == e) ? (bit ? 1.0 : -1.0) : (bit ? xi[ff] : 1 - xi[ff]);
-                              }
-                            for (int d = 0; d < dim; ++d)
-                              J[d][e] += gg * V[v][d];
-                          }
-                        for (int d = 0; d < dim; ++d)
-                          x[d] += N0 * V[v][d];
-                      }
-                    double det;
-                    if (dim == 2)
-                      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-                    else
-                      det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-                    const double w =
-                      sh_.qw[qx] * sh_.qw[qy] * (dim == 3 ? sh_.qw[qz] : 1.0) * std::abs(det);
-                    const double hv = hfun(x);
-                    for (int c3 = 0; c3 < ((dim == 3) ? n : 1); ++c3)
-                      for (int b = 0; b < n; ++b)
-                        for (int a = 0; a < n; ++a)
-                          {
-                            const long gx = (long)cx * sh_.p + a, gy = (long)cy * sh_.p + b,
-                                       gz = (long)cz * sh_.p + c3;
-                            if (node_constrained(g, gx, gy, gz))
-                              continue;
-                            const double phi = sh_.S[qx * n + a] * sh_.S[qy * n + b] *
-                                               (dim == 3 ? sh_.S[qz * n + c3] : 1.0);
-                            h[gx + g.nn[0] * (gy + g.nn[1] * gz)] += hv * phi * w;
-                          }
-                  }
+            const int  ox = color & 1, oy = (color >> 1) & 1, oz = (color >> 2) & 1;
+            const long ncol = hc[0] * hc[1] * hc[2];
+#pragma omp parallel
+            {
+              std::vector<double> fq(size_t(n) * n * nz), t1(fq.size()), t2(fq.size());
+#pragma omp for schedule(static)
+              for (long k = 0; k < ncol; ++k)
+                {
+                  const int cx = int(2 * (k % hc[0]) + ox), cy = int(2 * ((k / hc[0]) % hc[1]) + oy);
+                  const int cz = dim == 3 ? int(2 * (k / (hc[0] * hc[1])) + oz) : 0;
+                  if (cx >= g.nc[0] || cy >= g.nc[1] || (dim == 3 && cz >= g.nc[2]))
+                    continue;
+                  double V[8][3] = {};
+                  for (int v = 0; v < (1 << dim); ++v)
+                    {
+                      const long id = (cx + (v & 1)) +
+                                      nv0 * ((cy + ((v >> 1) & 1)) + nv1 * (cz + ((v >> 2) & 1)));
+                      for (int d = 0; d < dim; ++d)
+                        V[v][d] = m.X[id * dim + d];
+                    }
+                  // fq[qz][qy][qx] = w_q |det J| h(x_q)
+                  for (int qz = 0; qz < nz; ++qz)
+                    for (int qy = 0; qy < n; ++qy)
+                      for (int qx = 0; qx < n; ++qx)
+                        {
+                          const double xi[3] = {sh_.qx[qx], sh_.qx[qy], dim == 3 ? sh_.qx[qz] : 0.0};
+                          double       x[3] = {0, 0, 0}, J[3][3] = {};
+                          for (int v = 0; v < (1 << dim); ++v)
+                            {
+                              double N0 = 1;
+                              for (int e = 0; e < dim; ++e)
+                                N0 *= ((v >> e) & 1) ? xi[e] : 1 - xi[e];
+                              for (int e = 0; e < dim; ++e)
+                                {
+                                  double gg = 1;
+                                  for (int ff = 0; ff < dim; ++ff)
+                                    {
+                                      const int bit = (v >> ff) & 1;
+                                      gg *= (ff == e) ? (bit ? 1.0 : -1.0) : (bit ? xi[ff] : 1 - xi[ff]);
+                                    }
+                                  for (int d = 0; d < dim; ++d)
+                                    J[d][e] += gg * V[v][d];
+                                }
+                              for (int d = 0; d < dim; ++d)
+                                x[d] += N0 * V[v][d];
+                            }
+                          double det;
+                          if (dim == 2)
+                            det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+                          else
+                            det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                  J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                  J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                          const double w =
+                            sh_.qw[qx] * sh_.qw[qy] * (dim == 3 ? sh_.qw[qz] : 1.0) * std::abs(det);
+                          fq[(size_t(qz) * n + qy) * n + qx] = hfun(x) * w;
+                        }
+                  // contract x, then y, then z: t[.][.][a] = sum_q S[q][a] f[.][.][q]
+                  for (int qz = 0; qz < nz; ++qz)
+                    for (int qy = 0; qy < n; ++qy)
+                      for (int a = 0; a < n; ++a)
+                        {
+                          double acc = 0;
+                          for (int qx = 0; qx < n; ++qx)
+                            acc += sh_.S[qx * n + a] * fq[(size_t(qz) * n + qy) * n + qx];
+                          t1[(size_t(qz) * n + qy) * n + a] = acc;
+                        }
+                  for (int qz = 0; qz < nz; ++qz)
+                    for (int b = 0; b < n; ++b)
+                      for (int a = 0; a < n; ++a)
+                        {
+                          double acc = 0;
+                          for (int qy = 0; qy < n; ++qy)
+                            acc += sh_.S[qy * n + b] * t1[(size_t(qz) * n + qy) * n + a];
+                          t2[(size_t(qz) * n + b) * n + a] = acc;
+                        }
+                  for (int c3 = 0; c3 < nz; ++c3)
+                    for (int b = 0; b < n; ++b)
+                      for (int a = 0; a < n; ++a)
+                        {
+                          double acc = 0;
+                          if (dim == 3)
+                            for (int qz = 0; qz < n; ++qz)
+                              acc += sh_.S[qz * n + c3] * t2[(size_t(qz) * n + b) * n + a];
+                          else
+                            acc = t2[size_t(b) * n + a];
+                          const long gx = (long)cx * sh_.p + a, gy = (long)cy * sh_.p + b,
+                                     gz = (long)cz * sh_.p + c3;
+                          if (node_constrained(g, gx, gy, gz))
+                            continue;
+                          h[gx + g.nn[0] * (gy + g.nn[1] * gz)] += acc;
+                        }
+                }
+            }
           }
         load_.upload(h.data(), h.size(), ctx_.stream);
       }

@@ -426,13 +426,14 @@ def test_dense_coarse_operator(name, dense):
 @pytest.mark.parametrize("name,method", [
     ("3d_q3_dg2_factored", 3), ("3d_q3_dg2_fp32_factored", 3), ("3d_q3_dg2_factored", 1),
     ("3d_q3_dg2_factored", 2), ("3d_q3_dg2_factored", 4), ("3d_q3_dg2_factored", 5),
-    ("3d_q4_dg3_fp32_factored", 3)])
+    ("3d_q3_dg2_factored", 6), ("3d_q4_dg3_fp32_factored", 3), ("3d_q4_dg3_fp32_factored", 4)])
 def test_vanka_device_setup_matches_host(name, method):
     """The factored Vanka set up on the GPU (element and patch matrices,
     batched generalized eigensolver, time blocks) applies the same operator
     as the one set up on the host, on every level, for each eigensolver:
     1 Jacobi, 2 divide and conquer, 4/5 Cholesky reduction + divide and
-    conquer/Jacobi, 3 the automatic choice."""
+    conquer/Jacobi, 6 the hand-written batched eigensolver, 3 the automatic
+    choice (6 for patches of up to 128 nodes)."""
     spec, o, _ = problems(name)
     c = CASES[name]
     g = st.Problem(dict(spec, vanka_device_setup=method),
@@ -557,6 +558,39 @@ def test_streamed_sweep_over_many_cells_per_workgroup():
     ra, ia = a.solve_slab(xa, prev, 0.0)
     rb, ib = b.solve_slab(xb, prev, 0.0)
     assert ra == 0 and rb == 0 and abs(ia - ib) <= 1, (ia, ib)
+    assert rel_err(xa.cpu().numpy(), xb.cpu().numpy()) < 1e-10
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("dim,p,refine", [(3, 4, 1), (3, 2, 2), (2, 5, 2)])
+def test_own_eigensolver_on_symmetric_patches(dim, p, refine):
+    """The hand-written batched eigensolver (Cholesky, Householder, implicit
+    QL with a rotation log) on an unperturbed mesh with a constant
+    coefficient, where the patch matrices have the symmetries of the square
+    or cube and hence exactly repeated eigenvalues: the Vanka it sets up
+    matches the host setup (dense generalized eigensolver) on every level, in
+    FP64, and the slab solve takes the same iterations."""
+    spec = dict(dim=dim, refine=refine, type=2, r=1, nts=1, p=p, r_min=0, vanka_factored=1)
+    a = st.Problem(dict(spec, vanka_device_setup=6))
+    b = st.Problem(dict(spec, vanka_device_setup=0))
+    rng = np.random.default_rng(5)
+    for l in range(a.n_levels):
+        info = a.level_info(l)
+        x = torch.tensor(rng.standard_normal(info["n_nodes"] * info["nt"]), dtype=torch.float64,
+                         device=DEV)
+        ya, yb = torch.zeros_like(x), torch.zeros_like(x)
+        print(f"[own eigensolver] dim {dim} Q{p} level {l}", flush=True)
+        a.vanka(l, ya, x)
+        b.vanka(l, yb, x)
+        assert rel_err(ya.cpu().numpy(), yb.cpu().numpy()) < 1e-11, l
+    n, nt = a.n_nodes, a.nt
+    xa = torch.zeros(n * nt, dtype=torch.float64, device=DEV)
+    xb = torch.zeros_like(xa)
+    prev = torch.ones(n, dtype=torch.float64, device=DEV)
+    ra, ia = a.solve_slab(xa, prev, 0.0)
+    rb, ib = b.solve_slab(xb, prev, 0.0)
+    assert ra == 0 and rb == 0 and ia == ib, (ia, ib)
     assert rel_err(xa.cpu().numpy(), xb.cpu().numpy()) < 1e-10
     a.close()
     b.close()
