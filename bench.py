@@ -403,10 +403,10 @@ def main():
                     help="cells per side 2^refine (headline 6, q3 and wave 5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--pmc", choices=["auto", "on", "off"], default="auto",
-                    help="rocprofv3 counter passes behind roofline.traffic (auto: on for "
-                         "workloads whose setup under the profiler takes seconds, off for "
-                         "the 64^3 headline)")
+    ap.add_argument("--pmc", choices=["on", "off"], default="on",
+                    help="rocprofv3 counter passes behind roofline.traffic (two extra "
+                         "processes that set up the FP32 multigrid problem and launch the "
+                         "dominant kernel; about 50 s each for the 64^3 headline)")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="override a solver spec key (e.g. atomic_scatter=1)")
     ap.add_argument("--kernel-only", default=None, metavar="KERNELS",
@@ -480,12 +480,8 @@ def main():
                    symbol=r"k_st_apply<3, \d, \d, double")
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
 
-    pmc = a.pmc == "on" or (a.pmc == "auto" and a.workload != "headline")
-    traffic, traffic_note = None, "counter passes off (--pmc)"
-    if a.pmc == "auto" and a.workload == "headline":
-        traffic_note = ("counter passes not run by default at 64^3: the setup under the "
-                        "profiler takes minutes (bench.py --pmc on runs them)")
-    if pmc:
+    traffic, traffic_note = None, "counter passes off (--pmc off)"
+    if a.pmc == "on":
         traffic, traffic_note = pmc_traffic(dom["only"], dom["symbol"], a.workload, a.refine,
                                             timeout_s=300 if a.workload == "headline" else 150)
         log(f"traffic: {traffic if traffic else traffic_note}")
