@@ -31,6 +31,7 @@ namespace
     const ProblemSpec d;
     A.set_max_jb(sizeof(T) == 4 ? d.op_jb : d.op_jb64);
     A.set_split_below(d.op_split);
+    A.set_op_kernel(d.op_kernel);
   }
 
   void check_precision(int precision)

@@ -11,7 +11,7 @@
 // (up to 2^dim) contributions of every node in a fixed order.  This avoids
 // the device-scope float atomics of the one-pass form, which are performed
 // one at a time at the memory side and cost more than the whole cell pass.
-#include "dev_operator.h"
+#include "dev_operator_wave.h"
 
 namespace stfem
 {
@@ -31,6 +31,18 @@ namespace stfem
       st_apply_cells<DIM, N, JB, T, TO_CELLS>(A, src, dst, j0, jn,
                                               xcd_block((long)blockIdx.x, (long)gridDim.x),
                                               (int)threadIdx.x, lds0, lds1, vlds);
+    }
+
+    template <int N, int NTI, typename T>
+    __global__ void __launch_bounds__(WaveLayout<N>::BLOCK)
+      k_st_apply_wave(const StOpArgs<T> A, const T *__restrict__ src, T *__restrict__ cells, int jb)
+    {
+      using W = WaveLayout<N>;
+      __shared__ T lds[W::GPB * W::CPG * W::PER];
+      const int j0 = (int)blockIdx.y * jb;
+      const int jn = (j0 + jb < A.nt_out) ? j0 + jb : A.nt_out;
+      st_apply_wave<N, NTI, T>(A, src, cells, j0, jn, xcd_block((long)blockIdx.x, (long)gridDim.x),
+                               lds);
     }
 
     template <int DIM, int N, int NT, typename T, bool SKIP_DIR>
@@ -81,6 +93,26 @@ namespace stfem
     {
       using Ly              = Layout<DIM, N>;
       const int  nto        = a.nt_out;
+      if constexpr (DIM == 3 && TO_CELLS)
+        if (a.kernel == 1 && a.nt_in <= 4)
+          {
+            // all output components in one pass over the inputs; on small
+            // grids one component per grid row
+            using W           = WaveLayout<N>;
+            const long blocks = (a.g.n_cells + W::CPB - 1) / W::CPB;
+            const int  jb     = blocks < a.split_below ? 1 : nto;
+            const dim3 grid{(unsigned)blocks, (unsigned)((nto + jb - 1) / jb), 1};
+            constexpr unsigned WB = W::BLOCK;
+            switch (a.nt_in)
+              {
+                case 1: k_st_apply_wave<N, 1, T><<<grid, WB, 0, s>>>(a, src, dst, jb); break;
+                case 2: k_st_apply_wave<N, 2, T><<<grid, WB, 0, s>>>(a, src, dst, jb); break;
+                case 3: k_st_apply_wave<N, 3, T><<<grid, WB, 0, s>>>(a, src, dst, jb); break;
+                default: k_st_apply_wave<N, 4, T><<<grid, WB, 0, s>>>(a, src, dst, jb); break;
+              }
+            STFEM_HIP(hipGetLastError());
+            return;
+          }
       const int  mj         = (a.max_jb >= 1 && a.max_jb <= 4) ? a.max_jb : 4;
       const long blocks     = (a.g.n_cells + Ly::CPB - 1) / Ly::CPB;
       int        chunks     = (nto + mj - 1) / mj;

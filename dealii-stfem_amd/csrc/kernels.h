@@ -40,6 +40,11 @@ namespace stfem
     // shorter chain per workgroup on small levels, at the price of
     // evaluating the inputs once per component (0: off)
     int           split_below = 0;
+    // 3D two-pass form: 1 (default) the cells of a workgroup are handled
+    // by independent wavefronts (lines redistributed through LDS within a
+    // wavefront), 0 by the whole workgroup (lines in registers, the other
+    // directions swept through LDS between workgroup barriers)
+    int           kernel = 1;
   };
   // cells != null: two-pass form with the cell-major scratch `cells` of
   // st_cells_size() values; cells == null: one pass with atomic adds.

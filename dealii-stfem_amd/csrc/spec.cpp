@@ -304,7 +304,7 @@ namespace stfem
       "dim", "subdiv", "lo", "hi", "refine", "p", "type", "r", "nts", "tau", "space_time_mg",
       "nts_min", "r_min", "time_before_space", "mg_space", "mg_float", "dirichlet",
       "atomic_scatter", "coarse_coef", "problem", "vanka_device_setup", "vanka_kernel", "op_jb",
-      "op_jb64", "op_split", "restrict_is_transpose", "smoothing_steps", "variable",
+      "op_jb64", "op_split", "op_kernel", "restrict_is_transpose", "smoothing_steps", "variable",
       "estimate_relaxation", "eig_iters", "eig_safety", "overwrite_transfer", "vanka_factored",
       "exact_single_cell", "use_graph", "fuse_residual", "dense_coarse_dofs", "maxiter",
       "abstol", "reltol", "basis", "orthogonalization", "extrapolate", "end_time", "frequency",
@@ -385,6 +385,7 @@ namespace stfem
     geti("op_jb", ps.op_jb);
     geti("op_jb64", ps.op_jb64);
     geti("op_split", ps.op_split);
+    geti("op_kernel", ps.op_kernel);
     getbi("restrict_is_transpose", ps.restrict_is_transpose);
     geti("smoothing_steps", ps.mg.smoothing_steps);
     getb("variable", ps.mg.variable);
@@ -443,6 +444,7 @@ namespace stfem
                   "spec: orthogonalization must be 0 or 1");
     STFEM_REQUIRE(ps.op_jb >= 1 && ps.op_jb <= 4 && ps.op_jb64 >= 0 && ps.op_jb64 <= 4,
                   "spec: op_jb must be in [1, 4], op_jb64 in [0, 4]");
+    STFEM_REQUIRE(ps.op_kernel == 0 || ps.op_kernel == 1, "spec: op_kernel must be 0 or 1");
     STFEM_REQUIRE(ps.vanka_kernel == 0 || (ps.vanka_kernel >= 2 && ps.vanka_kernel <= 5),
                   "spec: vanka_kernel must be 0, 2, 3, 4 or 5");
     STFEM_REQUIRE(ps.coarse_coef >= 0 && ps.coarse_coef <= 3, "spec: coarse_coef must be in [0, 3]");
@@ -473,7 +475,7 @@ namespace stfem
       << ";atomic_scatter=" << atomic_scatter << ";coarse_coef=" << coarse_coef
       << ";problem=" << (problem == 1 ? "wave" : "heat")
       << ";vanka_device_setup=" << vanka_device_setup << ";vanka_kernel=" << vanka_kernel
-      << ";op_jb=" << op_jb << ";op_jb64=" << op_jb64 << ";op_split=" << op_split
+      << ";op_jb=" << op_jb << ";op_jb64=" << op_jb64 << ";op_split=" << op_split << ";op_kernel=" << op_kernel
       << ";restrict_is_transpose=" << restrict_is_transpose
       << ";smoothing_steps=" << mg.smoothing_steps << ";variable=" << int(mg.variable)
       << ";estimate_relaxation=" << int(mg.estimate_relaxation)

@@ -94,6 +94,7 @@ namespace stfem
     // 0: 2 for FP64 from Q4 on, where four do not fit two waves per SIMD, else 4)
     int          op_jb = 4, op_jb64 = 0;
     int          op_split = 128;         // StOpArgs::split_below of every operator
+    int          op_kernel = 1;          // StOpArgs::kernel of every operator (3D cell pass)
     // time restriction: the transpose of the prolongation (default, gmg.h:557)
     // or the L2 restriction / projection matrices (gmg.h:101-113)
     int          restrict_is_transpose = 1;

@@ -246,6 +246,7 @@ namespace stfem
     args.nt_out     = nt_out_;
     args.max_jb     = max_jb_ > 0 ? max_jb_ : (sizeof(T) == 8 && grid_.p >= 4 ? 2 : 4);
     args.split_below = split_below_;
+    args.kernel     = op_kernel_;
     launch_st_apply<T>(args, src, dst, cells ? cells : cells_.get(), mode, rhs, ctx_.stream);
   }
   template <typename T>
@@ -1691,6 +1692,7 @@ namespace stfem
                                                            ww.rhs_vM, tp);
             rhsv_->set_max_jb(spec.op_jb64);
             rhsv_->set_split_below(spec.op_split);
+            rhsv_->set_op_kernel(spec.op_kernel);
             // TimeIntegratorWave velocity matrices (time_integrators.h:219-233)
             const DMat Ai = w1.Alpha.inverse();
             DMat       AixB = Ai * w1.Beta, AixG = Ai * w1.Gamma, AixZ = Ai * w1.Zeta;
@@ -1714,6 +1716,7 @@ namespace stfem
           {
             op->set_max_jb(spec.op_jb64);
             op->set_split_below(spec.op_split);
+            op->set_op_kernel(spec.op_kernel);
           }
         alpha1_ = w1.Alpha;
         gamma1_ = w1.Gamma;
@@ -1800,6 +1803,7 @@ namespace stfem
                                                              spec.atomic_scatter == 0);
             levels[l].A->set_max_jb(sizeof(T) == 4 ? spec.op_jb : spec.op_jb64);
             levels[l].A->set_split_below(spec.op_split);
+            levels[l].A->set_op_kernel(spec.op_kernel);
             levels[l].V  = std::make_unique<PreconditionVanka<T>>(ctx_, m, sh_, w.Alpha, w.Beta, form,
                                                                   spec.vanka_device_setup);
             levels[l].V->set_variant(spec.vanka_kernel);

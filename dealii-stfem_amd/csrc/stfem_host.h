@@ -204,6 +204,8 @@ namespace stfem
     // grids of fewer cell workgroups than this compute one output
     // component per workgroup (StOpArgs::split_below)
     void set_split_below(int b) { split_below_ = b; }
+    // 3D cell pass: 1 wavefront-local cells, 0 workgroup cells (StOpArgs::kernel)
+    void set_op_kernel(int k) { op_kernel_ = k; }
     long        n_nodes() const { return grid_.n_nodes; }
     int         n_blocks() const { return nt_out_; }
     const double *vertices() const { return verts_.get(); }
@@ -225,7 +227,7 @@ namespace stfem
     DBuf<T>      coef_;
     DMat         A_, B_;
     int          nt_in_, nt_out_;
-    int          max_jb_ = 4, split_below_ = 0;
+    int          max_jb_ = 4, split_below_ = 0, op_kernel_ = 1;
     DBuf<T>      dA_, dB_, dAt_, dBt_;
     mutable DBuf<T> cells_; // cell-major results of the two-pass form
   };

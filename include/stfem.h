@@ -107,9 +107,12 @@ int stfem_time_mg_sequence(int n_sp_lvl, int k_max, int k_min, int nts, int nts_
  * modified Gram-Schmidt coefficients of
  * an FGMRES step from one pass of dots against the basis, corrected with
  * the basis' Gram matrix; 0: the chain of axpy+dot passes),
- * op_jb/op_jb64 (output components per operator pass, FP32 levels/FP64
- * operators; op_jb64=0, the default, picks 2 from Q4 on and 4 below),
- * op_split (operators with fewer cell workgroups than this, 128 by
+ * op_kernel (3D cell pass with at most 4 input time components: 1, the
+ * default, cells on groups of wavefronts, all output components in one
+ * pass; 0 the workgroup kernel, which 2D operators always use),
+ * op_jb/op_jb64 (output components per pass of the workgroup kernel, FP32
+ * levels/FP64 operators; op_jb64=0, the default, picks 2 from Q4 on and 4
+ * below), op_split (operators with fewer cell workgroups than this, 128 by
  * default, compute one output time component per workgroup).
  * Unknown keys are an error (STFEM_ERR_ARG).
  * vertices: finest-level vertex coordinates ((nc+1)^dim points, dim each,

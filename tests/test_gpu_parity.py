@@ -139,15 +139,19 @@ def test_level_operators(name):
 
 @pytest.mark.parametrize("name", ["3d_q3_dg2", "3d_q4_dg3_fp32", "2d_dg1_nts2", "3d_q2_cg2",
                                   "2d_dg2_distorted"])
-@pytest.mark.parametrize("chunking", [dict(op_split=0, op_jb=4, op_jb64=4),
-                                      dict(op_split=0, op_jb=2, op_jb64=2),
-                                      dict(op_split=0, op_jb=1, op_jb64=3),
-                                      dict(op_split=1 << 30, op_jb=4, op_jb64=4)])
+@pytest.mark.parametrize("chunking", [dict(op_split=0),
+                                      dict(op_split=1 << 30),
+                                      dict(op_kernel=0, op_split=0, op_jb=4, op_jb64=4),
+                                      dict(op_kernel=0, op_split=0, op_jb=2, op_jb64=2),
+                                      dict(op_kernel=0, op_split=0, op_jb=1, op_jb64=3),
+                                      dict(op_kernel=0, op_split=1 << 30, op_jb=4, op_jb64=4)])
 def test_operator_output_chunking(name, chunking):
-    """The operator kernel computes the output time components in chunks of
-    op_jb (FP32 levels) / op_jb64 (FP64) per workgroup, side by side as grid
-    rows, and one component per workgroup on grids below op_split
-    workgroups: every split gives the oracle's level and fine operators."""
+    """Both 3D cell-pass kernels give the oracle's level and fine operators
+    under every split of the output time components: the wavefront-group
+    kernel (op_kernel=1, default) all components per workgroup, or one per
+    grid row on grids below op_split workgroups; the workgroup kernel
+    (op_kernel=0, also the 2D kernel) chunks of op_jb (FP32 levels) /
+    op_jb64 (FP64) components side by side as grid rows."""
     spec, o, _ = problems(name)
     c = CASES[name]
     dim = spec["dim"]
