@@ -157,6 +157,169 @@ namespace stfem
         }
     }
 
+    // Element matrices of 3D cells by sum factorization (row-major NL x NL,
+    // FP64), cells c0 + blockIdx.x.  With phi_i = s(ai) s(bi) s(ci)
+    // (s = 1D basis at the Gauss points, s' its derivative), the mass matrix
+    // is sum_q phi_i phi_j JxW and the stiffness matrix sum_q sum_{d,e}
+    // (d_d phi_i) G_de (d_e phi_j) with G = kappa JxW J^{-1} J^{-T}: ten
+    // weighted fields f (the mass, and the nine (d, e) pairs) whose 1D
+    // factors in direction x are s or s' on each side.  The quadrature sum
+    // is contracted one direction at a time:
+    //   T1[f][ai,aj][qb,qc] = sum_qa Fx(f,i)[qa][ai] Fx(f,j)[qa][aj] w_f[qa,qb,qc]
+    //   T2[qc]              = sum_qb Fy(f,i)[qb][bi] Fy(f,j)[qb][bj] T1[f][ai,aj][qb,qc]
+    //   A[ci][cj]          += sum_qc Fz(f,i)[qc][ci] Fz(f,j)[qc][cj] T2[qc]
+    // (T2 per (ai, aj, bi, bj) in registers): about 10 (N^5 + N^6 + N^7)
+    // multiply-adds per cell instead of N^9 of the entry-by-entry sums.
+    // Dynamic LDS: 10 N^3 (weights) + 10 N^4 (T1) doubles.
+    template <int N>
+    __global__ void __launch_bounds__(256)
+      k_element_matrices_sf(const GridDesc g, const double *__restrict__ X, int coef_mode,
+                            double coef_value, const double *__restrict__ coef,
+                            const SetupTables tb, long c0, double *__restrict__ Me,
+                            double *__restrict__ Ke)
+    {
+      constexpr int NQ = N * N * N, NL = NQ, N2 = N * N, NF = 10;
+      extern __shared__ double smem[];
+      double *const wf = smem;           // [NF][NQ]
+      double *const T1 = wf + NF * NQ;   // [NF][ai * N + aj][qb + N * qc]
+      __shared__ double Sl[N][N], Dl[N][N];
+      for (int e = threadIdx.x; e < N * N; e += blockDim.x)
+        {
+          Sl[e / N][e % N] = tb.S[e / N][e % N];
+          Dl[e / N][e % N] = tb.Sd[e / N][e % N];
+        }
+      const long cell = c0 + blockIdx.x;
+      int        cx, cy, cz;
+      cell_pos(g, cell, cx, cy, cz);
+      const long nv0 = g.nc[0] + 1, nv1 = g.nc[1] + 1;
+      // weights at the Gauss points: JxW and kappa JxW J^{-1} J^{-T}
+      for (int q = threadIdx.x; q < NQ; q += blockDim.x)
+        {
+          const int    qa = q % N, qb = (q / N) % N, qc = q / N2;
+          const double xi[3] = {tb.qx[qa], tb.qx[qb], tb.qx[qc]};
+          double       J[3][3] = {};
+          for (int v = 0; v < 8; ++v)
+            {
+              const long id = (cx + (v & 1)) + nv0 * ((cy + ((v >> 1) & 1)) + nv1 * (cz + ((v >> 2) & 1)));
+              for (int e = 0; e < 3; ++e)
+                {
+                  double gg = 1;
+                  for (int f = 0; f < 3; ++f)
+                    {
+                      const int bit = (v >> f) & 1;
+                      gg *= (f == e) ? (bit ? 1.0 : -1.0) : (bit ? xi[f] : 1 - xi[f]);
+                    }
+                  for (int d = 0; d < 3; ++d)
+                    J[d][e] += gg * X[id * 3 + d];
+                }
+            }
+          double a[3][3];
+          a[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+          a[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+          a[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+          a[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+          a[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+          a[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+          a[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+          a[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+          a[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+          const double det = J[0][0] * a[0][0] + J[0][1] * a[1][0] + J[0][2] * a[2][0];
+          double       Ji[3][3];
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+              Ji[i][j] = a[i][j] / det;
+          const double w   = tb.qw[qa] * tb.qw[qb] * tb.qw[qc];
+          const double JxW = fabs(det) * w;
+          double       kap = coef_value;
+          if (coef_mode == 1)
+            kap = coef[cell];
+          else if (coef_mode == 2)
+            kap = coef[cell * NQ + q];
+          wf[q] = JxW;
+          for (int d = 0; d < 3; ++d)
+            for (int e = 0; e < 3; ++e)
+              wf[(1 + 3 * d + e) * NQ + q] =
+                kap * JxW * (Ji[d][0] * Ji[e][0] + Ji[d][1] * Ji[e][1] + Ji[d][2] * Ji[e][2]);
+        }
+      __syncthreads();
+      // 1D factor of field f, side (0: i, 1: j), direction dir: the
+      // derivative where the field's gradient component points along dir
+      auto deriv_side = [](int f, int side, int dir) {
+        if (f == 0)
+          return false;
+        const int d = (f - 1) / 3, e = (f - 1) % 3;
+        return (side == 0 ? d : e) == dir;
+      };
+      // contraction over qa
+      for (int idx = threadIdx.x; idx < NF * N2 * N2; idx += blockDim.x)
+        {
+          const int f = idx / (N2 * N2), r = idx - f * N2 * N2;
+          const int ij = r / N2, bc = r - ij * N2;
+          const int ai = ij / N, aj = ij - ai * N;
+          const bool di = deriv_side(f, 0, 0), dj = deriv_side(f, 1, 0);
+          double     acc = 0;
+#pragma unroll
+          for (int qa = 0; qa < N; ++qa)
+            acc += (di ? Dl[qa][ai] : Sl[qa][ai]) * (dj ? Dl[qa][aj] : Sl[qa][aj]) *
+                   wf[f * NQ + qa + N * bc];
+          T1[idx] = acc;
+        }
+      __syncthreads();
+      // contractions over qb (into registers) and qc, one (ai, aj, bi, bj,
+      // ci) per thread and step: the row entries cj = 0..N-1
+      const size_t base = size_t(blockIdx.x) * NL * NL;
+      for (int tup = threadIdx.x; tup < N2 * N2 * N; tup += blockDim.x)
+        {
+          const int ij = tup / (N2 * N), rest = tup - ij * N2 * N;
+          const int bij = rest / N, ci = rest - bij * N;
+          const int ai = ij / N, aj = ij - ai * N, bi = bij / N, bj = bij - bi * N;
+          double    m[N], k[N];
+#pragma unroll
+          for (int cj = 0; cj < N; ++cj)
+            m[cj] = k[cj] = 0;
+#pragma unroll 1
+          for (int f = 0; f < NF; ++f)
+            {
+              const bool dyi = deriv_side(f, 0, 1), dyj = deriv_side(f, 1, 1);
+              const bool dzi = deriv_side(f, 0, 2), dzj = deriv_side(f, 1, 2);
+              const double(*Fyi)[N] = dyi ? Dl : Sl;
+              const double(*Fyj)[N] = dyj ? Dl : Sl;
+              const double(*Fzi)[N] = dzi ? Dl : Sl;
+              const double(*Fzj)[N] = dzj ? Dl : Sl;
+              const double *t1 = T1 + (f * N2 + ij) * N2;
+              double        t2[N];
+#pragma unroll
+              for (int qc = 0; qc < N; ++qc)
+                {
+                  double acc = 0;
+#pragma unroll
+                  for (int qb = 0; qb < N; ++qb)
+                    acc += Fyi[qb][bi] * Fyj[qb][bj] * t1[qb + N * qc];
+                  t2[qc] = acc * Fzi[qc][ci];
+                }
+#pragma unroll
+              for (int cj = 0; cj < N; ++cj)
+                {
+                  double acc = 0;
+#pragma unroll
+                  for (int qc = 0; qc < N; ++qc)
+                    acc += Fzj[qc][cj] * t2[qc];
+                  if (f == 0)
+                    m[cj] = acc;
+                  else
+                    k[cj] += acc;
+                }
+            }
+          const size_t row = base + size_t(ai + N * bi + N2 * ci) * NL + aj + N * bj;
+#pragma unroll
+          for (int cj = 0; cj < N; ++cj)
+            {
+              Me[row + N2 * cj] = m[cj];
+              Ke[row + N2 * cj] = k[cj];
+            }
+        }
+    }
+
     __device__ __forceinline__ bool constrained_node(const GridDesc &g, long gx, long gy, long gz)
     {
       const int d = g.dirichlet;
@@ -294,8 +457,18 @@ namespace stfem
                      const double *coef, const SetupTables &tb, long c0, long nc, double *Me,
                      double *Ke, hipStream_t s)
     {
-      k_element_matrices<DIM, N><<<dim3((unsigned)nc), 256, 0, s>>>(g, X, coef_mode, coef_value,
-                                                                    coef, tb, c0, Me, Ke);
+      if constexpr (DIM == 3)
+        {
+          constexpr size_t lds = size_t(10) * (N * N * N + N * N * N * N) * sizeof(double);
+          if (lds > 64 * 1024)
+            STFEM_HIP(hipFuncSetAttribute((const void *)k_element_matrices_sf<N>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+          k_element_matrices_sf<N><<<dim3((unsigned)nc), 256, lds, s>>>(g, X, coef_mode, coef_value,
+                                                                        coef, tb, c0, Me, Ke);
+        }
+      else
+        k_element_matrices<DIM, N><<<dim3((unsigned)nc), 256, 0, s>>>(g, X, coef_mode, coef_value,
+                                                                      coef, tb, c0, Me, Ke);
     }
     template <int DIM, int N>
     void patches_dn(const GridDesc &g, long p0, long np, long e0, const double *Me,
