@@ -66,7 +66,17 @@ namespace stfem
       static constexpr int CPB   = CPG * GPB;                  // cells per workgroup
       static constexpr int NC    = N * N * N;                  // values per cell
       static constexpr int NB    = 4;                          // LDS line buffers per cell
-      static constexpr int PER   = NB * NC + 24;               // LDS values per cell (buffers, vertices)
+      // LDS layout of a cell's buffer: point (x, y, z) at x + N y + SZ z,
+      // buffers BS apart, cells PER apart (vertex coordinates after the
+      // buffers).  The padding is chosen by a model of the bank conflicts
+      // of the line accesses (32 banks of 4 bytes, 32 lanes of 4-byte and
+      // 16 lanes of 8-byte accesses at a time): N = 4 halves the conflict
+      // cycles of the unpadded layout, N = 3 and N = 5 take 30 % and 12 %
+      // off.
+      static constexpr int SZ    = N == 3 ? 9 : N == 4 ? 19 : N * N;
+      static constexpr int BS    = N == 4 ? 76 : N * N * N;
+      static constexpr int PER   = N == 3 ? 155 : N == 4 ? 336 : N == 5 ? 537 : NB * BS + 24;
+      static_assert(BS >= (N - 1) * (1 + N + SZ) + 1 && PER >= NB * BS + 24, "LDS layout");
       static __device__ __forceinline__ void sync()
       {
         if constexpr (G == 1)
@@ -79,14 +89,14 @@ namespace stfem
     // Lines of a cell in the three ownerships.  Lane (t0, t1) of the cell
     // holds the points (k, t0, t1) (OWN 0, lines along x), (t0, k, t1)
     // (OWN 1, along y) or (t0, t1, k) (OWN 2, along z), k = 0..N-1; a
-    // buffer holds the cell's values lexicographically (x fastest).
-    template <int N, int OWN>
+    // buffer holds the cell's values at x + N y + SZ z (x fastest, z lines padded).
+    template <int N, int OWN, int SZ = N * N>
     struct Line
     {
-      static constexpr int stride = OWN == 0 ? 1 : OWN == 1 ? N : N * N;
+      static constexpr int stride = OWN == 0 ? 1 : OWN == 1 ? N : SZ;
       static __device__ __forceinline__ int base(int t0, int t1)
       {
-        return OWN == 0 ? N * t0 + N * N * t1 : OWN == 1 ? t0 + N * N * t1 : t0 + N * t1;
+        return OWN == 0 ? N * t0 + SZ * t1 : OWN == 1 ? t0 + SZ * t1 : t0 + N * t1;
       }
       template <typename T>
       static __device__ __forceinline__ void put(T *buf, int b, const T (&v)[N])
@@ -195,9 +205,9 @@ namespace stfem
                                                   T *__restrict__ lds)
     {
       using W = WaveLayout<N>;
-      using LX = Line<N, 0>;
-      using LY = Line<N, 1>;
-      using LZ = Line<N, 2>;
+      using LX = Line<N, 0, W::SZ>;
+      using LY = Line<N, 1, W::SZ>;
+      using LZ = Line<N, 2, W::SZ>;
       using Tb = Tab<T>;
       constexpr int TPC = W::TPC, CPG = W::CPG, NC = W::NC, p = N - 1;
 
@@ -210,10 +220,10 @@ namespace stfem
       const long cell  = (blk * W::GPB + grp) * CPG + cs;
       const bool valid = active && cell < A.g.n_cells;
       T *const   B0    = lds + (grp * CPG + cs) * W::PER;
-      T *const   B1    = B0 + NC;
-      T *const   B2    = B1 + NC;
-      T *const   B3    = B2 + NC;
-      T *const   Vl    = B3 + NC;
+      T *const   B1    = B0 + W::BS;
+      T *const   B2    = B1 + W::BS;
+      T *const   B3    = B2 + W::BS;
+      T *const   Vl    = B3 + W::BS;
       const int  bx = LX::base(t0, t1), by = LY::base(t0, t1), bz = LZ::base(t0, t1);
 
       // --- cell position; the z line (t0, t1) of nodes and its Dirichlet mask
