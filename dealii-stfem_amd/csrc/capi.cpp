@@ -521,6 +521,7 @@ extern "C"
             case 3: pb.gmg_vmult((double *)b.get(), (const double *)a.get()); break;
             case 4: pb.level_prolongate(level, b.get(), a.get()); break;
             case 5: pb.level_restrict(level, b.get(), a.get()); break;
+            case 6: pb.level_vanka_cells(level, a.get()); break;
             default: throw Error(STATUS_ERR_ARG, "unknown kernel selector");
           }
       };

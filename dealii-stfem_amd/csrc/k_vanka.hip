@@ -222,7 +222,8 @@ namespace stfem
     if (g.n_cells == 0)
       return;
     STFEM_REQUIRE(!opc || cells, "vanka factored: the fused residual needs the two-pass form");
-    STFEM_REQUIRE(mode == 0 || mode == 1, "vanka factored: mode must be 0 or 1");
+    STFEM_REQUIRE(mode == 0 || mode == 1 || (mode == 3 && cells),
+                  "vanka factored: mode must be 0, 1 or (two-pass form) 3");
     STFEM_REQUIRE(nt <= 16, "vanka factored: at most 16 time dofs per patch");
     const int    nloc = patch_size(g);
     const int    thr  = nloc >= 256 ? 256 : ((nloc + 63) / 64) * 64;
@@ -233,7 +234,8 @@ namespace stfem
       if (variant == 5 && cells && !opc &&
           launch_vanka_v5(g, nt, S, lam, ta, tb, rhs, ax, cells, omega, s))
         {
-          launch_gather_cells<T>(g, nt, cells, u, nullptr, mode, false, s);
+          if (mode != 3)
+            launch_gather_cells<T>(g, nt, cells, u, nullptr, mode, false, s);
           return;
         }
     // compile-time patch shapes first
@@ -293,7 +295,8 @@ namespace stfem
             k_vanka_factored_v3<T, NL, NT, false><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
               M, g.n_cells, S, lam, ta, tb, rhs, ax, cells, omega);
           STFEM_HIP(hipGetLastError());
-          launch_gather_cells<T>(g, NT, cells, u, nullptr, mode, false, s);
+          if (mode != 3)
+            launch_gather_cells<T>(g, NT, cells, u, nullptr, mode, false, s);
           done = true;
         }
     };
@@ -313,7 +316,8 @@ namespace stfem
           k_vanka_factored_v2<T, NL, NT, true><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
             M, S, lam, ta, tb, rhs, ax, u, cells, omega);
           STFEM_HIP(hipGetLastError());
-          launch_gather_cells<T>(g, NT, cells, u, nullptr, mode, false, s);
+          if (mode != 3)
+            launch_gather_cells<T>(g, NT, cells, u, nullptr, mode, false, s);
         }
       else
         {
@@ -340,6 +344,7 @@ namespace stfem
         STFEM_HIP(hipGetLastError());
         return;
       }
+    STFEM_REQUIRE(mode != 3, "vanka factored: no two-pass kernel for this patch shape");
     if (mode == 0)
       launch_fill_zero<T>(u, size_t(g.n_nodes) * nt, s);
     if (full <= max_lds_per_block())

@@ -87,6 +87,8 @@ namespace stfem
   // LDS-staged kernel, 3 the register-tiled kernel (patches of at most 128
   // nodes), 4 its software-pipelined persistent form (FP32); shapes a
   // variant does not cover fall back to the others.
+  // mode 0: u = omega V r, 1: u += omega V r, 3 (two-pass form): only the
+  // cell pass into `cells` (u unused; for timing the sweep kernel alone).
   // opc non-null (two-pass form only, shapes with
   // vanka_fused_residual_supported): A x at the patch nodes is summed from
   // the operator's cell-major buffer opc (StOpArgs's level, nt components)

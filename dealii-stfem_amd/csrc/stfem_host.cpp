@@ -898,6 +898,14 @@ namespace stfem
            vanka_fused_residual_supported<T>(grid_, nt_);
   }
   template <typename T>
+  void PreconditionVanka<T>::cell_pass(const T *src) const
+  {
+    STFEM_REQUIRE(form_ == Form::factored && cells_.get(),
+                  "Vanka cell pass: needs the factored form with the two-pass buffer");
+    launch_vanka_factored<T>(grid_, nt_, S_.get(), lam_.get(), ta_.get(), tb_.get(), src, nullptr,
+                             nullptr, T(1), cells_.get(), 3, variant_, ctx_.stream);
+  }
+  template <typename T>
   void PreconditionVanka<T>::relax_opc(T *u, const T *rhs, const T *opc, T omega) const
   {
     const PhaseScope scope(ctx_, PhaseTimers::vanka);
@@ -1859,6 +1867,10 @@ namespace stfem
       void level_vanka(int l, void *dst, const void *src) override
       {
         gmg_->level(l).V->vmult((T *)dst, (const T *)src);
+      }
+      void level_vanka_cells(int l, const void *src) override
+      {
+        gmg_->level(l).V->cell_pass((const T *)src);
       }
       void level_prolongate(int l, void *dst, const void *src) override
       {

@@ -258,6 +258,9 @@ namespace stfem
     // fused_residual_ok(): one node gather fewer per sweep
     bool   fused_residual_ok() const;
     void   relax_opc(T *u, const T *rhs, const T *opc, T omega) const;
+    // the sweep kernel of vmult alone (factored form with the cell-major
+    // buffer): corrections of src into the cell buffer, no node gather
+    void   cell_pass(const T *src) const;
     long   n_cells() const { return grid_.n_cells; }
     long   n_nodes() const { return grid_.n_nodes; }
     int    n_blocks() const { return nt_; }
@@ -458,6 +461,7 @@ namespace stfem
     virtual double level_omega(int l) const = 0;
     virtual void level_vmult(int l, void *dst, const void *src, bool transpose) = 0;
     virtual void level_vanka(int l, void *dst, const void *src) = 0;
+    virtual void level_vanka_cells(int l, const void *src) = 0;
     virtual void level_prolongate(int l, void *dst, const void *src) = 0;
     virtual void level_restrict(int l, void *dst, const void *src) = 0;
     virtual void fine_vmult(double *dst, const double *src, bool transpose) = 0;

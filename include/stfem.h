@@ -240,7 +240,8 @@ int stfem_nodes_to_blocks(long n_nodes, int nt, const double *nodes, double *blo
 /* ---- measurement (HIP events on the library stream) ----------------- */
 /* average milliseconds of one application over `reps` after `warmup`:
  * what 0 fine FP64 vmult, 1 level vmult, 2 level Vanka, 3 GMG V-cycle,
- * 4 prolongate_and_add into `level`, 5 restrict_and_add from `level` */
+ * 4 prolongate_and_add into `level`, 5 restrict_and_add from `level`, 6 the
+ * sweep kernel of the level's factored Vanka alone (no node gather) */
 int stfem_time_kernel(stfem_problem p, int what, int level, int warmup, int reps, double *ms);
 /* TimerOutput scopes of the reference (printTiming): device time (ms) and
  * call counts of the phases {vmult, vanka, gmg, step} since creation or the

@@ -361,6 +361,19 @@ def test_transfer_timing_and_bytes_model():
     assert g.time_kernel(5, L, warmup=1, reps=2) > 0
 
 
+def test_vanka_sweep_kernel_timing():
+    """The sweep kernel of the factored Vanka can be timed alone (selector 6:
+    the cell pass without the node gather), and a level whose Vanka has no
+    two-pass buffer (explicit inverses) refuses it with an error status."""
+    spec, o, g = problems("3d_q3_dg2_fp32_factored")
+    L = g.n_levels - 1
+    print("[vanka sweep kernel timing]", flush=True)
+    assert g.time_kernel(6, L, warmup=1, reps=2) > 0
+    spec, o, g = problems("3d_q3_dg2")  # explicit inverses
+    with pytest.raises(RuntimeError):
+        g.time_kernel(6, g.n_levels - 1, warmup=1, reps=2)
+
+
 def test_graph_replay_matches_eager():
     """The V-cycle replayed from a captured HIP graph equals the eagerly
     launched V-cycle on every replay, with other device work in between."""
