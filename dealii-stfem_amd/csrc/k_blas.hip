@@ -306,13 +306,6 @@ namespace stfem
 
     inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
-    template <typename T>
-    __global__ void k_unit(long n, long j, T *v)
-    {
-      for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-           i += (long)gridDim.x * blockDim.x)
-        v[i] = (i == j) ? T(1) : T(0);
-    }
     __global__ void k_add_diagonal(long n, double a, double *M)
     {
       const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -325,13 +318,91 @@ namespace stfem
       if (i < n)
         x[i] *= a;
     }
-    template <typename T>
-    __global__ void k_store_column(long n, const T *__restrict__ v, double *__restrict__ M, long ld,
-                                   long j)
+    struct ProbeColour
     {
-      const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-      if (i < n)
-        M[j * ld + i] = double(v[i]);
+      int c[3];
+    };
+    __device__ __forceinline__ void lattice(const GridDesc &g, long node, long q[3])
+    {
+      q[0] = node % g.nn[0];
+      const long r = node / g.nn[0];
+      q[1]         = g.dim == 3 ? r % g.nn[1] : r;
+      q[2]         = g.dim == 3 ? r / g.nn[1] : 0;
+    }
+    template <typename T>
+    __global__ void k_probe_vector(const GridDesc in, int nt_in, ProbeColour c, int t, int stride,
+                                   T *__restrict__ x)
+    {
+      const long n = in.n_nodes * nt_in;
+      for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+           i += (long)gridDim.x * blockDim.x)
+        {
+          const long node = i / nt_in;
+          long       q[3];
+          lattice(in, node, q);
+          bool hit = int(i - node * nt_in) == t;
+          for (int d = 0; d < in.dim; ++d)
+            hit = hit && q[d] % stride == c.c[d];
+          x[i] = hit ? T(1) : T(0);
+        }
+    }
+    template <typename T>
+    __global__ void k_probe_store(const GridDesc in, int nt_in, const GridDesc out, int nt_out,
+                                  ProbeColour c, int t, int stride, int sn, int sd, int radius,
+                                  const T *__restrict__ y, double *__restrict__ M, long ld,
+                                  int *__restrict__ bad)
+    {
+      const long n = out.n_nodes * nt_out;
+      for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+           i += (long)gridDim.x * blockDim.x)
+        {
+          const T v = y[i];
+          long    o[3];
+          lattice(out, i / nt_out, o);
+          long node = 0, mul = 1;
+          bool ok   = true;
+          for (int d = 0; d < in.dim && ok; ++d)
+            {
+              // the colour's inputs g = c + stride m near o: |sn g - sd o| <= sd radius
+              const long base = (sd * o[d]) / sn; // g closest from below (scaled)
+              long       m0   = (base - c.c[d]) / stride - 1;
+              long       hitg = -1;
+              for (long m = m0; m <= m0 + 3; ++m)
+                {
+                  const long g = c.c[d] + stride * m;
+                  if (g < 0 || g >= in.nn[d])
+                    continue;
+                  const long dist = sn * g - sd * o[d];
+                  if (dist <= long(sd) * radius && -dist <= long(sd) * radius)
+                    hitg = g;
+                }
+              ok = hitg >= 0;
+              node += hitg * mul;
+              mul *= in.nn[d];
+            }
+          if (ok)
+            M[(node * nt_in + t) * ld + i] = double(v);
+          else if (v != T(0))
+            atomicAdd(bad, 1);
+        }
+    }
+    __global__ void k_transpose(long m, long n, const double *__restrict__ A, double *__restrict__ B)
+    {
+      __shared__ double tile[32][33];
+      const long i0 = long(blockIdx.x) * 32, j0 = long(blockIdx.y) * 32;
+      const int  tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 32 x 8
+      for (int k = ty; k < 32; k += 8)
+        {
+          const long i = i0 + tx, j = j0 + k; // A(i, j), i < m, j < n
+          tile[k][tx] = (i < m && j < n) ? A[i + j * m] : 0.0;
+        }
+      __syncthreads();
+      for (int k = ty; k < 32; k += 8)
+        {
+          const long j = j0 + tx, i = i0 + k; // B(j, i) = A(i, j)
+          if (i < m && j < n)
+            B[j + i * n] = tile[tx][k];
+        }
     }
     template <typename T>
     __global__ void k_pack_rows(long n, const double *__restrict__ M, T *__restrict__ R, long ld)
@@ -531,15 +602,6 @@ namespace stfem
     STFEM_HIP(hipGetLastError());
   }
 
-  template <typename T>
-  void launch_unit(long n, long j, T *v, hipStream_t s)
-  {
-    if (n == 0)
-      return;
-    const long blocks = std::min<long>((n + 255) / 256, 1024);
-    k_unit<T><<<dim3((unsigned)blocks), 256, 0, s>>>(n, j, v);
-    STFEM_HIP(hipGetLastError());
-  }
   void launch_add_diagonal(long n, double a, double *M, hipStream_t s)
   {
     if (n == 0)
@@ -552,14 +614,6 @@ namespace stfem
     if (n == 0)
       return;
     k_scale<<<nblk(n), 256, 0, s>>>(n, a, x);
-    STFEM_HIP(hipGetLastError());
-  }
-  template <typename T>
-  void launch_store_column(long n, const T *v, double *M, long ld, long j, hipStream_t s)
-  {
-    if (n == 0)
-      return;
-    k_store_column<T><<<nblk(n), 256, 0, s>>>(n, v, M, ld, j);
     STFEM_HIP(hipGetLastError());
   }
   template <typename T>
@@ -582,12 +636,54 @@ namespace stfem
     STFEM_HIP(hipGetLastError());
   }
 
-  template void launch_unit<double>(long, long, double *, hipStream_t);
-  template void launch_unit<float>(long, long, float *, hipStream_t);
-  template void launch_store_column<double>(long, const double *, double *, long, long,
+  template <typename T>
+  void launch_probe_vector(const GridDesc &in, int nt_in, const int *c, int t, int stride, T *x,
+                           hipStream_t s)
+  {
+    const long n = in.n_nodes * nt_in;
+    if (n == 0)
+      return;
+    ProbeColour pc{{c[0], in.dim > 1 ? c[1] : 0, in.dim > 2 ? c[2] : 0}};
+    const long  blocks = std::min<long>((n + 255) / 256, 4096);
+    k_probe_vector<T><<<dim3((unsigned)blocks), 256, 0, s>>>(in, nt_in, pc, t, stride, x);
+    STFEM_HIP(hipGetLastError());
+  }
+  template <typename T>
+  void launch_probe_store(const GridDesc &in, int nt_in, const GridDesc &out, int nt_out,
+                          const int *c, int t, int stride, int scale_num, int scale_den,
+                          int radius, const T *y, double *M, long ld, int *bad, hipStream_t s)
+  {
+    STFEM_REQUIRE(in.dim == out.dim && stride >= 1 && scale_num >= 1 && scale_den >= 1 &&
+                    radius >= 0 && long(stride) * scale_num > 2L * scale_den * radius,
+                  "probe: supports of one colour must be disjoint");
+    const long n = out.n_nodes * nt_out;
+    if (n == 0)
+      return;
+    ProbeColour pc{{c[0], in.dim > 1 ? c[1] : 0, in.dim > 2 ? c[2] : 0}};
+    const long  blocks = std::min<long>((n + 255) / 256, 4096);
+    k_probe_store<T><<<dim3((unsigned)blocks), 256, 0, s>>>(in, nt_in, out, nt_out, pc, t, stride,
+                                                            scale_num, scale_den, radius, y, M, ld,
+                                                            bad);
+    STFEM_HIP(hipGetLastError());
+  }
+  void launch_transpose(long m, long n, const double *A, double *B, hipStream_t s)
+  {
+    if (m == 0 || n == 0)
+      return;
+    const dim3 grid{(unsigned)((m + 31) / 32), (unsigned)((n + 31) / 32), 1};
+    k_transpose<<<grid, 256, 0, s>>>(m, n, A, B);
+    STFEM_HIP(hipGetLastError());
+  }
+  template void launch_probe_vector<double>(const GridDesc &, int, const int *, int, int, double *,
                                             hipStream_t);
-  template void launch_store_column<float>(long, const float *, double *, long, long,
+  template void launch_probe_vector<float>(const GridDesc &, int, const int *, int, int, float *,
                                            hipStream_t);
+  template void launch_probe_store<double>(const GridDesc &, int, const GridDesc &, int,
+                                           const int *, int, int, int, int, int, const double *,
+                                           double *, long, int *, hipStream_t);
+  template void launch_probe_store<float>(const GridDesc &, int, const GridDesc &, int, const int *,
+                                          int, int, int, int, int, const float *, double *, long,
+                                          int *, hipStream_t);
   template void launch_pack_rows<double>(long, const double *, double *, long, hipStream_t);
   template void launch_pack_rows<float>(long, const double *, float *, long, hipStream_t);
   template void launch_gemv_rows<double>(long, long, const double *, const double *, double *,

@@ -236,15 +236,28 @@ namespace stfem
                             hipStream_t s);
 
   // ------------------------------------------------- dense coarse operator
-  // v = e_j (length n)
-  template <typename T>
-  void launch_unit(long n, long j, T *v, hipStream_t s);
   // M[i + i*n] += a (column-major n x n);  x[0..n) *= a
   void launch_add_diagonal(long n, double a, double *M, hipStream_t s);
   void launch_scale(long n, double a, double *x, hipStream_t s);
-  // column j of the column-major FP64 matrix M (leading dimension ld) = v
+  // Probing of a local linear map (columns of its dense matrix in batches).
+  // A colour is (c[0..dim), t): the inputs (node, t) of the grid `in` with
+  // node lattice position g = c (mod stride) in every direction.
+  // launch_probe_vector: x = the sum of the colour's unit vectors.
+  // launch_probe_store: y = the map applied to that x; each output entry i
+  // (node of the grid `out` at lattice position o) belongs to the one input
+  // g of the colour with |scale_num * g - scale_den * o| <= scale_den *
+  // radius in every direction (none if there is no such g): M[j * ld + i] =
+  // y[i] for its input column j = node(g) * nt_in + t.  A nonzero y[i] that
+  // belongs to no input (supports not disjoint) is counted in *bad.
   template <typename T>
-  void launch_store_column(long n, const T *v, double *M, long ld, long j, hipStream_t s);
+  void launch_probe_vector(const GridDesc &in, int nt_in, const int *c, int t, int stride, T *x,
+                           hipStream_t s);
+  template <typename T>
+  void launch_probe_store(const GridDesc &in, int nt_in, const GridDesc &out, int nt_out,
+                          const int *c, int t, int stride, int scale_num, int scale_den,
+                          int radius, const T *y, double *M, long ld, int *bad, hipStream_t s);
+  // B (n x m, column-major, leading dimension n) = A^T (A: m x n, leading m)
+  void launch_transpose(long m, long n, const double *A, double *B, hipStream_t s);
   // R[i*ld + j] = T(M[i + j*n]) for i, j < n (column-major n x n FP64 into
   // row-major precision-T rows of ld >= n entries, padding zeroed)
   template <typename T>

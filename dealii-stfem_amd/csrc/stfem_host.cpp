@@ -1174,16 +1174,33 @@ namespace stfem
       return;
     hipStream_t s = ctx_.stream;
     DeviceBlas  blas(s);
-    // dense n_out x n_in matrix of a linear map given by its launcher
-    auto dense_of = [&](long n_out, long n_in, auto &&apply) {
+    // dense n_out x n_in matrix of a local linear map (structured grids)
+    // given by its launcher, by probing: the inputs of
+    // one colour (lattice position = c mod stride per direction, one time
+    // component) have disjoint output supports, so one application gives
+    // all their columns; stride * sn > 2 * sd * radius, where an input at
+    // lattice position g reaches the outputs o with |sn g - sd o| <= sd radius
+    DBuf<int> bad(1);
+    STFEM_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int), s));
+    auto dense_probe = [&](const GridDesc &gin, int nt_in, const GridDesc &gout, int nt_out,
+                           int stride, int sn, int sd, int radius, auto &&apply) {
+      const long   n_in = gin.n_nodes * nt_in, n_out = gout.n_nodes * nt_out;
       DBuf<double> D(size_t(n_out) * n_in);
       DBuf<T>      x(n_in), y(n_out);
-      for (long j = 0; j < n_in; ++j)
-        {
-          launch_unit<T>(n_in, j, x.get(), s);
-          apply(y.get(), x.get());
-          launch_store_column<T>(n_out, y.get(), D.get(), n_out, j, s);
-        }
+      launch_fill_zero<double>(D.get(), D.size(), s);
+      int       c[3] = {0, 0, 0};
+      const int cz   = gin.dim > 2 ? std::min(stride, gin.nn[2]) : 1;
+      const int cy   = std::min(stride, gin.nn[1]), cx = std::min(stride, gin.nn[0]);
+      for (c[2] = 0; c[2] < cz; ++c[2])
+        for (c[1] = 0; c[1] < cy; ++c[1])
+          for (c[0] = 0; c[0] < cx; ++c[0])
+            for (int t = 0; t < nt_in; ++t)
+              {
+                launch_probe_vector<T>(gin, nt_in, c, t, stride, x.get(), s);
+                apply(y.get(), x.get());
+                launch_probe_store<T>(gin, nt_in, gout, nt_out, c, t, stride, sn, sd, radius,
+                                      y.get(), D.get(), n_out, bad.get(), s);
+              }
       return D;
     };
     auto copy_of = [&](const DBuf<double> &a) {
@@ -1204,8 +1221,13 @@ namespace stfem
       {
         const long n = lv_[l].A->m();
         const auto &Lv = lv_[l];
-        DBuf<double> A = dense_of(n, n, [&](T *y, const T *x) { Lv.A->vmult(y, x); });
-        DBuf<double> V = dense_of(n, n, [&](T *y, const T *x) { Lv.V->vmult(y, x); });
+        // A and V couple the nodes of the cells around a node: |o - g| <= p
+        const GridDesc &gl = Lv.A->grid();
+        const int       ntl = Lv.A->n_blocks(), pl = gl.p;
+        DBuf<double> A = dense_probe(gl, ntl, gl, ntl, 2 * pl + 1, 1, 1, pl,
+                                     [&](T *y, const T *x) { Lv.A->vmult(y, x); });
+        DBuf<double> V = dense_probe(gl, ntl, gl, ntl, 2 * pl + 1, 1, 1, pl,
+                                     [&](T *y, const T *x) { Lv.V->vmult(y, x); });
         const int sweeps = steps(l) * opt_.smoothing_steps;
         // (Gs, S): s sweeps; G = I - w V A is the one-sweep iteration matrix
         auto sweep_pair = [&](double w) {
@@ -1266,14 +1288,31 @@ namespace stfem
             continue;
           }
         const TwoLevelTransfer<T> &tr = tr_[l];
-        DBuf<double> P = dense_of(n, nc, [&](T *y, const T *x) {
+        const GridDesc &gc  = lv_[l - 1].A->grid();
+        const int       ntc = lv_[l - 1].A->n_blocks();
+        auto prolong = [&](T *y, const T *x) {
           launch_fill_zero<T>(y, size_t(n), s);
           tr.prolongate_and_add(y, x);
-        });
-        DBuf<double> R = dense_of(nc, n, [&](T *y, const T *x) {
-          launch_fill_zero<T>(y, size_t(nc), s);
-          tr.restrict_and_add(y, x);
-        });
+        };
+        DBuf<double> P, R;
+        if (tr.time)
+          {
+            // time transfers act node by node
+            P = dense_probe(gc, ntc, gl, ntl, 1, 1, 1, 0, prolong);
+            R = dense_probe(gl, ntl, gc, ntc, 1, 1, 1, 0, [&](T *y, const T *x) {
+              launch_fill_zero<T>(y, size_t(nc), s);
+              tr.restrict_and_add(y, x);
+            });
+          }
+        else
+          {
+            // h: a coarse node reaches the fine nodes of its coarse cells,
+            // |o - 2 g| <= 2p; the restriction is the transpose (the same 1D
+            // weights, gmg.h:371-380)
+            P = dense_probe(gc, ntc, gl, ntl, 2 * pl + 1, 2, 1, 2 * pl, prolong);
+            R = DBuf<double>(size_t(nc) * n);
+            launch_transpose(n, nc, P.get(), R.get(), s);
+          }
         DBuf<double> X = identity(n); // I - A S
         blas.gemm(n, n, n, -1.0, A.get(), S.get(), 1.0, X.get());
         DBuf<double> Y(size_t(nc) * n); // R X
@@ -1288,6 +1327,10 @@ namespace stfem
         Mc = std::move(M);
         nc = n;
       }
+    int hbad = 0;
+    STFEM_HIP(hipMemcpyAsync(&hbad, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    ctx_.sync();
+    STFEM_REQUIRE(hbad == 0, "dense coarse operator: probed columns overlap");
     const long vw = long(16 / sizeof(T));
     dense_ld_     = (nc + vw - 1) / vw * vw;
     dense_rows_.alloc(size_t(nc) * dense_ld_);
