@@ -354,8 +354,8 @@ namespace stfem
       //     D_y^T H_y + D_z^T H_z (summed on the x lines), then the
       //     transposed interpolation along x, y, z (result on z lines)
       const long L    = long(nc0) * N;
-      const long cs_c = long(N) * A.nt_out * L; // cell_slot stride of the local z index
-      T *const   dstl = cells + ((((long(cz) * nc1 + cy) * N) * N + t1) * A.nt_out) * L + long(cx) * N + t0;
+      const long cs_c = long(N) * L * A.nt_out; // cell_slot stride of the local z index
+      T *const   dstl = cells + ((((long(cz) * nc1 + cy) * N) * N + t1) * L + long(cx) * N + t0) * A.nt_out;
       for (int j = j0; j < jn; ++j)
         {
           T V[N], U[N];
@@ -443,7 +443,7 @@ namespace stfem
             {
 #pragma unroll
               for (int k = 0; k < N; ++k)
-                dstl[k * cs_c + long(j) * L] = a[k];
+                dstl[k * cs_c + j] = a[k];
             }
         }
     }

@@ -60,7 +60,11 @@ namespace stfem
                    hipStream_t s)
     {
       const dim3 grid{(unsigned)((g.n_nodes + 255) / 256), 1, 1};
-      switch (nto)
+      // the compile-time forms access a node's components as vectors: node
+      // vectors not 16-byte aligned take the run-time form
+      const bool aligned = reinterpret_cast<uintptr_t>(dst) % 16 == 0 &&
+                           reinterpret_cast<uintptr_t>(rhs) % 16 == 0;
+      switch (aligned ? nto : 0)
         {
           case 1: k_gather_nodes<DIM, N, 1, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, 1, cells, dst, rhs, mode); break;
           case 2: k_gather_nodes<DIM, N, 2, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, 2, cells, dst, rhs, mode); break;
