@@ -33,6 +33,18 @@ namespace stfem
     __constant__ float  cDf[MAX_N + 1][MAX_N][MAX_N];
     __constant__ float  cWf[MAX_N + 1][MAX_N];
     __constant__ float  cXf[MAX_N + 1][MAX_N];
+    // Even-odd forms of the four 1D kernels (0: S, 1: D, 2: S^T, 3: D^T),
+    // matrices with A[q][c] = sigma A[N-1-q][N-1-c] (sigma = +1 for S,
+    // -1 for D: both point sets are symmetric about the cell centre).
+    // With h = N/2, e_c = x_c + x_{N-1-c}, o_c = x_c - x_{N-1-c} (c < h) and
+    // m = x_h (odd N): y_q = P_q + Q_q and y_{N-1-q} = sigma (P_q - Q_q) for
+    // q < h, P_q = sum_c E[q][c] e_c + M[q] m, Q_q = sum_c O[q][c] o_c,
+    // E/O = (A[q][c] +/- A[q][N-1-c]) / 2, M[q] = A[q][h]; the middle row
+    // y_h = sum_c R[c] (e_c or, for sigma = -1, o_c) + Rh m (sigma = +1).
+    // Layout per matrix: E (h x h), O (h x h), M (h), R (h), Rh.
+    constexpr int EO_LEN = 2 * (MAX_N / 2) * (MAX_N / 2) + 2 * (MAX_N / 2) + 1;
+    __constant__ double cEOd[MAX_N + 1][4][EO_LEN];
+    __constant__ float  cEOf[MAX_N + 1][4][EO_LEN];
 
     template <typename T>
     struct Tab;
@@ -43,6 +55,7 @@ namespace stfem
       static __device__ __forceinline__ double D(int n, int q, int a) { return cDd[n][q][a]; }
       static __device__ __forceinline__ double W(int n, int q) { return cWd[n][q]; }
       static __device__ __forceinline__ double X(int n, int q) { return cXd[n][q]; }
+      static __device__ __forceinline__ double EO(int n, int mat, int k) { return cEOd[n][mat][k]; }
     };
     template <>
     struct Tab<float>
@@ -51,6 +64,7 @@ namespace stfem
       static __device__ __forceinline__ float D(int n, int q, int a) { return cDf[n][q][a]; }
       static __device__ __forceinline__ float W(int n, int q) { return cWf[n][q]; }
       static __device__ __forceinline__ float X(int n, int q) { return cXf[n][q]; }
+      static __device__ __forceinline__ float EO(int n, int mat, int k) { return cEOf[n][mat][k]; }
     };
 
     template <int DIM, int N>
@@ -627,6 +641,43 @@ namespace stfem
               Df[q][a] = float(D[q * N + a]);
             }
         }
+      // even-odd coefficients of S, D, S^T, D^T
+      double eod[4][EO_LEN] = {}, sd[MAX_N][MAX_N], dd[MAX_N][MAX_N];
+      float  eof[4][EO_LEN] = {};
+      const int h = N / 2;
+      for (int mat = 0; mat < 4; ++mat)
+        {
+          for (int q = 0; q < N; ++q)
+            for (int c = 0; c < N; ++c)
+              {
+                const double *src = mat % 2 == 0 ? S : D;
+                // A[q][c]: the matrix (S, D) or its transpose
+                const double v = mat < 2 ? src[q * N + c] : src[c * N + q];
+                (mat % 2 == 0 ? sd : dd)[q][c] = v;
+              }
+          const double(*A)[MAX_N] = mat % 2 == 0 ? sd : dd;
+          double *e = eod[mat];
+          for (int q = 0; q < h; ++q)
+            for (int c = 0; c < h; ++c)
+              {
+                e[q * h + c]         = 0.5 * (A[q][c] + A[q][N - 1 - c]);
+                e[h * h + q * h + c] = 0.5 * (A[q][c] - A[q][N - 1 - c]);
+              }
+          if (N % 2)
+            {
+              for (int q = 0; q < h; ++q)
+                e[2 * h * h + q] = A[q][h];
+              for (int c = 0; c < h; ++c)
+                e[2 * h * h + h + c] = A[h][c];
+              e[2 * h * h + 2 * h] = A[h][h];
+            }
+          for (int k = 0; k < EO_LEN; ++k)
+            eof[mat][k] = float(e[k]);
+        }
+      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cEOd), eod, sizeof(eod), size_t(N) * sizeof(eod),
+                                  hipMemcpyHostToDevice));
+      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cEOf), eof, sizeof(eof), size_t(N) * sizeof(eof),
+                                  hipMemcpyHostToDevice));
       const size_t o2 = size_t(N) * sizeof(Sd), o1 = size_t(N) * sizeof(Wd);
       const size_t f2 = size_t(N) * sizeof(Sf), f1 = size_t(N) * sizeof(Wf);
       STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cSd), Sd, sizeof(Sd), o2, hipMemcpyHostToDevice));

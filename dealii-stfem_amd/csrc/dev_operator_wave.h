@@ -114,60 +114,89 @@ namespace stfem
       }
     };
 
-    // 1D kernels on a line (Gauss points q, nodes c):
-    //   interp  out_q = sum_c S(q,c) in_c      interp_t  out_c = sum_q S(q,c) in_q
-    //   deriv   out_q = sum_m D(q,m) in_m      deriv_t   out_m = sum_q D(q,m) in_q
-    template <int N, typename T>
-    __device__ __forceinline__ void interp(const T (&in)[N], T (&out)[N])
+    // y = A x for one of the four 1D kernels on a line (Gauss points q,
+    // nodes c):  MAT 0 interp    y_q = sum_c S(q,c) x_c
+    //            MAT 1 deriv     y_q = sum_m D(q,m) x_m
+    //            MAT 2 interp_t  y_c = sum_q S(q,c) x_q
+    //            MAT 3 deriv_t   y_m = sum_q D(q,m) x_q
+    // N^2 FMAs, or for N >= 5 the even-odd form (N = 5: 21 operations
+    // instead of 25, N = 6: 30 instead of 36).
+    template <int N, int MAT, typename T>
+    __device__ __forceinline__ void kernel1d(const T (&x)[N], T (&y)[N])
     {
-#pragma unroll
-      for (int q = 0; q < N; ++q)
+      using Tb = Tab<T>;
+      if constexpr (N >= 5)
         {
-          T s = Tab<T>::S(N, q, 0) * in[0];
+          constexpr int  h     = N / 2;
+          constexpr bool sym   = MAT % 2 == 0; // sigma = +1 for S, S^T
+          T              e[h], o[h];
 #pragma unroll
-          for (int c = 1; c < N; ++c)
-            s += Tab<T>::S(N, q, c) * in[c];
-          out[q] = s;
+          for (int c = 0; c < h; ++c)
+            {
+              e[c] = x[c] + x[N - 1 - c];
+              o[c] = x[c] - x[N - 1 - c];
+            }
+#pragma unroll
+          for (int q = 0; q < h; ++q)
+            {
+              T P = Tb::EO(N, MAT, q * h) * e[0];
+              T Q = Tb::EO(N, MAT, h * h + q * h) * o[0];
+#pragma unroll
+              for (int c = 1; c < h; ++c)
+                {
+                  P += Tb::EO(N, MAT, q * h + c) * e[c];
+                  Q += Tb::EO(N, MAT, h * h + q * h + c) * o[c];
+                }
+              if constexpr (N % 2)
+                P += Tb::EO(N, MAT, 2 * h * h + q) * x[h];
+              y[q]         = P + Q;
+              y[N - 1 - q] = sym ? P - Q : Q - P;
+            }
+          if constexpr (N % 2)
+            {
+              T m = sym ? Tb::EO(N, MAT, 2 * h * h + 2 * h) * x[h] : T(0);
+#pragma unroll
+              for (int c = 0; c < h; ++c)
+                m += Tb::EO(N, MAT, 2 * h * h + h + c) * (sym ? e[c] : o[c]);
+              y[h] = m;
+            }
+        }
+      else
+        {
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            {
+              auto A = [&](int c) {
+                return MAT == 0 ? Tb::S(N, q, c) : MAT == 1 ? Tb::D(N, q, c)
+                                                 : MAT == 2 ? Tb::S(N, c, q) : Tb::D(N, c, q);
+              };
+              T s = A(0) * x[0];
+#pragma unroll
+              for (int c = 1; c < N; ++c)
+                s += A(c) * x[c];
+              y[q] = s;
+            }
         }
     }
     template <int N, typename T>
-    __device__ __forceinline__ void interp_t(const T (&in)[N], T (&out)[N])
+    __device__ __forceinline__ void interp(const T (&in)[N], T (&out)[N])
     {
-#pragma unroll
-      for (int c = 0; c < N; ++c)
-        {
-          T s = Tab<T>::S(N, 0, c) * in[0];
-#pragma unroll
-          for (int q = 1; q < N; ++q)
-            s += Tab<T>::S(N, q, c) * in[q];
-          out[c] = s;
-        }
+      kernel1d<N, 0>(in, out);
     }
     template <int N, typename T>
     __device__ __forceinline__ void deriv(const T (&in)[N], T (&out)[N])
     {
-#pragma unroll
-      for (int q = 0; q < N; ++q)
-        {
-          T s = Tab<T>::D(N, q, 0) * in[0];
-#pragma unroll
-          for (int m = 1; m < N; ++m)
-            s += Tab<T>::D(N, q, m) * in[m];
-          out[q] = s;
-        }
+      kernel1d<N, 1>(in, out);
+    }
+    template <int N, typename T>
+    __device__ __forceinline__ void interp_t(const T (&in)[N], T (&out)[N])
+    {
+      kernel1d<N, 2>(in, out);
     }
     template <int N, typename T>
     __device__ __forceinline__ void deriv_t(const T (&in)[N], T (&out)[N])
     {
-#pragma unroll
-      for (int m = 0; m < N; ++m)
-        {
-          T s = Tab<T>::D(N, 0, m) * in[0];
-#pragma unroll
-          for (int q = 1; q < N; ++q)
-            s += Tab<T>::D(N, q, m) * in[q];
-          out[m] = s;
-        }
+      kernel1d<N, 3>(in, out);
     }
 
     // Q1 Jacobian J[d][e] = dx_d / dxi_e of the cell with vertex
