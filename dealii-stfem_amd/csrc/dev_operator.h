@@ -33,9 +33,9 @@ namespace stfem
     __constant__ float  cDf[MAX_N + 1][MAX_N][MAX_N];
     __constant__ float  cWf[MAX_N + 1][MAX_N];
     __constant__ float  cXf[MAX_N + 1][MAX_N];
-    // Even-odd forms of the four 1D kernels (0: S, 1: D, 2: S^T, 3: D^T),
-    // matrices with A[q][c] = sigma A[N-1-q][N-1-c] (sigma = +1 for S,
-    // -1 for D: both point sets are symmetric about the cell centre).
+    // Even-odd forms of the four 1D kernels (0: S, 1: D, 2: S^T, 3: D^T) in
+    // FP64, matrices with A[q][c] = sigma A[N-1-q][N-1-c] (sigma = +1 for
+    // S, -1 for D: both point sets are symmetric about the cell centre).
     // With h = N/2, e_c = x_c + x_{N-1-c}, o_c = x_c - x_{N-1-c} (c < h) and
     // m = x_h (odd N): y_q = P_q + Q_q and y_{N-1-q} = sigma (P_q - Q_q) for
     // q < h, P_q = sum_c E[q][c] e_c + M[q] m, Q_q = sum_c O[q][c] o_c,
@@ -44,7 +44,6 @@ namespace stfem
     // Layout per matrix: E (h x h), O (h x h), M (h), R (h), Rh.
     constexpr int EO_LEN = 2 * (MAX_N / 2) * (MAX_N / 2) + 2 * (MAX_N / 2) + 1;
     __constant__ double cEOd[MAX_N + 1][4][EO_LEN];
-    __constant__ float  cEOf[MAX_N + 1][4][EO_LEN];
 
     template <typename T>
     struct Tab;
@@ -64,7 +63,6 @@ namespace stfem
       static __device__ __forceinline__ float D(int n, int q, int a) { return cDf[n][q][a]; }
       static __device__ __forceinline__ float W(int n, int q) { return cWf[n][q]; }
       static __device__ __forceinline__ float X(int n, int q) { return cXf[n][q]; }
-      static __device__ __forceinline__ float EO(int n, int mat, int k) { return cEOf[n][mat][k]; }
     };
 
     template <int DIM, int N>
@@ -643,7 +641,6 @@ namespace stfem
         }
       // even-odd coefficients of S, D, S^T, D^T
       double eod[4][EO_LEN] = {}, sd[MAX_N][MAX_N], dd[MAX_N][MAX_N];
-      float  eof[4][EO_LEN] = {};
       const int h = N / 2;
       for (int mat = 0; mat < 4; ++mat)
         {
@@ -671,12 +668,8 @@ namespace stfem
                 e[2 * h * h + h + c] = A[h][c];
               e[2 * h * h + 2 * h] = A[h][h];
             }
-          for (int k = 0; k < EO_LEN; ++k)
-            eof[mat][k] = float(e[k]);
         }
       STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cEOd), eod, sizeof(eod), size_t(N) * sizeof(eod),
-                                  hipMemcpyHostToDevice));
-      STFEM_HIP(hipMemcpyToSymbol(HIP_SYMBOL(cEOf), eof, sizeof(eof), size_t(N) * sizeof(eof),
                                   hipMemcpyHostToDevice));
       const size_t o2 = size_t(N) * sizeof(Sd), o1 = size_t(N) * sizeof(Wd);
       const size_t f2 = size_t(N) * sizeof(Sf), f1 = size_t(N) * sizeof(Wf);

@@ -119,13 +119,16 @@ namespace stfem
     //            MAT 1 deriv     y_q = sum_m D(q,m) x_m
     //            MAT 2 interp_t  y_c = sum_q S(q,c) x_q
     //            MAT 3 deriv_t   y_m = sum_q D(q,m) x_q
-    // N^2 FMAs, or for N >= 5 the even-odd form (N = 5: 21 operations
-    // instead of 25, N = 6: 30 instead of 36).
+    // N^2 FMAs, or in FP64 for N >= 5 the even-odd form (N = 5: 21
+    // operations instead of 25, N = 6: 30 instead of 36; measured at 64^3
+    // Q4: FP64 cell pass 6 % faster, FP32 - VALU-bound at four waves per
+    // SIMD, where the N independent FMA chains of the plain form issue
+    // back to back - 3 % slower, so FP32 keeps the plain form).
     template <int N, int MAT, typename T>
     __device__ __forceinline__ void kernel1d(const T (&x)[N], T (&y)[N])
     {
       using Tb = Tab<T>;
-      if constexpr (N >= 5)
+      if constexpr (N >= 5 && sizeof(T) == 8)
         {
           constexpr int  h     = N / 2;
           constexpr bool sym   = MAT % 2 == 0; // sigma = +1 for S, S^T
