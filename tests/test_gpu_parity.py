@@ -361,6 +361,36 @@ def test_transfer_timing_and_bytes_model():
     assert g.time_kernel(5, L, warmup=1, reps=2) > 0
 
 
+@pytest.mark.parametrize("name", ["3d_q3_dg2", "3d_q4_dg3_fp32_factored"])
+def test_node_gather_unaligned_vectors(name):
+    """The node gathers access a node's time components as vectors when the
+    node vectors are 16-byte aligned and fall back to scalar accesses
+    otherwise: a destination offset by one value gives the same bits (the
+    operator, and the factored Vanka's two-pass form)."""
+    spec, o, g = problems(name)
+    rng = np.random.default_rng(29)
+    dt = dtype_of(g)
+    for l in (g.n_levels - 1, max(0, g.n_levels - 2)):
+        info = g.level_info(l)
+        m = info["n_nodes"] * info["nt"]
+        x = torch.tensor(rng.standard_normal(m), dtype=dt, device=DEV)
+        y = torch.zeros(m, dtype=dt, device=DEV)
+        buf = torch.zeros(m + 1, dtype=dt, device=DEV)
+        print(f"[case {name}] level {l} unaligned gather", flush=True)
+        g.vmult(l, y, x)
+        g.vmult(l, buf[1:], x)
+        assert buf[1:].data_ptr() % 16 != 0
+        assert torch.equal(y, buf[1:]), l
+        g.vanka(l, y, x)
+        g.vanka(l, buf[1:], x)
+        if spec.get("vanka_factored"):
+            assert torch.equal(y, buf[1:]), l
+        else:
+            # explicit inverses add their corrections with atomics: equal
+            # up to the order of summation
+            assert rel_err(buf[1:].cpu().numpy(), y.cpu().numpy()) < tol_of(g), l
+
+
 def test_vanka_sweep_kernel_timing():
     """The sweep kernel of the factored Vanka can be timed alone (selector 6:
     the cell pass without the node gather), and a level whose Vanka has no
