@@ -5,7 +5,10 @@
 // B_c = Alpha (x) K_c + Beta (x) M_c; these kernels produce the eigenpairs
 // that factor B_c^{-1} (the factored form of the smoother).
 //
-// Four launches per chunk of patches, each patch matrix held in LDS:
+// Four launches per chunk of patches, each patch matrix held in LDS (one
+// patch per CU; 1024 threads per workgroup, so that the matrix-vector
+// products and rank-one updates of the sequential steps are spread over
+// sixteen wavefronts):
 //   1. k_eig_cholesky       one workgroup per patch: M = L L^T (right-looking,
 //                           pivots checked), L written back symmetric;
 //   2. k_eig_tridiag        one workgroup per patch: C = L^{-1} K L^{-T} by
@@ -31,6 +34,14 @@ namespace stfem
   namespace
   {
     constexpr int EIG_MAXN = 128;
+    // One workgroup of EIG_T threads per patch (the patch matrix fills most
+    // of a CU's LDS, so a CU holds one patch; the threads of the 2D loops
+    // form an EIG_TI x EIG_TJ grid, the matrix-vector products EIG_P
+    // partial sums per row)
+    constexpr int EIG_T  = 1024;
+    constexpr int EIG_TJ = 32;
+    constexpr int EIG_TI = EIG_T / EIG_TJ;
+    constexpr int EIG_P  = EIG_T / EIG_MAXN;
 
     __device__ __forceinline__ double wave_sum(double s)
     {
@@ -43,13 +54,13 @@ namespace stfem
     // M = L L^T of the patch mass matrix; on exit Mp holds L in both
     // triangles (Mp[i + j n] = L[max(i,j)][min(i,j)]).  info = 1 for a
     // non-positive pivot.
-    __global__ void __launch_bounds__(256)
+    __global__ void __launch_bounds__(EIG_T)
       k_eig_cholesky(int n, double *__restrict__ Mp, int *__restrict__ info)
     {
       extern __shared__ double A[]; // n x n, row-major
       const size_t base = size_t(blockIdx.x) * n * n;
-      const int    tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
-      for (int e = tid; e < n * n; e += 256)
+      const int    tid = threadIdx.x, ti = tid / EIG_TJ, tj = tid % EIG_TJ;
+      for (int e = tid; e < n * n; e += EIG_T)
         A[e] = Mp[base + e];
       // right-looking elimination of the upper triangle: after step k, row
       // k holds d_k L[j][k] (j > k) with the pivot d_k = L[k][k]^2
@@ -64,16 +75,16 @@ namespace stfem
               return; // uniform: every thread read the same pivot
             }
           const double inv = 1.0 / dk;
-          for (int i = k + 1 + ti; i < n; i += 16)
+          for (int i = k + 1 + ti; i < n; i += EIG_TI)
             {
               const double f = A[k * n + i] * inv;
-              for (int j = k + 1 + tj; j < n; j += 16)
+              for (int j = k + 1 + tj; j < n; j += EIG_TJ)
                 if (j >= i)
                   A[i * n + j] -= f * A[k * n + j];
             }
         }
       __syncthreads();
-      for (int e = tid; e < n * n; e += 256)
+      for (int e = tid; e < n * n; e += EIG_T)
         {
           const int j = e / n, i = e - j * n;
           const int lo = i < j ? i : j, hi = i < j ? j : i;
@@ -86,19 +97,19 @@ namespace stfem
     // C = L^{-1} K L^{-T}, then Householder tridiagonalization of C:
     // d[k] = T[k][k], e[k] = T[k][k+1] (e[n-1] = 0), h[k] and v_k in row k
     // of Kp (columns k+1 .. n-1); h[k] = 0 marks an identity step.
-    __global__ void __launch_bounds__(256)
+    __global__ void __launch_bounds__(EIG_T)
       k_eig_tridiag(int n, const double *__restrict__ Lm, double *__restrict__ Kp,
                     double *__restrict__ dd, double *__restrict__ ee, double *__restrict__ hh)
     {
-      extern __shared__ double A[]; // n x n row-major, v[128], p[2][128], lc[2][128]
+      extern __shared__ double A[]; // n x n row-major, v[128], p[EIG_P][128], lc[2][128]
       double *const v  = A + n * n;
       double *const p  = v + EIG_MAXN;
-      double *const lc = p + 2 * EIG_MAXN; // columns k, k+1 of L
+      double *const lc = p + EIG_P * EIG_MAXN; // columns k, k+1 of L
       __shared__ double sc[3];
       const size_t base = size_t(blockIdx.x) * n * n, vb = size_t(blockIdx.x) * n;
       const double *L   = Lm + base; // L(i, k) = L[i + k n] (symmetric storage)
-      const int    tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
-      for (int e = tid; e < n * n; e += 256)
+      const int    tid = threadIdx.x, ti = tid / EIG_TJ, tj = tid % EIG_TJ;
+      for (int e = tid; e < n * n; e += EIG_T)
         A[e] = Kp[base + e];
       if (tid < n)
         lc[tid] = L[tid];
@@ -111,20 +122,20 @@ namespace stfem
           const bool    nxt = k + 1 < n && tid < n;
           const double  ln  = nxt ? L[tid + (k + 1) * n] : 0.0;
           const double  rk  = 1.0 / lk[k];
-          for (int i = k + 1 + ti; i < n; i += 16)
+          for (int i = k + 1 + ti; i < n; i += EIG_TI)
             {
               const double f = lk[i] * rk;
-              for (int j = tj; j < n; j += 16)
+              for (int j = tj; j < n; j += EIG_TJ)
                 A[i * n + j] -= f * A[k * n + j];
             }
           if (nxt)
             lc[((k + 1) & 1) * EIG_MAXN + tid] = ln;
         }
       __syncthreads();
-      for (int i = ti; i < n; i += 16)
+      for (int i = ti; i < n; i += EIG_TI)
         {
           const double rk = 1.0 / L[i + i * n];
-          for (int j = tj; j < n; j += 16)
+          for (int j = tj; j < n; j += EIG_TJ)
             A[i * n + j] *= rk;
         }
       __syncthreads();
@@ -138,26 +149,26 @@ namespace stfem
           const bool    nxt = k + 1 < n && tid < n;
           const double  ln  = nxt ? L[tid + (k + 1) * n] : 0.0;
           const double  rk  = 1.0 / lk[k];
-          for (int j = k + 1 + tj; j < n; j += 16)
+          for (int j = k + 1 + tj; j < n; j += EIG_TJ)
             {
               const double f = lk[j] * rk;
-              for (int i = ti; i < n; i += 16)
+              for (int i = ti; i < n; i += EIG_TI)
                 A[i * n + j] -= A[i * n + k] * f;
             }
           if (nxt)
             lc[((k + 1) & 1) * EIG_MAXN + tid] = ln;
         }
       __syncthreads();
-      for (int j = tj; j < n; j += 16)
+      for (int j = tj; j < n; j += EIG_TJ)
         {
           const double rk = 1.0 / L[j + j * n];
-          for (int i = ti; i < n; i += 16)
+          for (int i = ti; i < n; i += EIG_TI)
             A[i * n + j] *= rk;
         }
       __syncthreads();
       // exact symmetry
-      for (int i = ti; i < n; i += 16)
-        for (int j = tj; j < n; j += 16)
+      for (int i = ti; i < n; i += EIG_TI)
+        for (int j = tj; j < n; j += EIG_TJ)
           if (j > i)
             {
               const double s = 0.5 * (A[i * n + j] + A[j * n + i]);
@@ -199,22 +210,23 @@ namespace stfem
           const double h = sc[0];
           if (h == 0.0)
             continue; // uniform
-          for (int j = tid; j < m; j += 256)
+          for (int j = tid; j < m; j += EIG_T)
             {
               const double vj = j ? A[k * n + o + j] : sc[2];
               v[j]            = vj;
               Kp[base + size_t(k) * n + o + j] = vj;
             }
           __syncthreads();
-          // p = C' v / h, two half sums per row (column reads: C' symmetric)
+          // p = C' v / h, EIG_P partial sums per row (column reads: C'
+          // symmetric)
           {
-            const int i = tid & (EIG_MAXN - 1), half = tid >> 7;
+            const int i = tid % EIG_MAXN, part = tid / EIG_MAXN;
             if (i < m)
               {
                 double s = 0;
-                for (int j = half; j < m; j += 2)
+                for (int j = part; j < m; j += EIG_P)
                   s += A[(o + j) * n + o + i] * v[j];
-                p[half * EIG_MAXN + i] = s;
+                p[part * EIG_MAXN + i] = s;
               }
           }
           __syncthreads();
@@ -224,7 +236,11 @@ namespace stfem
               double s = 0;
               for (int i = tid; i < m; i += 64)
                 {
-                  const double pi = (p[i] + p[EIG_MAXN + i]) / h;
+                  double pi = p[i];
+#pragma unroll
+                  for (int q = 1; q < EIG_P; ++q)
+                    pi += p[q * EIG_MAXN + i];
+                  pi /= h;
                   p[i]            = pi;
                   s += pi * v[i];
                 }
@@ -233,10 +249,10 @@ namespace stfem
                 p[i] -= kc * v[i];
             }
           __syncthreads();
-          for (int i = ti; i < m; i += 16)
+          for (int i = ti; i < m; i += EIG_TI)
             {
               const double vi = v[i], wi = p[i];
-              for (int j = tj; j < m; j += 16)
+              for (int j = tj; j < m; j += EIG_TJ)
                 A[(o + i) * n + o + j] -= vi * p[j] + wi * v[j];
             }
         }
@@ -355,24 +371,24 @@ namespace stfem
     // reads one column and writes one; the (cos, sin) are held 64 per lane
     // register (this block and the next one in flight) and broadcast with
     // readlane.
-    __global__ void __launch_bounds__(256)
+    __global__ void __launch_bounds__(EIG_T)
       k_eig_vectors(int n, const double *__restrict__ Lm, double *__restrict__ Kp,
                     const double *__restrict__ hh, long cap, int ccap,
                     const double2 *__restrict__ rot, const int2 *__restrict__ chains,
                     const int *__restrict__ cnt_in)
     {
-      // n x n row-major, v[2][128], w[2][128], lc[2][128], hs[128]
+      // n x n row-major, v[2][128], w[EIG_P][128], lc[2][128], hs[128]
       extern __shared__ double Z[];
       double *const v  = Z + n * n;
       double *const w  = v + 2 * EIG_MAXN;
-      double *const lc = w + 2 * EIG_MAXN;
+      double *const lc = w + EIG_P * EIG_MAXN;
       double *const hs = lc + 2 * EIG_MAXN;
       const size_t  base = size_t(blockIdx.x) * n * n, vb = size_t(blockIdx.x) * n;
       const double *L    = Lm + base;
-      const int     tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
-      for (int e = tid; e < n * n; e += 256)
+      const int     tid = threadIdx.x, ti = tid / EIG_TJ, tj = tid % EIG_TJ;
+      for (int e = tid; e < n * n; e += EIG_T)
         Z[e] = (e % (n + 1) == 0) ? 1.0 : 0.0;
-      for (int k = tid; k + 2 < n; k += 256)
+      for (int k = tid; k + 2 < n; k += EIG_T)
         hs[k] = hh[vb + k];
       if (n >= 3 && tid < 2)
         v[((n - 3) & 1) * EIG_MAXN + tid] = Kp[base + size_t(n - 3) * n + n - 2 + tid];
@@ -477,21 +493,30 @@ namespace stfem
           if (h != 0.0) // uniform
             {
               {
-                const int c = tid & (EIG_MAXN - 1), half = tid >> 7;
+                const int c = tid % EIG_MAXN, part = tid / EIG_MAXN;
                 if (c < n)
                   {
                     double s = 0;
-                    for (int i = half; i < m; i += 2)
+                    for (int i = part; i < m; i += EIG_P)
                       s += vk[i] * Z[(o + i) * n + c];
-                    w[half * EIG_MAXN + c] = s;
+                    w[part * EIG_MAXN + c] = s;
                   }
               }
               __syncthreads();
-              for (int i = ti; i < m; i += 16)
+              if (tid < n)
+                {
+                  double s = w[tid];
+#pragma unroll
+                  for (int q = 1; q < EIG_P; ++q)
+                    s += w[q * EIG_MAXN + tid];
+                  w[tid] = s;
+                }
+              __syncthreads();
+              for (int i = ti; i < m; i += EIG_TI)
                 {
                   const double vi = vk[i] / h;
-                  for (int c = tj; c < n; c += 16)
-                    Z[(o + i) * n + c] -= vi * (w[c] + w[EIG_MAXN + c]);
+                  for (int c = tj; c < n; c += EIG_TJ)
+                    Z[(o + i) * n + c] -= vi * w[c];
                 }
             }
           if (nx)
@@ -509,17 +534,17 @@ namespace stfem
           const bool    nxt = k - 1 >= 1 && tid < n;
           const double  ln  = nxt ? L[tid + (k - 1) * n] : 0.0;
           const double  rk  = 1.0 / lk[k];
-          for (int i = ti; i < k; i += 16)
+          for (int i = ti; i < k; i += EIG_TI)
             {
               const double f = lk[i] * rk; // L[k][i]
-              for (int c = tj; c < n; c += 16)
+              for (int c = tj; c < n; c += EIG_TJ)
                 Z[i * n + c] -= f * Z[k * n + c];
             }
           if (nxt)
             lc[((k - 1) & 1) * EIG_MAXN + tid] = ln;
         }
       __syncthreads();
-      for (int e = tid; e < n * n; e += 256)
+      for (int e = tid; e < n * n; e += EIG_T)
         {
           const int c = e / n, r = e - c * n; // column-major output position
           Kp[base + e] = Z[r * n + c] / L[r + r * n];
@@ -529,7 +554,7 @@ namespace stfem
 
   size_t patch_eigen_lds(int n)
   {
-    return sizeof(double) * (size_t(n) * n + 7 * EIG_MAXN);
+    return sizeof(double) * (size_t(n) * n + (5 + EIG_P) * EIG_MAXN);
   }
 
   // implicit QL needs about n^2 rotations per patch (0.7 - 1.3 n^2 measured
@@ -576,14 +601,14 @@ namespace stfem
     }
     const long cap = patch_eigen_rotation_cap(n);
     const dim3 grid{(unsigned)np, 1, 1};
-    k_eig_cholesky<<<grid, 256, sizeof(double) * size_t(n) * n, s>>>(n, Mp, info);
+    k_eig_cholesky<<<grid, EIG_T, sizeof(double) * size_t(n) * n, s>>>(n, Mp, info);
     STFEM_HIP(hipGetLastError());
-    k_eig_tridiag<<<grid, 256, lds, s>>>(n, Mp, Kp, lam, ee, hh);
+    k_eig_tridiag<<<grid, EIG_T, lds, s>>>(n, Mp, Kp, lam, ee, hh);
     STFEM_HIP(hipGetLastError());
     const int  ccap = patch_eigen_chain_cap(n);
     k_eig_ql<<<grid, 64, 0, s>>>(n, lam, ee, cap, ccap, rot, chains, counts, info);
     STFEM_HIP(hipGetLastError());
-    k_eig_vectors<<<grid, 256, lds, s>>>(n, Mp, Kp, hh, cap, ccap, rot, chains, counts);
+    k_eig_vectors<<<grid, EIG_T, lds, s>>>(n, Mp, Kp, hh, cap, ccap, rot, chains, counts);
     STFEM_HIP(hipGetLastError());
   }
 } // namespace stfem
