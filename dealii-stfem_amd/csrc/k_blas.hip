@@ -306,13 +306,15 @@ namespace stfem
 
     inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
-    __global__ void k_add_diagonal(long n, double a, double *M)
+    template <typename C>
+    __global__ void k_add_diagonal(long n, C a, C *M)
     {
       const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
       if (i < n)
         M[i + i * n] += a;
     }
-    __global__ void k_scale(long n, double a, double *x)
+    template <typename C>
+    __global__ void k_scale(long n, C a, C *x)
     {
       const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
       if (i < n)
@@ -346,10 +348,10 @@ namespace stfem
           x[i] = hit ? T(1) : T(0);
         }
     }
-    template <typename T>
+    template <typename T, typename C>
     __global__ void k_probe_store(const GridDesc in, int nt_in, const GridDesc out, int nt_out,
                                   ProbeColour c, int t, int stride, int sn, int sd, int radius,
-                                  const T *__restrict__ y, double *__restrict__ M, long ld,
+                                  const T *__restrict__ y, C *__restrict__ M, long ld,
                                   int *__restrict__ bad)
     {
       const long n = out.n_nodes * nt_out;
@@ -381,20 +383,21 @@ namespace stfem
               mul *= in.nn[d];
             }
           if (ok)
-            M[(node * nt_in + t) * ld + i] = double(v);
+            M[(node * nt_in + t) * ld + i] = C(v);
           else if (v != T(0))
             atomicAdd(bad, 1);
         }
     }
-    __global__ void k_transpose(long m, long n, const double *__restrict__ A, double *__restrict__ B)
+    template <typename C>
+    __global__ void k_transpose(long m, long n, const C *__restrict__ A, C *__restrict__ B)
     {
-      __shared__ double tile[32][33];
+      __shared__ C tile[32][33];
       const long i0 = long(blockIdx.x) * 32, j0 = long(blockIdx.y) * 32;
       const int  tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 32 x 8
       for (int k = ty; k < 32; k += 8)
         {
           const long i = i0 + tx, j = j0 + k; // A(i, j), i < m, j < n
-          tile[k][tx] = (i < m && j < n) ? A[i + j * m] : 0.0;
+          tile[k][tx] = (i < m && j < n) ? A[i + j * m] : C(0);
         }
       __syncthreads();
       for (int k = ty; k < 32; k += 8)
@@ -404,8 +407,8 @@ namespace stfem
             B[j + i * n] = tile[tx][k];
         }
     }
-    template <typename T>
-    __global__ void k_pack_rows(long n, const double *__restrict__ M, T *__restrict__ R, long ld)
+    template <typename T, typename C>
+    __global__ void k_pack_rows(long n, const C *__restrict__ M, T *__restrict__ R, long ld)
     {
       const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
       if (idx >= n * ld)
@@ -602,27 +605,29 @@ namespace stfem
     STFEM_HIP(hipGetLastError());
   }
 
-  void launch_add_diagonal(long n, double a, double *M, hipStream_t s)
+  template <typename C>
+  void launch_add_diagonal(long n, C a, C *M, hipStream_t s)
   {
     if (n == 0)
       return;
-    k_add_diagonal<<<nblk(n), 256, 0, s>>>(n, a, M);
+    k_add_diagonal<C><<<nblk(n), 256, 0, s>>>(n, a, M);
     STFEM_HIP(hipGetLastError());
   }
-  void launch_scale(long n, double a, double *x, hipStream_t s)
+  template <typename C>
+  void launch_scale(long n, C a, C *x, hipStream_t s)
   {
     if (n == 0)
       return;
-    k_scale<<<nblk(n), 256, 0, s>>>(n, a, x);
+    k_scale<C><<<nblk(n), 256, 0, s>>>(n, a, x);
     STFEM_HIP(hipGetLastError());
   }
-  template <typename T>
-  void launch_pack_rows(long n, const double *M, T *R, long ld, hipStream_t s)
+  template <typename T, typename C>
+  void launch_pack_rows(long n, const C *M, T *R, long ld, hipStream_t s)
   {
     STFEM_REQUIRE(ld >= n, "pack_rows: leading dimension below the order");
     if (n == 0)
       return;
-    k_pack_rows<T><<<nblk(n * ld), 256, 0, s>>>(n, M, R, ld);
+    k_pack_rows<T, C><<<nblk(n * ld), 256, 0, s>>>(n, M, R, ld);
     STFEM_HIP(hipGetLastError());
   }
   template <typename T>
@@ -648,10 +653,10 @@ namespace stfem
     k_probe_vector<T><<<dim3((unsigned)blocks), 256, 0, s>>>(in, nt_in, pc, t, stride, x);
     STFEM_HIP(hipGetLastError());
   }
-  template <typename T>
+  template <typename T, typename C>
   void launch_probe_store(const GridDesc &in, int nt_in, const GridDesc &out, int nt_out,
                           const int *c, int t, int stride, int scale_num, int scale_den,
-                          int radius, const T *y, double *M, long ld, int *bad, hipStream_t s)
+                          int radius, const T *y, C *M, long ld, int *bad, hipStream_t s)
   {
     STFEM_REQUIRE(in.dim == out.dim && stride >= 1 && scale_num >= 1 && scale_den >= 1 &&
                     radius >= 0 && long(stride) * scale_num > 2L * scale_den * radius,
@@ -661,31 +666,39 @@ namespace stfem
       return;
     ProbeColour pc{{c[0], in.dim > 1 ? c[1] : 0, in.dim > 2 ? c[2] : 0}};
     const long  blocks = std::min<long>((n + 255) / 256, 4096);
-    k_probe_store<T><<<dim3((unsigned)blocks), 256, 0, s>>>(in, nt_in, out, nt_out, pc, t, stride,
+    k_probe_store<T, C><<<dim3((unsigned)blocks), 256, 0, s>>>(in, nt_in, out, nt_out, pc, t, stride,
                                                             scale_num, scale_den, radius, y, M, ld,
                                                             bad);
     STFEM_HIP(hipGetLastError());
   }
-  void launch_transpose(long m, long n, const double *A, double *B, hipStream_t s)
+  template <typename C>
+  void launch_transpose(long m, long n, const C *A, C *B, hipStream_t s)
   {
     if (m == 0 || n == 0)
       return;
     const dim3 grid{(unsigned)((m + 31) / 32), (unsigned)((n + 31) / 32), 1};
-    k_transpose<<<grid, 256, 0, s>>>(m, n, A, B);
+    k_transpose<C><<<grid, 256, 0, s>>>(m, n, A, B);
     STFEM_HIP(hipGetLastError());
   }
   template void launch_probe_vector<double>(const GridDesc &, int, const int *, int, int, double *,
                                             hipStream_t);
   template void launch_probe_vector<float>(const GridDesc &, int, const int *, int, int, float *,
                                            hipStream_t);
-  template void launch_probe_store<double>(const GridDesc &, int, const GridDesc &, int,
-                                           const int *, int, int, int, int, int, const double *,
-                                           double *, long, int *, hipStream_t);
-  template void launch_probe_store<float>(const GridDesc &, int, const GridDesc &, int, const int *,
-                                          int, int, int, int, int, const float *, double *, long,
-                                          int *, hipStream_t);
-  template void launch_pack_rows<double>(long, const double *, double *, long, hipStream_t);
-  template void launch_pack_rows<float>(long, const double *, float *, long, hipStream_t);
+  template void launch_probe_store<double, double>(const GridDesc &, int, const GridDesc &, int,
+                                                   const int *, int, int, int, int, int,
+                                                   const double *, double *, long, int *,
+                                                   hipStream_t);
+  template void launch_probe_store<float, float>(const GridDesc &, int, const GridDesc &, int,
+                                                 const int *, int, int, int, int, int,
+                                                 const float *, float *, long, int *, hipStream_t);
+  template void launch_add_diagonal<double>(long, double, double *, hipStream_t);
+  template void launch_add_diagonal<float>(long, float, float *, hipStream_t);
+  template void launch_scale<double>(long, double, double *, hipStream_t);
+  template void launch_scale<float>(long, float, float *, hipStream_t);
+  template void launch_transpose<double>(long, long, const double *, double *, hipStream_t);
+  template void launch_transpose<float>(long, long, const float *, float *, hipStream_t);
+  template void launch_pack_rows<double, double>(long, const double *, double *, long, hipStream_t);
+  template void launch_pack_rows<float, float>(long, const float *, float *, long, hipStream_t);
   template void launch_gemv_rows<double>(long, long, const double *, const double *, double *,
                                          hipStream_t);
   template void launch_gemv_rows<float>(long, long, const float *, const float *, float *,

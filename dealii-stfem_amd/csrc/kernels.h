@@ -237,8 +237,10 @@ namespace stfem
 
   // ------------------------------------------------- dense coarse operator
   // M[i + i*n] += a (column-major n x n);  x[0..n) *= a
-  void launch_add_diagonal(long n, double a, double *M, hipStream_t s);
-  void launch_scale(long n, double a, double *x, hipStream_t s);
+  template <typename C>
+  void launch_add_diagonal(long n, C a, C *M, hipStream_t s);
+  template <typename C>
+  void launch_scale(long n, C a, C *x, hipStream_t s);
   // Probing of a local linear map (columns of its dense matrix in batches).
   // A colour is (c[0..dim), t): the inputs (node, t) of the grid `in` with
   // node lattice position g = c (mod stride) in every direction.
@@ -252,16 +254,17 @@ namespace stfem
   template <typename T>
   void launch_probe_vector(const GridDesc &in, int nt_in, const int *c, int t, int stride, T *x,
                            hipStream_t s);
-  template <typename T>
+  template <typename T, typename C>
   void launch_probe_store(const GridDesc &in, int nt_in, const GridDesc &out, int nt_out,
                           const int *c, int t, int stride, int scale_num, int scale_den,
-                          int radius, const T *y, double *M, long ld, int *bad, hipStream_t s);
+                          int radius, const T *y, C *M, long ld, int *bad, hipStream_t s);
   // B (n x m, column-major, leading dimension n) = A^T (A: m x n, leading m)
-  void launch_transpose(long m, long n, const double *A, double *B, hipStream_t s);
+  template <typename C>
+  void launch_transpose(long m, long n, const C *A, C *B, hipStream_t s);
   // R[i*ld + j] = T(M[i + j*n]) for i, j < n (column-major n x n FP64 into
   // row-major precision-T rows of ld >= n entries, padding zeroed)
-  template <typename T>
-  void launch_pack_rows(long n, const double *M, T *R, long ld, hipStream_t s);
+  template <typename T, typename C>
+  void launch_pack_rows(long n, const C *M, T *R, long ld, hipStream_t s);
   // y = R x for the row-major R above (one wavefront per row, FP64 sums);
   // ld must be a multiple of 16 bytes / sizeof(T)
   template <typename T>

@@ -1141,6 +1141,13 @@ namespace stfem
                             rocblas_int(n), rocblas_int(k), &alpha, A, rocblas_int(m), B,
                             rocblas_int(k), &beta, C, rocblas_int(m)));
       }
+      void gemm(long m, long n, long k, float alpha, const float *A, const float *B, float beta,
+                float *C)
+      {
+        check(rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, rocblas_int(m),
+                            rocblas_int(n), rocblas_int(k), &alpha, A, rocblas_int(m), B,
+                            rocblas_int(k), &beta, C, rocblas_int(m)));
+      }
     };
   } // namespace
 
@@ -1152,10 +1159,10 @@ namespace stfem
   // where one relaxation sweep is u <- G u + H def, G = I - w V A, H = w V,
   // s = 2^(L-l) * smoothing_steps sweeps, and S_l = sum_{k<s} G^k H is the
   // map of s sweeps from zero (one sweep with the exact-level weight on
-  // one-cell levels).  A, V, P, R are taken column by column from the level
-  // kernels applied to unit vectors, so M_lc is the V-cycle of the launched
-  // kernels up to rounding; the products run in FP64 and M_lc is stored in
-  // the level precision.
+  // one-cell levels).  A, V, P, R are the level kernels' matrices (probed
+  // with unit vectors, below), so M_lc is the V-cycle of the launched
+  // kernels up to rounding; the products run in the level precision (FP32
+  // levels: rocBLAS SGEMM, FP64 levels: DGEMM), in which M_lc is stored.
   template <typename T>
   void GMG<T>::setup_dense_coarse()
   {
@@ -1185,9 +1192,9 @@ namespace stfem
     auto dense_probe = [&](const GridDesc &gin, int nt_in, const GridDesc &gout, int nt_out,
                            int stride, int sn, int sd, int radius, auto &&apply) {
       const long   n_in = gin.n_nodes * nt_in, n_out = gout.n_nodes * nt_out;
-      DBuf<double> D(size_t(n_out) * n_in);
+      DBuf<T> D(size_t(n_out) * n_in);
       DBuf<T>      x(n_in), y(n_out);
-      launch_fill_zero<double>(D.get(), D.size(), s);
+      launch_fill_zero<T>(D.get(), D.size(), s);
       int       c[3] = {0, 0, 0};
       const int cz   = gin.dim > 2 ? std::min(stride, gin.nn[2]) : 1;
       const int cy   = std::min(stride, gin.nn[1]), cx = std::min(stride, gin.nn[0]);
@@ -1198,24 +1205,24 @@ namespace stfem
               {
                 launch_probe_vector<T>(gin, nt_in, c, t, stride, x.get(), s);
                 apply(y.get(), x.get());
-                launch_probe_store<T>(gin, nt_in, gout, nt_out, c, t, stride, sn, sd, radius,
+                launch_probe_store<T, T>(gin, nt_in, gout, nt_out, c, t, stride, sn, sd, radius,
                                       y.get(), D.get(), n_out, bad.get(), s);
               }
       return D;
     };
-    auto copy_of = [&](const DBuf<double> &a) {
-      DBuf<double> b(a.size());
-      STFEM_HIP(hipMemcpyAsync(b.get(), a.get(), a.size() * sizeof(double),
+    auto copy_of = [&](const DBuf<T> &a) {
+      DBuf<T> b(a.size());
+      STFEM_HIP(hipMemcpyAsync(b.get(), a.get(), a.size() * sizeof(T),
                                hipMemcpyDeviceToDevice, s));
       return b;
     };
     auto identity = [&](long n) {
-      DBuf<double> I(size_t(n) * n);
-      launch_fill_zero<double>(I.get(), I.size(), s);
-      launch_add_diagonal(n, 1.0, I.get(), s);
+      DBuf<T> I(size_t(n) * n);
+      launch_fill_zero<T>(I.get(), I.size(), s);
+      launch_add_diagonal<T>(n, T(1), I.get(), s);
       return I;
     };
-    DBuf<double> Mc;
+    DBuf<T> Mc;
     long         nc = 0;
     for (int l = 0; l <= lc; ++l)
       {
@@ -1224,20 +1231,20 @@ namespace stfem
         // A and V couple the nodes of the cells around a node: |o - g| <= p
         const GridDesc &gl = Lv.A->grid();
         const int       ntl = Lv.A->n_blocks(), pl = gl.p;
-        DBuf<double> A = dense_probe(gl, ntl, gl, ntl, 2 * pl + 1, 1, 1, pl,
+        DBuf<T> A = dense_probe(gl, ntl, gl, ntl, 2 * pl + 1, 1, 1, pl,
                                      [&](T *y, const T *x) { Lv.A->vmult(y, x); });
-        DBuf<double> V = dense_probe(gl, ntl, gl, ntl, 2 * pl + 1, 1, 1, pl,
+        DBuf<T> V = dense_probe(gl, ntl, gl, ntl, 2 * pl + 1, 1, 1, pl,
                                      [&](T *y, const T *x) { Lv.V->vmult(y, x); });
         const int sweeps = steps(l) * opt_.smoothing_steps;
         // (Gs, S): s sweeps; G = I - w V A is the one-sweep iteration matrix
         auto sweep_pair = [&](double w) {
-          DBuf<double> G = identity(n);
-          blas.gemm(n, n, n, -w, V.get(), A.get(), 1.0, G.get());
-          DBuf<double> H = copy_of(V);
-          launch_scale(long(H.size()), w, H.get(), s);
+          DBuf<T> G = identity(n);
+          blas.gemm(n, n, n, T(-w), V.get(), A.get(), T(1), G.get());
+          DBuf<T> H = copy_of(V);
+          launch_scale<T>(long(H.size()), T(w), H.get(), s);
           return std::make_pair(std::move(G), std::move(H));
         };
-        DBuf<double> Gs, S;
+        DBuf<T> Gs, S;
         if (exact_level(l))
           {
             auto gh = sweep_pair(double(exact_omega(l, sweeps)));
@@ -1247,7 +1254,7 @@ namespace stfem
         else
           {
             auto gh = sweep_pair(double(T(lv_[l].omega)));
-            DBuf<double> Gb = std::move(gh.first), Sb = std::move(gh.second);
+            DBuf<T> Gb = std::move(gh.first), Sb = std::move(gh.second);
             bool         have = false;
             for (int k = sweeps; k > 0; k >>= 1)
               {
@@ -1262,20 +1269,20 @@ namespace stfem
                     else
                       {
                         // (Gs, S) followed by the base block (Gb, Sb)
-                        DBuf<double> S2 = copy_of(Sb);
-                        blas.gemm(n, n, n, 1.0, Gb.get(), S.get(), 1.0, S2.get());
-                        DBuf<double> G2(size_t(n) * n);
-                        blas.gemm(n, n, n, 1.0, Gb.get(), Gs.get(), 0.0, G2.get());
+                        DBuf<T> S2 = copy_of(Sb);
+                        blas.gemm(n, n, n, T(1), Gb.get(), S.get(), T(1), S2.get());
+                        DBuf<T> G2(size_t(n) * n);
+                        blas.gemm(n, n, n, T(1), Gb.get(), Gs.get(), T(0), G2.get());
                         S  = std::move(S2);
                         Gs = std::move(G2);
                       }
                   }
                 if (k > 1)
                   {
-                    DBuf<double> S2 = copy_of(Sb);
-                    blas.gemm(n, n, n, 1.0, Gb.get(), Sb.get(), 1.0, S2.get());
-                    DBuf<double> G2(size_t(n) * n);
-                    blas.gemm(n, n, n, 1.0, Gb.get(), Gb.get(), 0.0, G2.get());
+                    DBuf<T> S2 = copy_of(Sb);
+                    blas.gemm(n, n, n, T(1), Gb.get(), Sb.get(), T(1), S2.get());
+                    DBuf<T> G2(size_t(n) * n);
+                    blas.gemm(n, n, n, T(1), Gb.get(), Gb.get(), T(0), G2.get());
                     Sb = std::move(S2);
                     Gb = std::move(G2);
                   }
@@ -1294,7 +1301,7 @@ namespace stfem
           launch_fill_zero<T>(y, size_t(n), s);
           tr.prolongate_and_add(y, x);
         };
-        DBuf<double> P, R;
+        DBuf<T> P, R;
         if (tr.time)
           {
             // time transfers act node by node
@@ -1310,20 +1317,20 @@ namespace stfem
             // |o - 2 g| <= 2p; the restriction is the transpose (the same 1D
             // weights, gmg.h:371-380)
             P = dense_probe(gc, ntc, gl, ntl, 2 * pl + 1, 2, 1, 2 * pl, prolong);
-            R = DBuf<double>(size_t(nc) * n);
-            launch_transpose(n, nc, P.get(), R.get(), s);
+            R = DBuf<T>(size_t(nc) * n);
+            launch_transpose<T>(n, nc, P.get(), R.get(), s);
           }
-        DBuf<double> X = identity(n); // I - A S
-        blas.gemm(n, n, n, -1.0, A.get(), S.get(), 1.0, X.get());
-        DBuf<double> Y(size_t(nc) * n); // R X
-        blas.gemm(nc, n, n, 1.0, R.get(), X.get(), 0.0, Y.get());
-        DBuf<double> Z(size_t(nc) * n); // M_{l-1} Y
-        blas.gemm(nc, n, nc, 1.0, Mc.get(), Y.get(), 0.0, Z.get());
-        DBuf<double> U = tr.prolongation_overwrites() ? DBuf<double>(size_t(n) * n) : copy_of(S);
-        blas.gemm(n, n, nc, 1.0, P.get(), Z.get(), tr.prolongation_overwrites() ? 0.0 : 1.0,
+        DBuf<T> X = identity(n); // I - A S
+        blas.gemm(n, n, n, T(-1), A.get(), S.get(), T(1), X.get());
+        DBuf<T> Y(size_t(nc) * n); // R X
+        blas.gemm(nc, n, n, T(1), R.get(), X.get(), T(0), Y.get());
+        DBuf<T> Z(size_t(nc) * n); // M_{l-1} Y
+        blas.gemm(nc, n, nc, T(1), Mc.get(), Y.get(), T(0), Z.get());
+        DBuf<T> U = tr.prolongation_overwrites() ? DBuf<T>(size_t(n) * n) : copy_of(S);
+        blas.gemm(n, n, nc, T(1), P.get(), Z.get(), tr.prolongation_overwrites() ? T(0) : T(1),
                   U.get());
-        DBuf<double> M = copy_of(S); // Gs U + S
-        blas.gemm(n, n, n, 1.0, Gs.get(), U.get(), 1.0, M.get());
+        DBuf<T> M = copy_of(S); // Gs U + S
+        blas.gemm(n, n, n, T(1), Gs.get(), U.get(), T(1), M.get());
         Mc = std::move(M);
         nc = n;
       }
@@ -1334,7 +1341,7 @@ namespace stfem
     const long vw = long(16 / sizeof(T));
     dense_ld_     = (nc + vw - 1) / vw * vw;
     dense_rows_.alloc(size_t(nc) * dense_ld_);
-    launch_pack_rows<T>(nc, Mc.get(), dense_rows_.get(), dense_ld_, s);
+    launch_pack_rows<T, T>(nc, Mc.get(), dense_rows_.get(), dense_ld_, s);
     ctx_.sync();
     dense_level_ = lc;
   }
