@@ -50,6 +50,17 @@ namespace stfem
       return cell_slot<DIM, N>(g.nc[0], g.nc[1], nto, cx, cy, cz, a, b, c, j);
     }
 
+    // Cell-block layout (the register-tiled Vanka sweep's results): the
+    // n_loc * nto values of cell c (local node index a + N b + N^2 c, the
+    // nto components innermost) are contiguous from c * cs on, with cs
+    // rounded up to whole 128-byte lines so that each cell writes its values
+    // as full lines (non-temporal stores that need no merging in L2).
+    template <typename T>
+    __host__ __device__ constexpr long cell_block_stride(int nloc, int nto)
+    {
+      return vanka_cell_block(nloc, nto, sizeof(T));
+    }
+
     // NT contiguous values at p (p a multiple of NT values from a 256-byte
     // aligned base): 16-byte vector accesses when NT * sizeof(T) is a
     // multiple of 16, one 8-byte access for two floats, scalars otherwise
@@ -206,11 +217,13 @@ namespace stfem
     // unchanged (mode 1); otherwise (Vanka, whose patches cover them) they
     // are summed like every other node.
     // NT > 0: compile-time number of components (all loads issued before
-    // the sums); NT == 0: nto at run time.
-    template <int DIM, int N, int NT, typename T, bool SKIP_DIR = true>
+    // the sums); NT == 0: nto at run time.  CB: the buffer is in the
+    // cell-block layout of block stride cs (same contributions, same order).
+    template <int DIM, int N, int NT, typename T, bool SKIP_DIR = true, bool CB = false>
     __device__ __forceinline__ void gather_node(const GridDesc &g, int nto,
                                                 const T *__restrict__ cells, T *__restrict__ dst,
-                                                const T *__restrict__ rhs, int mode, long node)
+                                                const T *__restrict__ rhs, int mode, long node,
+                                                long cs = 0)
     {
       constexpr int p   = N - 1;
       const long    nn0 = g.nn[0], nn1 = g.nn[1];
@@ -261,14 +274,29 @@ namespace stfem
         }
       // the next cell's copy of a shared node: along x the adjacent entry of
       // the same row ((cx, p) -> (cx + 1, 0)); along y and z the rows of the
-      // next cell with local index 0 instead of p
+      // next cell with local index 0 instead of p.  Cell blocks: the next
+      // cell's block with the local index lowered by p (x), p N (y), p N^2 (z).
       constexpr int MAXS = 1 << DIM;
-      const long    L    = long(g.nc[0]) * N;
-      const long    base = (DIM == 3) ? cell_slot<DIM, N>(g, nt, c0[0], c0[1], c0[2], l0[0], l0[1], l0[2], 0)
-                                      : cell_slot<DIM, N>(g, nt, c0[0], c0[1], 0, l0[0], l0[1], 0, 0);
-      const long    dx   = nt;
-      const long    dy   = (DIM == 3) ? (long(N) * N - p) * L * nt : (long(N) - p) * L * nt;
-      const long    dz   = (DIM == 3) ? (long(g.nc[1]) * N * N - long(p) * N) * L * nt : 0;
+      long          base, dx, dy, dz;
+      if constexpr (CB)
+        {
+          const long cell = (DIM == 3) ? c0[0] + long(g.nc[0]) * (c0[1] + long(g.nc[1]) * c0[2])
+                                       : c0[0] + long(g.nc[0]) * c0[1];
+          const int  loc  = (DIM == 3) ? l0[0] + N * (l0[1] + N * l0[2]) : l0[0] + N * l0[1];
+          base            = cell * cs + long(loc) * nt;
+          dx              = cs - long(p) * nt;
+          dy              = long(g.nc[0]) * cs - long(p) * N * nt;
+          dz = (DIM == 3) ? long(g.nc[0]) * g.nc[1] * cs - long(p) * N * N * nt : 0;
+        }
+      else
+        {
+          const long L = long(g.nc[0]) * N;
+          base = (DIM == 3) ? cell_slot<DIM, N>(g, nt, c0[0], c0[1], c0[2], l0[0], l0[1], l0[2], 0)
+                            : cell_slot<DIM, N>(g, nt, c0[0], c0[1], 0, l0[0], l0[1], 0, 0);
+          dx   = nt;
+          dy   = (DIM == 3) ? (long(N) * N - p) * L * nt : (long(N) - p) * L * nt;
+          dz   = (DIM == 3) ? (long(g.nc[1]) * N * N - long(p) * N) * L * nt : 0;
+        }
       // contribution k = kx + 2 ky + 4 kz is used where the node is shared
       // in those directions; unused ones add exact zeros, so the sum is the
       // one over the used contributions in the fixed order

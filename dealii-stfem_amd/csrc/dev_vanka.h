@@ -532,7 +532,9 @@ namespace stfem
     // the registers, one workgroup per CU).  Partial sums over the 16 tiles of a column
     // (of a row) meet in LDS laid out [tile][i][index], which makes the
     // reduction reads consecutive across lanes.  Four barriers per cell.
-    // The corrections go to the cell-major buffer (two-pass form).
+    // The corrections go to the cell buffer in the cell-block layout
+    // (cell c's values contiguous from c * CS on, 128-byte lines; two-pass
+    // form), which the node gather reads with the block stride CS.
     template <typename T, int NL, int NT>
     struct V3
     {
@@ -544,6 +546,7 @@ namespace stfem
       static constexpr int    AP  = 16 * R;                                // padded row count
       static constexpr int    NX  = (NT * NL + 255) / 256;                 // residual values per thread
       static constexpr int    RS  = NT * AP + 1; // odd tile stride of the row partials
+      static constexpr long   CS  = vanka_cell_block(NL, NT, sizeof(T)); // cell-block stride
       static constexpr size_t lds = sizeof(T) * (size_t(NT * NL) + size_t(NT * KP) +
                                                  (size_t(16) * NT * KP > size_t(16) * RS ?
                                                     size_t(16) * NT * KP : size_t(16) * RS));
@@ -552,7 +555,9 @@ namespace stfem
     // AXC: A x at the patch nodes is summed here from the operator's
     // cell-major buffer opc (the value its node gather would store), which
     // saves the gather launch of the residual; ax is not read.
-    template <typename T, int NL, int NT, bool PIPE, bool AXC = false>
+    // NTS: the cell's results are stored non-temporally (cell buffers well
+    // above the last-level cache); otherwise with ordinary stores
+    template <typename T, int NL, int NT, bool PIPE, bool AXC = false, bool NTS = true>
     __device__ __forceinline__ void vanka_v3_range(const PatchMap &M, long n_cells,
                                                    const T *__restrict__ S,
                                                    const T *__restrict__ lam,
@@ -735,8 +740,8 @@ namespace stfem
             const int a = tid;
             float     iv;
             (void)node_ct<NL>(M, cx, cy, cz, a, iv);
-            const T   sc = omega * T(iv);
-            const int la = a % n, lb = (a / n) % n, lc = (Sh::dim == 3) ? a / (n * n) : 0;
+            const T sc = omega * T(iv);
+            T       o[NT];
 #pragma unroll
             for (int i = 0; i < NT; ++i)
               {
@@ -744,9 +749,22 @@ namespace stfem
 #pragma unroll
                 for (int q = 0; q < 16; ++q)
                   acc += red[q * RS + i * AP + a];
-                cells[cell_slot<Sh::dim, n>(M.nc0, M.nc1, NT, cx, cy, cz, la, lb, lc, i)] =
-                  sc * acc;
+                o[i] = sc * acc;
               }
+            // the cell's block of the cell-block layout: node a's NT values
+            // at a * NT; with NTS one 16-byte value per node is stored
+            // non-temporally (the lanes of a wavefront write whole lines)
+            T *const dst = cells + (size_t)cell * C::CS + (size_t)a * NT;
+            if constexpr (NTS && NT * sizeof(T) == 16)
+              {
+                vec_t v;
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                  v[e] = o[e];
+                __builtin_nontemporal_store(v, reinterpret_cast<vec_t *>(dst));
+              }
+            else
+              store_comps<T, NT>(dst, o);
           }
       };
 

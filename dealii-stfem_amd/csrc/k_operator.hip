@@ -45,20 +45,24 @@ namespace stfem
                                lds);
     }
 
-    template <int DIM, int N, int NT, typename T, bool SKIP_DIR>
+    template <int DIM, int N, int NT, typename T, bool SKIP_DIR, bool CB>
     __global__ void __launch_bounds__(256)
       k_gather_nodes(const GridDesc g, int nto, const T *__restrict__ cells, T *__restrict__ dst,
-                     const T *__restrict__ rhs, int mode)
+                     const T *__restrict__ rhs, int mode, long cs)
     {
       const long node = xcd_block((long)blockIdx.x, (long)gridDim.x) * 256 + threadIdx.x;
       if (node < g.n_nodes)
-        gather_node<DIM, N, NT, T, SKIP_DIR>(g, nto, cells, dst, rhs, mode, node);
+        gather_node<DIM, N, NT, T, SKIP_DIR, CB>(g, nto, cells, dst, rhs, mode, node, cs);
     }
 
-    template <int DIM, int N, typename T, bool SKIP_DIR>
+    // cs > 0: the buffer is in the cell-block layout of stride cs
+    template <int DIM, int N, typename T, bool SKIP_DIR, bool CB = false>
     void gather_dn(const GridDesc &g, int nto, const T *cells, T *dst, const T *rhs, int mode,
-                   hipStream_t s)
+                   hipStream_t s, long cs = 0)
     {
+      if constexpr (!CB)
+        if (cs > 0)
+          return gather_dn<DIM, N, T, SKIP_DIR, true>(g, nto, cells, dst, rhs, mode, s, cs);
       const dim3 grid{(unsigned)((g.n_nodes + 255) / 256), 1, 1};
       // the compile-time forms access a node's components as vectors: node
       // vectors not 16-byte aligned take the run-time form
@@ -66,24 +70,24 @@ namespace stfem
                            reinterpret_cast<uintptr_t>(rhs) % 16 == 0;
       switch (aligned ? nto : 0)
         {
-          case 1: k_gather_nodes<DIM, N, 1, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, 1, cells, dst, rhs, mode); break;
-          case 2: k_gather_nodes<DIM, N, 2, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, 2, cells, dst, rhs, mode); break;
-          case 3: k_gather_nodes<DIM, N, 3, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, 3, cells, dst, rhs, mode); break;
-          case 4: k_gather_nodes<DIM, N, 4, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, 4, cells, dst, rhs, mode); break;
+          case 1: k_gather_nodes<DIM, N, 1, T, SKIP_DIR, CB><<<grid, 256, 0, s>>>(g, 1, cells, dst, rhs, mode, cs); break;
+          case 2: k_gather_nodes<DIM, N, 2, T, SKIP_DIR, CB><<<grid, 256, 0, s>>>(g, 2, cells, dst, rhs, mode, cs); break;
+          case 3: k_gather_nodes<DIM, N, 3, T, SKIP_DIR, CB><<<grid, 256, 0, s>>>(g, 3, cells, dst, rhs, mode, cs); break;
+          case 4: k_gather_nodes<DIM, N, 4, T, SKIP_DIR, CB><<<grid, 256, 0, s>>>(g, 4, cells, dst, rhs, mode, cs); break;
           default:
-            k_gather_nodes<DIM, N, 0, T, SKIP_DIR><<<grid, 256, 0, s>>>(g, nto, cells, dst, rhs, mode);
+            k_gather_nodes<DIM, N, 0, T, SKIP_DIR, CB><<<grid, 256, 0, s>>>(g, nto, cells, dst, rhs, mode, cs);
             break;
         }
       STFEM_HIP(hipGetLastError());
     }
     template <typename T, bool SKIP_DIR>
     void gather_any(const GridDesc &g, int nto, const T *cells, T *dst, const T *rhs, int mode,
-                    hipStream_t s)
+                    hipStream_t s, long cs = 0)
     {
       const int N = g.p + 1;
 #define STFEM_GATHER_CASE(D, NN)                                                                   \
   if (g.dim == D && N == NN)                                                                       \
-    return gather_dn<D, NN, T, SKIP_DIR>(g, nto, cells, dst, rhs, mode, s);
+    return gather_dn<D, NN, T, SKIP_DIR>(g, nto, cells, dst, rhs, mode, s, cs);
       STFEM_GATHER_CASE(2, 2) STFEM_GATHER_CASE(2, 3) STFEM_GATHER_CASE(2, 4)
       STFEM_GATHER_CASE(2, 5) STFEM_GATHER_CASE(2, 6) STFEM_GATHER_CASE(3, 2)
       STFEM_GATHER_CASE(3, 3) STFEM_GATHER_CASE(3, 4) STFEM_GATHER_CASE(3, 5)
@@ -205,21 +209,25 @@ namespace stfem
 
   template <typename T>
   void launch_gather_cells(const GridDesc &g, int nto, const T *cells, T *dst, const T *rhs,
-                           int mode, bool skip_dirichlet, hipStream_t s)
+                           int mode, bool skip_dirichlet, hipStream_t s, long cs)
   {
     STFEM_REQUIRE(nto >= 1 && mode >= 0 && mode <= 2, "gather: bad arguments");
     STFEM_REQUIRE(mode != 2 || rhs, "gather: mode 2 needs rhs");
+    STFEM_REQUIRE(cs == 0 || (cs >= long(nto) * (g.dim == 3 ? (g.p + 1) * (g.p + 1) * (g.p + 1)
+                                                            : (g.p + 1) * (g.p + 1)) &&
+                              cs % nto == 0),
+                  "gather: cell-block stride below the cell's values");
     if (g.n_nodes == 0)
       return;
     if (skip_dirichlet)
-      gather_any<T, true>(g, nto, cells, dst, rhs, mode, s);
+      gather_any<T, true>(g, nto, cells, dst, rhs, mode, s, cs);
     else
-      gather_any<T, false>(g, nto, cells, dst, rhs, mode, s);
+      gather_any<T, false>(g, nto, cells, dst, rhs, mode, s, cs);
   }
   template void launch_gather_cells<double>(const GridDesc &, int, const double *, double *,
-                                            const double *, int, bool, hipStream_t);
+                                            const double *, int, bool, hipStream_t, long);
   template void launch_gather_cells<float>(const GridDesc &, int, const float *, float *,
-                                           const float *, int, bool, hipStream_t);
+                                           const float *, int, bool, hipStream_t, long);
 
   template void launch_st_apply<double>(const StOpArgs<double> &, const double *, double *,
                                         double *, int, const double *, hipStream_t);

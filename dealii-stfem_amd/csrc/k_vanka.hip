@@ -65,7 +65,7 @@ namespace stfem
     }
 
     // !PIPE: register budget for four (FP32) or two (FP64) workgroups per CU
-    template <typename T, int NL, int NT, bool PIPE>
+    template <typename T, int NL, int NT, bool PIPE, bool NTS = true>
     __global__ void __launch_bounds__(256, PIPE ? 1 : (sizeof(T) == 4 ? 4 : 2))
       k_vanka_factored_v3(const PatchMap M, long n_cells, const T *__restrict__ S,
                           const T *__restrict__ lam, const T *__restrict__ ta,
@@ -73,10 +73,14 @@ namespace stfem
                           const T *__restrict__ ax, T *__restrict__ cells, T omega)
     {
       extern __shared__ __align__(16) unsigned char smem_raw[];
-      vanka_v3_range<T, NL, NT, PIPE>(M, n_cells, S, lam, ta, tb, rhs, ax, cells, omega,
-                                      (int)threadIdx.x,
-                                      reinterpret_cast<T *>(smem_raw));
+      vanka_v3_range<T, NL, NT, PIPE, false, NTS>(M, n_cells, S, lam, ta, tb, rhs, ax, cells,
+                                                  omega, (int)threadIdx.x,
+                                                  reinterpret_cast<T *>(smem_raw));
     }
+    // cell buffers above this size take non-temporal stores of the sweep's
+    // results (below it they stay in the 256 MB last-level cache for the
+    // node gather that follows)
+    constexpr size_t NT_STORE_BYTES = size_t(128) << 20;
     // the same sweep with A x summed from the operator's cell buffer opc
     template <typename T, int NL, int NT>
     __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2)
@@ -257,6 +261,9 @@ namespace stfem
             {
               STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v3<T, NL, NT, false>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+              STFEM_HIP(
+                hipFuncSetAttribute((const void *)k_vanka_factored_v3<T, NL, NT, false, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
               STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v3<T, NL, NT, true>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
               STFEM_HIP(hipFuncSetAttribute((const void *)k_vanka_factored_v3_opc<T, NL, NT>,
@@ -291,12 +298,16 @@ namespace stfem
               k_vanka_factored_v3<T, NL, NT, true><<<dim3((unsigned)grid), dim3(256), lds, s>>>(
                 M, g.n_cells, S, lam, ta, tb, rhs, ax, cells, omega);
             }
-          else
+          else if (size_t(g.n_cells) * V3<T, NL, NT>::CS * sizeof(T) > NT_STORE_BYTES)
             k_vanka_factored_v3<T, NL, NT, false><<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(
               M, g.n_cells, S, lam, ta, tb, rhs, ax, cells, omega);
+          else
+            k_vanka_factored_v3<T, NL, NT, false, false>
+              <<<dim3((unsigned)g.n_cells), dim3(256), lds, s>>>(M, g.n_cells, S, lam, ta, tb, rhs,
+                                                                 ax, cells, omega);
           STFEM_HIP(hipGetLastError());
           if (mode != 3)
-            launch_gather_cells<T>(g, NT, cells, u, nullptr, mode, false, s);
+            launch_gather_cells<T>(g, NT, cells, u, nullptr, mode, false, s, V3<T, NL, NT>::CS);
           done = true;
         }
     };

@@ -75,12 +75,21 @@ namespace stfem
     const int vw = int(16 / tsize);
     return (nloc + vw - 1) / vw * vw;
   }
+  // values per cell of the cell-block layout in which the register-tiled
+  // factored sweep leaves its per-cell results: n_c * nt rounded up to whole
+  // 128-byte lines (the size per cell of the sweep's cell buffer)
+  inline constexpr long vanka_cell_block(int nloc, int nt, size_t tsize)
+  {
+    const long v = 128 / long(tsize);
+    return (long(nloc) * nt + v - 1) / v * v;
+  }
   // Factored form: Binv_c = (I x S_c) blockdiag_k (W_{c,k}) (I x S_c^T),
   // S_c (n_c x n_c, row-major with the row stride vanka_s_ld(n_c, sizeof(T)),
   // padding zero), W_{c,k} = (lam_{c,k} Alpha + Beta)^{-1} applied from the
   // eigenvalues lam (n_c per cell) and the level's time matrices ta = Alpha,
   // tb = Beta (nt x nt row-major, device).
-  // cells (n_cells * n_c * nt values) non-null: two-pass form for the
+  // cells (n_cells * vanka_cell_block(n_c, nt, sizeof(T)) values) non-null:
+  // two-pass form for the
   // compile-time patch shapes, mode 0 u = correction, 1 u += correction;
   // otherwise (and for the other shapes) u += correction with atomics
   // (mode 0 then zero-fills u first).  variant: 0 automatic, 2 the
@@ -99,12 +108,14 @@ namespace stfem
                              int mode, int variant, hipStream_t s, const T *opc = nullptr);
   template <typename T>
   bool vanka_fused_residual_supported(const GridDesc &g, int nt);
-  // Node gather of a cell-major buffer (cell_slot layout, nto components):
-  // mode 0 dst = sum, 1 dst += sum, 2 dst = rhs - sum; skip_dirichlet:
-  // Dirichlet nodes are left out (zero for modes 0, 2, unchanged for 1).
+  // Node gather of a cell-major buffer (cell_slot layout, nto components;
+  // cs > 0: cell c's values contiguous from c * cs on, local node index
+  // a + N b + N^2 c, components innermost): mode 0 dst = sum, 1 dst += sum,
+  // 2 dst = rhs - sum; skip_dirichlet: Dirichlet nodes are left out (zero
+  // for modes 0, 2, unchanged for 1).
   template <typename T>
   void launch_gather_cells(const GridDesc &g, int nto, const T *cells, T *dst, const T *rhs,
-                           int mode, bool skip_dirichlet, hipStream_t s);
+                           int mode, bool skip_dirichlet, hipStream_t s, long cs = 0);
   // p[0..n) = 0 with a kernel (safe to record into a HIP graph)
   template <typename T>
   void launch_fill_zero(T *p, size_t n, hipStream_t s);

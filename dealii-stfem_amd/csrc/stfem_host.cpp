@@ -581,7 +581,7 @@ namespace stfem
         const size_t bsp = size_t(nl) * ld;
         S_.alloc(size_t(nc) * bsp);
         lam_.alloc(size_t(nc) * nl);
-        cells_.alloc(size_t(nc) * nl * nt);
+        cells_.alloc(size_t(nc) * vanka_cell_block(nl, nt, sizeof(T)));
         const long     ch = std::max<long>(1, std::min<long>(nc, 4096));
         std::vector<T> sS, sW;
         for (long c0 = 0; c0 < nc; c0 += ch)
@@ -708,7 +708,7 @@ namespace stfem
     const size_t bsp = size_t(nl) * vanka_s_ld(nl, sizeof(T)); // row-padded S block
     S_.alloc(size_t(nc) * bsp);
     lam_.alloc(size_t(nc) * nl);
-    cells_.alloc(size_t(nc) * nl * nt);
+    cells_.alloc(size_t(nc) * vanka_cell_block(nl, nt, sizeof(T)));
     DBuf<double> X, coef, dA, dB;
     X.upload(m.X.data(), m.X.size(), s);
     if (m.coef_mode != 0)
