@@ -125,6 +125,10 @@ namespace stfem
       const double *v[MD];
       int           m, dx, dy, dxy;
     };
+    // VEC: every vector 16-byte aligned; thread work items are pairs of
+    // consecutive entries read as one 16-byte load each (the last entry of
+    // an odd n goes to the first thread)
+    template <bool VEC>
     __global__ void __launch_bounds__(RT)
       k_multi_dot(long n, DotList L, const double *__restrict__ x, const double *__restrict__ y,
                   double *__restrict__ partial)
@@ -133,19 +137,44 @@ namespace stfem
 #pragma unroll
       for (int i = 0; i < MD; ++i)
         ax[i] = ay[i] = 0.0;
-      for (long e = (long)blockIdx.x * RT + threadIdx.x; e < n; e += (long)RB * RT)
-        {
-          const double xe = x[e], ye = y[e];
-          axy += xe * ye;
+      auto one = [&](long e) {
+        const double xe = x[e], ye = y[e];
+        axy += xe * ye;
 #pragma unroll
-          for (int i = 0; i < MD; ++i)
-            if (i < L.m)
-              {
-                const double ve = L.v[i][e];
-                ax[i] += ve * xe;
-                ay[i] += ve * ye;
-              }
+        for (int i = 0; i < MD; ++i)
+          if (i < L.m)
+            {
+              const double ve = L.v[i][e];
+              ax[i] += ve * xe;
+              ay[i] += ve * ye;
+            }
+      };
+      if constexpr (VEC)
+        {
+          const long n2 = n / 2;
+          for (long e = (long)blockIdx.x * RT + threadIdx.x; e < n2; e += (long)RB * RT)
+            {
+              const double2 xe = reinterpret_cast<const double2 *>(x)[e];
+              const double2 ye = reinterpret_cast<const double2 *>(y)[e];
+              axy += xe.x * ye.x;
+              axy += xe.y * ye.y;
+#pragma unroll
+              for (int i = 0; i < MD; ++i)
+                if (i < L.m)
+                  {
+                    const double2 ve = reinterpret_cast<const double2 *>(L.v[i])[e];
+                    ax[i] += ve.x * xe.x;
+                    ax[i] += ve.y * xe.y;
+                    ay[i] += ve.x * ye.x;
+                    ay[i] += ve.y * ye.y;
+                  }
+            }
+          if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0)
+            one(n - 1);
         }
+      else
+        for (long e = (long)blockIdx.x * RT + threadIdx.x; e < n; e += (long)RB * RT)
+          one(e);
       // wavefront sums, then the four wavefront sums in order
       __shared__ double red[RT / 64][2 * MD + 1];
       const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -227,6 +256,8 @@ namespace stfem
       const double *v[MA];
       int           m;
     };
+    // VEC: as in k_multi_dot (pairs of entries, 16-byte accesses)
+    template <bool VEC>
     __global__ void __launch_bounds__(RT)
       k_multi_axpy_norm(long n, AxpyList L, const double *__restrict__ h, double *__restrict__ x,
                         double *__restrict__ pnorm)
@@ -235,15 +266,41 @@ namespace stfem
       if (threadIdx.x < L.m)
         hs[threadIdx.x] = h[threadIdx.x];
       __syncthreads();
-      double s = 0.0;
-      for (long e = (long)blockIdx.x * RT + threadIdx.x; e < n; e += (long)RB * RT)
+      double s    = 0.0;
+      auto   one  = [&](long e) {
+        double w = x[e];
+        for (int i = 0; i < L.m; ++i)
+          w -= hs[i] * L.v[i][e];
+        x[e] = w;
+        s += w * w;
+      };
+      if constexpr (VEC)
         {
-          double w = x[e];
-          for (int i = 0; i < L.m; ++i)
-            w -= hs[i] * L.v[i][e];
-          x[e] = w;
-          s += w * w;
+          const long n2 = n / 2;
+          for (long e = (long)blockIdx.x * RT + threadIdx.x; e < n2; e += (long)RB * RT)
+            {
+              double2 w = reinterpret_cast<const double2 *>(x)[e];
+#pragma unroll 4
+              for (int i = 0; i < L.m; ++i)
+                {
+                  const double2 v = reinterpret_cast<const double2 *>(L.v[i])[e];
+                  w.x -= hs[i] * v.x;
+                  w.y -= hs[i] * v.y;
+                }
+              // streamed out (whole 128-byte lines per wavefront), not kept
+              // in the caches that the basis vectors stream through
+              typedef double vd2 __attribute__((ext_vector_type(2)));
+              vd2 wv = {w.x, w.y};
+              __builtin_nontemporal_store(wv, reinterpret_cast<vd2 *>(x) + e);
+              s += w.x * w.x;
+              s += w.y * w.y;
+            }
+          if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0)
+            one(n - 1);
         }
+      else
+        for (long e = (long)blockIdx.x * RT + threadIdx.x; e < n; e += (long)RB * RT)
+          one(e);
       s = block_sum(s);
       if (threadIdx.x == 0)
         pnorm[blockIdx.x] = s;
@@ -521,6 +578,11 @@ namespace stfem
   {
     STFEM_REQUIRE(j >= 0 && j < MA && j < MG && j < ldg, "Gram-Schmidt step: basis too large");
     const double *vj = v[j];
+    // 16-byte accesses when every vector allows them
+    auto al  = [](const double *p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0; };
+    bool vec = al(x);
+    for (int i = 0; i <= j; ++i)
+      vec = vec && al(v[i]);
     // (v_i, x) -> d = i, (v_i, v_j) -> d = j + 1 + i for i < j; (v_j, x) -> d = j
     for (int i0 = 0; i0 < j || i0 == 0; i0 += MD)
       {
@@ -531,7 +593,10 @@ namespace stfem
         L.dxy = (i0 == 0) ? j : -1;
         for (int i = 0; i < L.m; ++i)
           L.v[i] = v[i0 + i];
-        k_multi_dot<<<RB, RT, 0, s>>>(n, L, x, vj, partial);
+        if (vec)
+          k_multi_dot<true><<<RB, RT, 0, s>>>(n, L, x, vj, partial);
+        else
+          k_multi_dot<false><<<RB, RT, 0, s>>>(n, L, x, vj, partial);
         if (j == 0)
           break;
       }
@@ -541,7 +606,10 @@ namespace stfem
     A.m = j + 1;
     for (int i = 0; i <= j; ++i)
       A.v[i] = v[i];
-    k_multi_axpy_norm<<<RB, RT, 0, s>>>(n, A, h, x, pnorm);
+    if (vec)
+      k_multi_axpy_norm<true><<<RB, RT, 0, s>>>(n, A, h, x, pnorm);
+    else
+      k_multi_axpy_norm<false><<<RB, RT, 0, s>>>(n, A, h, x, pnorm);
     STFEM_HIP(hipGetLastError());
   }
 
