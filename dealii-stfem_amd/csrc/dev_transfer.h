@@ -12,7 +12,7 @@
 // zero-Dirichlet rows/columns removed, applied as one 1D sweep per direction.
 #pragma once
 
-#include "kernels.h"
+#include "dev_cells.h"
 
 namespace stfem
 {
@@ -90,6 +90,77 @@ namespace stfem
               s += wl[k] * xl[k];
         }
       out[idx] = add ? out[idx] + s : s;
+    }
+
+    // node pt of the output of the same 1D banded operator, all NT
+    // components at once: the row's tables are read once and every tap loads
+    // the node's NT contiguous values (16-byte accesses where NT * sizeof(T)
+    // is a multiple of 16, the vectors 16-byte aligned); the sums run over
+    // the taps in order, as in sweep_item, so the results are the same
+    template <typename T, typename I, int KM, int NT>
+    __device__ __forceinline__ void sweep_node(I pt, const T *__restrict__ in,
+                                               T *__restrict__ out, int d, int n0, int n1, int m,
+                                               const int *__restrict__ cnt,
+                                               const int *__restrict__ col,
+                                               const T *__restrict__ w, int K, int add)
+    {
+      const I   o0 = I((d == 0) ? m : n0), o1 = I((d == 1) ? m : n1);
+      const I   i0 = pt % o0, r = pt / o0, i1 = r % o1, i2 = r / o1;
+      const int od = int((d == 0) ? i0 : (d == 1) ? i1 : i2);
+      const I   stride = (d == 0) ? I(1) : (d == 1) ? I(n0) : I(n0) * I(n1);
+      const I j0 = (d == 0) ? I(0) : i0, j1 = (d == 1) ? I(0) : i1, j2 = (d == 2) ? I(0) : i2;
+      const I ib = j0 + I(n0) * (j1 + I(n1) * j2);
+      const int c = cnt[od];
+      T         s[NT];
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        s[j] = T(0);
+      auto tap = [&](T wk, int ck) {
+        T x[NT];
+        load_comps<T, NT>(in + (ib + I(ck) * stride) * I(NT), x);
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          s[j] += wk * x[j];
+      };
+      if constexpr (KM == 0)
+        {
+#pragma unroll 4
+          for (int k = 0; k < c; ++k)
+            tap(w[od * K + k], col[od * K + k]);
+        }
+      else
+        {
+          int cl[KM];
+          T   wl[KM];
+#pragma unroll
+          for (int k = 0; k < KM; ++k)
+            if (k < K)
+              {
+                cl[k] = col[od * K + k];
+                wl[k] = w[od * K + k];
+              }
+          T xl[KM][NT];
+#pragma unroll
+          for (int k = 0; k < KM; ++k)
+            if (k < c)
+              load_comps<T, NT>(in + (ib + I(cl[k]) * stride) * I(NT), xl[k]);
+#pragma unroll
+          for (int k = 0; k < KM; ++k)
+            if (k < c)
+#pragma unroll
+              for (int j = 0; j < NT; ++j)
+                s[j] += wl[k] * xl[k][j];
+        }
+      T *const o = out + pt * I(NT);
+      if (add)
+        {
+          T v[NT];
+          load_comps<T, NT>(o, v);
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            s[j] = v[j] + s[j];
+        }
+      store_comps<T, NT>(o, s);
     }
 
     inline long sweep_total(int nt, int d, const int n_in[3], int m_out)

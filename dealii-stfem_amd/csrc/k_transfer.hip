@@ -3,8 +3,12 @@
 // Time transfers (MGTwoLevelTransferST, tests/include/gmg.h:121-135 of
 // nlsmrg/dealii-stfem) are one streaming pass over the node-major vector;
 // space transfers (deal.II MGTwoLevelTransfer built at gmg.h:371-380) are one
-// 1D banded sweep per direction.  Per-entry device code is in dev_transfer.h.
+// 1D banded sweep per direction, one thread per node (all its time
+// components) or per entry.  The device code is in dev_transfer.h.
 #include "dev_transfer.h"
+
+#include <cstdint>
+#include <type_traits>
 
 namespace stfem
 {
@@ -27,6 +31,51 @@ namespace stfem
       const I idx = I(blockIdx.x) * I(blockDim.x) + I(threadIdx.x);
       if (idx < total)
         sweep_item<T, I, KM>(idx, in, out, nt, d, n0, n1, m, cnt, col, w, K, add);
+    }
+
+    template <typename T, typename I, int KM, int NT>
+    __global__ void k_sweep_node(I nodes, const T *__restrict__ in, T *__restrict__ out, int d,
+                                 int n0, int n1, int m, const int *__restrict__ cnt,
+                                 const int *__restrict__ col, const T *__restrict__ w, int K,
+                                 int add)
+    {
+      const I pt = I(blockIdx.x) * I(blockDim.x) + I(threadIdx.x);
+      if (pt < nodes)
+        sweep_node<T, I, KM, NT>(pt, in, out, d, n0, n1, m, cnt, col, w, K, add);
+    }
+
+    // one thread per node and all its nt (1-4) components where the vectors
+    // allow the node's vector accesses; KM as for k_sweep
+    template <typename T, typename I>
+    bool launch_sweep_node(hipStream_t s, I nodes, const T *in, T *out, int nt, int d, int n0,
+                           int n1, int m, const int *cnt, const int *col, const T *w, int K,
+                           int add)
+    {
+      if (nt < 1 || nt > 4 || reinterpret_cast<uintptr_t>(in) % 16 != 0 ||
+          reinterpret_cast<uintptr_t>(out) % 16 != 0)
+        return false;
+      const dim3 grid((unsigned)((nodes + 255) / 256));
+      auto go = [&](auto nt_c, auto km_c) {
+        constexpr int NT = decltype(nt_c)::value, KM = decltype(km_c)::value;
+        k_sweep_node<T, I, KM, NT><<<grid, dim3(256), 0, s>>>(nodes, in, out, d, n0, n1, m, cnt,
+                                                              col, w, K, add);
+      };
+      auto by_k = [&](auto nt_c) {
+        if (K <= 4)
+          go(nt_c, std::integral_constant<int, 4>{});
+        else if (K <= 8)
+          go(nt_c, std::integral_constant<int, 8>{});
+        else
+          go(nt_c, std::integral_constant<int, 0>{});
+      };
+      switch (nt)
+        {
+          case 1: by_k(std::integral_constant<int, 1>{}); break;
+          case 2: by_k(std::integral_constant<int, 2>{}); break;
+          case 3: by_k(std::integral_constant<int, 3>{}); break;
+          default: by_k(std::integral_constant<int, 4>{}); break;
+        }
+      return true;
     }
 
     // k_sweep with the smallest tap bound KM >= K for rows of up to 8 taps
@@ -117,10 +166,17 @@ namespace stfem
       return;
     const long total_in = (long)n_in[0] * n_in[1] * n_in[2] * nt;
     const dim3 grid((unsigned)((total + 255) / 256));
-    if (std::max(total, total_in) <= k_max_u32)
-      launch_sweep_k<T, unsigned>(grid, s, unsigned(total), in, out, nt, d, n_in[0], n_in[1],
-                                  m_out, cnt, col, w, K, add ? 1 : 0);
-    else
+    const bool u32 = std::max(total, total_in) <= k_max_u32;
+    // one thread per node where possible, else one per entry
+    if (u32)
+      {
+        if (!launch_sweep_node<T, unsigned>(s, unsigned(total / nt), in, out, nt, d, n_in[0],
+                                            n_in[1], m_out, cnt, col, w, K, add ? 1 : 0))
+          launch_sweep_k<T, unsigned>(grid, s, unsigned(total), in, out, nt, d, n_in[0], n_in[1],
+                                      m_out, cnt, col, w, K, add ? 1 : 0);
+      }
+    else if (!launch_sweep_node<T, unsigned long>(s, total / nt, in, out, nt, d, n_in[0], n_in[1],
+                                                  m_out, cnt, col, w, K, add ? 1 : 0))
       launch_sweep_k<T, unsigned long>(grid, s, total, in, out, nt, d, n_in[0], n_in[1], m_out,
                                        cnt, col, w, K, add ? 1 : 0);
     STFEM_HIP(hipGetLastError());
