@@ -391,6 +391,34 @@ def test_node_gather_unaligned_vectors(name):
             assert rel_err(buf[1:].cpu().numpy(), y.cpu().numpy()) < tol_of(g), l
 
 
+@pytest.mark.parametrize("name", ["3d_q3_dg2", "3d_q4_dg3_fp32_factored"])
+def test_transfers_unaligned_vectors(name):
+    """The space transfer sweeps take one thread per node with vector
+    accesses when the vectors are 16-byte aligned and one thread per entry
+    otherwise; both sum a row's taps in the same order, so source and
+    destination vectors offset by one value give the same bits."""
+    spec, o, g = problems(name)
+    rng = np.random.default_rng(41)
+    dt = dtype_of(g)
+    for l in range(1, g.n_levels):
+        fi, ci = g.level_info(l), g.level_info(l - 1)
+        mf, mc = fi["n_nodes"] * fi["nt"], ci["n_nodes"] * ci["nt"]
+        for what, (m_src, m_dst) in (("prolongate", (mc, mf)), ("restrict", (mf, mc))):
+            x = torch.tensor(rng.standard_normal(m_src), dtype=dt, device=DEV)
+            y0 = torch.tensor(rng.standard_normal(m_dst), dtype=dt, device=DEV)
+            xs = torch.zeros(m_src + 1, dtype=dt, device=DEV)
+            ys = torch.zeros(m_dst + 1, dtype=dt, device=DEV)
+            xs[1:] = x
+            ys[1:] = y0
+            y = y0.clone()
+            assert xs[1:].data_ptr() % 16 != 0 and ys[1:].data_ptr() % 16 != 0
+            print(f"[case {name}] level {l} unaligned {what}", flush=True)
+            f = g.prolongate_and_add if what == "prolongate" else g.restrict_and_add
+            f(l, y, x)
+            f(l, ys[1:], xs[1:])
+            assert torch.equal(y, ys[1:]), (l, what)
+
+
 def test_vanka_sweep_kernel_timing():
     """The sweep kernel of the factored Vanka can be timed alone (selector 6:
     the cell pass without the node gather), and a level whose Vanka has no
