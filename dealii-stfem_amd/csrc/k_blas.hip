@@ -97,6 +97,34 @@ namespace stfem
       if (i < n)
         y[i] = a * x[i] + (b == T(0) ? T(0) : b * y[i]);
     }
+    // the same with 16 bytes per thread (vectors 16-byte aligned); the last
+    // n % VW entries go to the first thread
+    template <typename T>
+    __global__ void k_axpby_vec(long n, T a, const T *__restrict__ x, T b, T *__restrict__ y)
+    {
+      constexpr int VW = 16 / int(sizeof(T));
+      typedef T     vec_t __attribute__((ext_vector_type(VW)));
+      const long    i  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      const long    nv = n / VW;
+      if (i < nv)
+        {
+          const vec_t xv = reinterpret_cast<const vec_t *>(x)[i];
+          vec_t       r;
+          if (b == T(0))
+            r = a * xv;
+          else
+            {
+              const vec_t yv = reinterpret_cast<const vec_t *>(y)[i];
+#pragma unroll
+              for (int e = 0; e < VW; ++e)
+                r[e] = a * xv[e] + b * yv[e];
+            }
+          reinterpret_cast<vec_t *>(y)[i] = r;
+        }
+      if (i == 0)
+        for (long t = nv * VW; t < n; ++t)
+          y[t] = a * x[t] + (b == T(0) ? T(0) : b * y[t]);
+    }
 
     struct MultiAxpy
     {
@@ -119,7 +147,8 @@ namespace stfem
     // Stage 1 of the dots (v_i, x) and (v_i, y), i < m, and (x, y): the
     // partial sums of workgroup b go to partial[d * RB + b] with d = dx + i,
     // dy + i and dxy (dxy < 0: no (x, y) dot).
-    constexpr int MD = 16; // left vectors per pass
+    typedef double vd2 __attribute__((ext_vector_type(2)));
+    constexpr int  MD = 16; // left vectors per pass
     struct DotList
     {
       const double *v[MD];
@@ -162,7 +191,9 @@ namespace stfem
               for (int i = 0; i < MD; ++i)
                 if (i < L.m)
                   {
-                    const double2 ve = reinterpret_cast<const double2 *>(L.v[i])[e];
+                    // basis vectors stream through once per pass
+                    const vd2 ve = __builtin_nontemporal_load(
+                      reinterpret_cast<const vd2 *>(L.v[i]) + e);
                     ax[i] += ve.x * xe.x;
                     ax[i] += ve.y * xe.y;
                     ay[i] += ve.x * ye.x;
@@ -283,13 +314,12 @@ namespace stfem
 #pragma unroll 4
               for (int i = 0; i < L.m; ++i)
                 {
-                  const double2 v = reinterpret_cast<const double2 *>(L.v[i])[e];
+                  const vd2 v = __builtin_nontemporal_load(reinterpret_cast<const vd2 *>(L.v[i]) + e);
                   w.x -= hs[i] * v.x;
                   w.y -= hs[i] * v.y;
                 }
               // streamed out (whole 128-byte lines per wavefront), not kept
               // in the caches that the basis vectors stream through
-              typedef double vd2 __attribute__((ext_vector_type(2)));
               vd2 wv = {w.x, w.y};
               __builtin_nontemporal_store(wv, reinterpret_cast<vd2 *>(x) + e);
               s += w.x * w.x;
@@ -314,6 +344,28 @@ namespace stfem
         d[i] = TD(s[i]);
     }
 
+    // four entries per thread (16- and 32-byte accesses; vectors 32-byte
+    // aligned); the last n % 4 entries go to the first thread
+    template <typename TD, typename TS>
+    __global__ void k_convert4(long n, const TS *__restrict__ s, TD *__restrict__ d)
+    {
+      typedef TS vs_t __attribute__((ext_vector_type(4)));
+      typedef TD vd_t __attribute__((ext_vector_type(4)));
+      const long i  = (long)blockIdx.x * blockDim.x + threadIdx.x;
+      const long nv = n / 4;
+      if (i < nv)
+        {
+          const vs_t v = reinterpret_cast<const vs_t *>(s)[i];
+          vd_t       r;
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            r[e] = TD(v[e]);
+          reinterpret_cast<vd_t *>(d)[i] = r;
+        }
+      if (i == 0)
+        for (long t = nv * 4; t < n; ++t)
+          d[t] = TD(s[t]);
+    }
     template <typename T>
     __global__ void k_b2i(long nn, int nt, const T *__restrict__ b, T *__restrict__ it)
     {
@@ -491,9 +543,12 @@ namespace stfem
       const T *r   = R + row * ld;
       double   acc = 0.0;
       const long nv = (n / VW) * VW;
+      // the matrix streams through once per application (non-temporal
+      // loads); the x chunks stay cached
+#pragma unroll 4
       for (long j = (long)lane * VW; j < nv; j += 64 * VW)
         {
-          const vec_t m  = *reinterpret_cast<const vec_t *>(r + j);
+          const vec_t m  = __builtin_nontemporal_load(reinterpret_cast<const vec_t *>(r + j));
           const vec_t xv = *reinterpret_cast<const vec_t *>(x + j);
 #pragma unroll
           for (int e = 0; e < VW; ++e)
@@ -618,7 +673,11 @@ namespace stfem
   {
     if (n == 0)
       return;
-    k_axpby<T><<<nblk(n), 256, 0, s>>>(n, a, x, b, y);
+    if (reinterpret_cast<uintptr_t>(x) % 16 == 0 && reinterpret_cast<uintptr_t>(y) % 16 == 0)
+      k_axpby_vec<T><<<nblk((n + 16 / sizeof(T) - 1) / (16 / sizeof(T))), 256, 0, s>>>(n, a, x, b,
+                                                                                     y);
+    else
+      k_axpby<T><<<nblk(n), 256, 0, s>>>(n, a, x, b, y);
     STFEM_HIP(hipGetLastError());
   }
 
@@ -644,7 +703,10 @@ namespace stfem
   {
     if (n == 0)
       return;
-    k_convert<TD, TS><<<nblk(n), 256, 0, s>>>(n, src, dst);
+    if (reinterpret_cast<uintptr_t>(src) % 32 == 0 && reinterpret_cast<uintptr_t>(dst) % 32 == 0)
+      k_convert4<TD, TS><<<nblk((n + 3) / 4), 256, 0, s>>>(n, src, dst);
+    else
+      k_convert<TD, TS><<<nblk(n), 256, 0, s>>>(n, src, dst);
     STFEM_HIP(hipGetLastError());
   }
 
