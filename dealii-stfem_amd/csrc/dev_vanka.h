@@ -733,6 +733,10 @@ namespace stfem
               red[tk * RS + i * AP + ta + 16 * m] = o[m][i];
         }
         __syncthreads();
+        // node values of more than 16 bytes (FP64) with NTS: the cell's block
+        // is staged in xs (free since the first product) and written by all
+        // threads as consecutive 16-byte non-temporal stores (whole lines)
+        constexpr bool STAGED = NTS && NT * sizeof(T) > 16 && (NT * sizeof(T)) % 16 == 0;
         if (tid < NL)
           {
             int cx, cy, cz;
@@ -755,7 +759,9 @@ namespace stfem
             // at a * NT; with NTS one 16-byte value per node is stored
             // non-temporally (the lanes of a wavefront write whole lines)
             T *const dst = cells + (size_t)cell * C::CS + (size_t)a * NT;
-            if constexpr (NTS && NT * sizeof(T) == 16)
+            if constexpr (STAGED)
+              store_comps<T, NT>(xs + a * NT, o);
+            else if constexpr (NTS && NT * sizeof(T) == 16)
               {
                 vec_t v;
 #pragma unroll
@@ -765,6 +771,17 @@ namespace stfem
               }
             else
               store_comps<T, NT>(dst, o);
+          }
+        if constexpr (STAGED)
+          {
+            __syncthreads();
+            constexpr int NCH = NL * NT / VW; // 16-byte chunks of the block
+            T *const      blk = cells + (size_t)cell * C::CS;
+            for (int t = tid; t < NCH; t += 256)
+              __builtin_nontemporal_store(*reinterpret_cast<const vec_t *>(xs + t * VW),
+                                          reinterpret_cast<vec_t *>(blk) + t);
+            if constexpr (PIPE)
+              __syncthreads(); // the next cell's residual goes to xs
           }
       };
 

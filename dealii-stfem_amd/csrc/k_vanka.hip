@@ -78,9 +78,9 @@ namespace stfem
                                                   reinterpret_cast<T *>(smem_raw));
     }
     // cell buffers above this size take non-temporal stores of the sweep's
-    // results (below it they stay in the 256 MB last-level cache for the
+    // results (up to it they stay in the 256 MB last-level cache for the
     // node gather that follows)
-    constexpr size_t NT_STORE_BYTES = size_t(128) << 20;
+    constexpr size_t NT_STORE_BYTES = size_t(256) << 20;
     // the same sweep with A x summed from the operator's cell buffer opc
     template <typename T, int NL, int NT>
     __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 4 : 2)
