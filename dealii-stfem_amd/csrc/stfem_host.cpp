@@ -1192,8 +1192,8 @@ namespace stfem
     auto dense_probe = [&](const GridDesc &gin, int nt_in, const GridDesc &gout, int nt_out,
                            int stride, int sn, int sd, int radius, auto &&apply) {
       const long   n_in = gin.n_nodes * nt_in, n_out = gout.n_nodes * nt_out;
-      DBuf<T> D(size_t(n_out) * n_in);
-      DBuf<T>      x(n_in), y(n_out);
+      DBuf<T>    D(size_t(n_out) * n_in);
+      DBuf<T>    x(n_in), y(n_out);
       launch_fill_zero<T>(D.get(), D.size(), s);
       int       c[3] = {0, 0, 0};
       const int cz   = gin.dim > 2 ? std::min(stride, gin.nn[2]) : 1;
@@ -1206,7 +1206,7 @@ namespace stfem
                 launch_probe_vector<T>(gin, nt_in, c, t, stride, x.get(), s);
                 apply(y.get(), x.get());
                 launch_probe_store<T, T>(gin, nt_in, gout, nt_out, c, t, stride, sn, sd, radius,
-                                      y.get(), D.get(), n_out, bad.get(), s);
+                                         y.get(), D.get(), n_out, bad.get(), s);
               }
       return D;
     };
@@ -1223,7 +1223,7 @@ namespace stfem
       return I;
     };
     DBuf<T> Mc;
-    long         nc = 0;
+    long    nc = 0;
     for (int l = 0; l <= lc; ++l)
       {
         const long n = lv_[l].A->m();
@@ -1232,9 +1232,9 @@ namespace stfem
         const GridDesc &gl = Lv.A->grid();
         const int       ntl = Lv.A->n_blocks(), pl = gl.p;
         DBuf<T> A = dense_probe(gl, ntl, gl, ntl, 2 * pl + 1, 1, 1, pl,
-                                     [&](T *y, const T *x) { Lv.A->vmult(y, x); });
+                                [&](T *y, const T *x) { Lv.A->vmult(y, x); });
         DBuf<T> V = dense_probe(gl, ntl, gl, ntl, 2 * pl + 1, 1, 1, pl,
-                                     [&](T *y, const T *x) { Lv.V->vmult(y, x); });
+                                [&](T *y, const T *x) { Lv.V->vmult(y, x); });
         const int sweeps = steps(l) * opt_.smoothing_steps;
         // (Gs, S): s sweeps; G = I - w V A is the one-sweep iteration matrix
         auto sweep_pair = [&](double w) {
@@ -1255,7 +1255,7 @@ namespace stfem
           {
             auto gh = sweep_pair(double(T(lv_[l].omega)));
             DBuf<T> Gb = std::move(gh.first), Sb = std::move(gh.second);
-            bool         have = false;
+            bool    have = false;
             for (int k = sweeps; k > 0; k >>= 1)
               {
                 if (k & 1)
