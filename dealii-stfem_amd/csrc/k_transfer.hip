@@ -44,6 +44,93 @@ namespace stfem
         sweep_node<T, I, KM, NT>(pt, in, out, d, n0, n1, m, cnt, col, w, K, add);
     }
 
+    // sweeps along x (d = 0): the workgroup stages RPW whole input rows in
+    // LDS (coalesced node-vector loads), then each thread computes output
+    // nodes of those rows from LDS; taps summed in the order of sweep_node
+    template <typename T, int KM, int NT, int RPW>
+    __global__ void __launch_bounds__(256)
+      k_sweep_x(long rows, const T *__restrict__ in, T *__restrict__ out, int n0, int m,
+                const int *__restrict__ cnt, const int *__restrict__ col,
+                const T *__restrict__ w, int K, int add)
+    {
+      extern __shared__ __align__(16) unsigned char smem_raw[];
+      T *const   buf = reinterpret_cast<T *>(smem_raw); // RPW x n0 nodes x NT
+      const long r0  = (long)blockIdx.x * RPW;
+      const int  tid = threadIdx.x;
+      for (int e = tid; e < RPW * n0; e += 256)
+        {
+          const int  rr = e / n0, i = e - rr * n0;
+          const long r  = r0 + rr;
+          if (r < rows)
+            {
+              T v[NT];
+              load_comps<T, NT>(in + (r * n0 + i) * NT, v);
+              store_comps<T, NT>(buf + e * NT, v);
+            }
+        }
+      __syncthreads();
+      for (int e = tid; e < RPW * m; e += 256)
+        {
+          const int  rr = e / m, o = e - rr * m;
+          const long r  = r0 + rr;
+          if (r >= rows)
+            continue;
+          const T *const row = buf + rr * n0 * NT;
+          const int      c   = cnt[o];
+          T              s[NT];
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            s[j] = T(0);
+          if constexpr (KM == 0)
+            {
+#pragma unroll 4
+              for (int k = 0; k < c; ++k)
+                {
+                  const T wk = w[o * K + k];
+                  T       x[NT];
+                  load_comps<T, NT>(row + col[o * K + k] * NT, x);
+#pragma unroll
+                  for (int j = 0; j < NT; ++j)
+                    s[j] += wk * x[j];
+                }
+            }
+          else
+            {
+              int cl[KM];
+              T   wl[KM];
+#pragma unroll
+              for (int k = 0; k < KM; ++k)
+                if (k < K)
+                  {
+                    cl[k] = col[o * K + k];
+                    wl[k] = w[o * K + k];
+                  }
+#pragma unroll
+              for (int k = 0; k < KM; ++k)
+                if (k < c)
+                  {
+                    T x[NT];
+                    load_comps<T, NT>(row + cl[k] * NT, x);
+#pragma unroll
+                    for (int j = 0; j < NT; ++j)
+                      s[j] += wl[k] * x[j];
+                  }
+            }
+          T *const dst = out + (r * m + o) * NT;
+          if (add)
+            {
+              T v[NT];
+              load_comps<T, NT>(dst, v);
+#pragma unroll
+              for (int j = 0; j < NT; ++j)
+                s[j] = v[j] + s[j];
+            }
+          store_comps<T, NT>(dst, s);
+        }
+    }
+    constexpr int    SWEEP_X_RPW = 2;          // input rows per workgroup
+    constexpr size_t SWEEP_X_LDS = 48 * 1024;  // largest staged rows
+
     // one thread per node and all its nt (1-4) components where the vectors
     // allow the node's vector accesses; KM as for k_sweep
     template <typename T, typename I>
@@ -54,11 +141,21 @@ namespace stfem
       if (nt < 1 || nt > 4 || reinterpret_cast<uintptr_t>(in) % 16 != 0 ||
           reinterpret_cast<uintptr_t>(out) % 16 != 0)
         return false;
-      const dim3 grid((unsigned)((nodes + 255) / 256));
+      const dim3   grid((unsigned)((nodes + 255) / 256));
+      const size_t xlds = size_t(SWEEP_X_RPW) * n0 * nt * sizeof(T);
       auto go = [&](auto nt_c, auto km_c) {
         constexpr int NT = decltype(nt_c)::value, KM = decltype(km_c)::value;
-        k_sweep_node<T, I, KM, NT><<<grid, dim3(256), 0, s>>>(nodes, in, out, d, n0, n1, m, cnt,
-                                                              col, w, K, add);
+        if (d == 0 && xlds <= SWEEP_X_LDS)
+          {
+            // rows of the output: nodes / m
+            const long rows = long(nodes) / m;
+            k_sweep_x<T, KM, NT, SWEEP_X_RPW>
+              <<<dim3((unsigned)((rows + SWEEP_X_RPW - 1) / SWEEP_X_RPW)), dim3(256), xlds, s>>>(
+                rows, in, out, n0, m, cnt, col, w, K, add);
+          }
+        else
+          k_sweep_node<T, I, KM, NT><<<grid, dim3(256), 0, s>>>(nodes, in, out, d, n0, n1, m, cnt,
+                                                                col, w, K, add);
       };
       auto by_k = [&](auto nt_c) {
         if (K <= 4)
