@@ -419,6 +419,28 @@ def test_transfers_unaligned_vectors(name):
             assert torch.equal(y, ys[1:]), (l, what)
 
 
+@pytest.mark.parametrize("name", ["3d_q3_dg2_factored", "3d_q4_dg3_fp32_factored"])
+def test_gmg_vmult_unaligned_vectors(name):
+    """The V-cycle's precision conversions (FP64 vectors in, level precision
+    inside) use 16-byte accesses on aligned vectors and scalar ones
+    otherwise: FP64 input and output offset by one value give the same
+    bits (factored Vanka: the V-cycle has no atomics, so two applications
+    agree bit for bit)."""
+    spec, o, g = problems(name)
+    rng = np.random.default_rng(43)
+    m = g.n_nodes * g.nt
+    x = torch.tensor(rng.standard_normal(m), dtype=torch.float64, device=DEV)
+    xs = torch.zeros(m + 1, dtype=torch.float64, device=DEV)
+    xs[1:] = x
+    y = torch.zeros(m, dtype=torch.float64, device=DEV)
+    ys = torch.zeros(m + 1, dtype=torch.float64, device=DEV)
+    assert xs[1:].data_ptr() % 16 != 0 and ys[1:].data_ptr() % 16 != 0
+    print(f"[case {name}] unaligned gmg vmult", flush=True)
+    g.gmg_vmult(y, x)
+    g.gmg_vmult(ys[1:], xs[1:])
+    assert torch.equal(y, ys[1:])
+
+
 def test_vanka_sweep_kernel_timing():
     """The sweep kernel of the factored Vanka can be timed alone (selector 6:
     the cell pass without the node gather), and a level whose Vanka has no
